@@ -1,0 +1,221 @@
+// adjoint.hip — adjoint differentiation (Jones & Gacon 2020, "Efficient calculation of
+// gradients in classical simulations of variational quantum algorithms").
+//
+// Statevector: psi = U_N..U_1|0>, lam = H psi, E = Re<psi|lam>.  Going backwards,
+// psi_{k-1} = U_k^dag psi_k and lam_{k-1} = U_k^dag lam_k; for every parameter entering U_k,
+// dE/dtheta += scale * 2 Re <lam_k| dU_k |psi_{k-1}>.  Runs of ops without parameters are
+// uncomputed as one fused program (a few stage sweeps) instead of op by op.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+
+#include "runtime.h"
+
+namespace goqu {
+
+using cd = std::complex<double>;
+
+static int invert_eop(const EOp &e, EOp &inv, std::string &err) {
+    inv = e;
+    switch (e.kind) {
+    case EOP_DENSE: {
+        const int d = 1 << e.k;
+        for (int r = 0; r < d; r++)
+            for (int c = 0; c < d; c++) {
+                inv.m[2 * (r * d + c)] = e.m[2 * (c * d + r)];
+                inv.m[2 * (r * d + c) + 1] = -e.m[2 * (c * d + r) + 1];
+            }
+        return GOQU_OK;
+    }
+    case EOP_DIAG:
+        for (size_t i = 1; i < inv.m.size(); i += 2) inv.m[i] = -inv.m[i];
+        return GOQU_OK;
+    case EOP_ANTI1:  // n0 = p0 a1, n1 = p1 a0  =>  a0 = conj(p1) n1, a1 = conj(p0) n0
+        inv.m = {e.m[2], -e.m[3], e.m[0], -e.m[1]};
+        return GOQU_OK;
+    case EOP_SWAP2:
+        inv.m = {e.m[0], -e.m[1]};
+        return GOQU_OK;
+    default:
+        err = "adjoint differentiation needs unitary ops (reset is not invertible)";
+        return GOQU_ERR_UNSUPPORTED;
+    }
+}
+
+// The op's derivative as a dense k-qubit matrix on its targets (controls stay controls).
+static int deriv_eop(const goqu_op_t &op, const double *dm, EOp &out, std::string &err) {
+    out = EOp();
+    out.kind = EOP_DENSE;
+    int nc = op.kind == GOQU_OP_CONTROLLED ? op.num_controls : 0;
+    const int nt = op.num_qubits - nc;
+    if (nt < 1 || nt > 4) {
+        err = "derivative of an op with more than 4 target qubits";
+        return GOQU_ERR_UNSUPPORTED;
+    }
+    out.k = nt;
+    for (int j = 0; j < nt; j++) out.q[j] = op.qubits[nc + j];
+    for (int j = 0; j < nc; j++) out.cmask |= 1ull << op.qubits[j];
+    const int d = 1 << nt;
+    out.m.assign(dm, dm + 2 * d * d);
+    return GOQU_OK;
+}
+
+static void ipow(int n, double &re, double &im) {
+    switch (((n % 4) + 4) % 4) {
+    case 1: re = 0, im = 1; break;
+    case 2: re = -1, im = 0; break;
+    case 3: re = 0, im = -1; break;
+    default: re = 1, im = 0; break;
+    }
+}
+
+// lam = H psi
+static int apply_hamiltonian(const State &psi, State &lam, int nt, const uint64_t *x, const uint64_t *z,
+                             const int32_t *ny, const double *coeffs, Scratch &scr, hipStream_t s) {
+    int rc0 = fill_basis(lam, ~0ull, s);  // all zeros
+    if (rc0) return rc0;
+    std::map<uint64_t, std::vector<int>> groups;
+    for (int t = 0; t < nt; t++) groups[x[t]].push_back(t);
+    for (auto &g : groups) {
+        std::vector<uint64_t> zs;
+        std::vector<double> cs;
+        for (int t : g.second) {
+            double yr, yi;
+            ipow(ny[t], yr, yi);
+            const double cr = coeffs[2 * t], ci = coeffs[2 * t + 1];
+            zs.push_back(z[t]);
+            cs.push_back(cr * yr - ci * yi);
+            cs.push_back(cr * yi + ci * yr);
+        }
+        int rc = apply_pauli_group(psi, lam, g.first, zs.data(), cs.data(), (int)zs.size(), scr, s);
+        if (rc) return rc;
+    }
+    return GOQU_OK;
+}
+
+int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goqu_deriv_t *derivs, int nd, int np,
+               int nt, const uint64_t *x, const uint64_t *z, const int32_t *ny, const double *coeffs, double *energy,
+               double *grads, Scratch &scr, hipStream_t s) {
+    const uint64_t valid = (1ull << nq_user) - 1;
+    for (int t = 0; t < nt; t++)
+        if ((x[t] & ~valid) || (z[t] & ~valid)) return fail(GOQU_ERR_INVALID_ARG, "Pauli mask beyond the qubit count");
+    std::vector<std::vector<int>> by_op(nops);
+    for (int i = 0; i < nd; i++) {
+        if (derivs[i].op < 0 || derivs[i].op >= nops || derivs[i].param < 0 || derivs[i].param >= np ||
+            !derivs[i].dmatrix)
+            return fail(GOQU_ERR_INVALID_ARG, "derivative entry " + std::to_string(i) + " out of range");
+        by_op[derivs[i].op].push_back(i);
+    }
+    // lower every op on its own (needed for the op-by-op inverse)
+    std::vector<std::vector<EOp>> low(nops);
+    std::vector<EOp> all;
+    std::string err;
+    for (int k = 0; k < nops; k++) {
+        int rc = lower_sv_ops(&ops[k], 1, nq_user, low[k], err);
+        if (rc) return fail(rc, "op " + std::to_string(k) + ": " + err);
+        all.insert(all.end(), low[k].begin(), low[k].end());
+    }
+    int rc = fill_basis(st, 0, s);
+    if (rc) return rc;
+    rc = run_eops(st, all, s);
+    if (rc) return rc;
+
+    State lam, work;
+    auto cleanup = [&]() {
+        if (lam.amps) (void)hipFree(lam.amps);
+        if (work.amps) (void)hipFree(work.amps);
+    };
+    lam.nq = work.nq = st.nq;
+    lam.prec = work.prec = st.prec;
+    lam.elem = work.elem = st.elem;
+    if (hipMalloc(&lam.amps, st.size() * st.elem) != hipSuccess ||
+        hipMalloc(&work.amps, st.size() * st.elem) != hipSuccess) {
+        (void)hipGetLastError();
+        cleanup();
+        return fail(GOQU_ERR_OUT_OF_MEMORY, "adjoint: cannot allocate two extra state vectors");
+    }
+    rc = apply_hamiltonian(st, lam, nt, x, z, ny, coeffs, scr, s);
+    double e2[2];
+    if (!rc) rc = inner_product(st, lam, e2, scr, s);
+    if (!rc) rc = copy_state(work, st, s);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    *energy = e2[0];
+    for (int p = 0; p < np; p++) grads[p] = 0;
+
+    std::vector<EOp> pend_psi, pend_lam;
+    for (int k = nops - 1; k >= 0 && !rc; k--) {
+        std::vector<EOp> inv;
+        for (int j = (int)low[k].size() - 1; j >= 0; j--) {
+            EOp ie;
+            rc = invert_eop(low[k][j], ie, err);
+            if (rc) {
+                cleanup();
+                return fail(rc, err);
+            }
+            inv.push_back(ie);
+        }
+        pend_psi.insert(pend_psi.end(), inv.begin(), inv.end());
+        if (!by_op[k].empty()) {
+            rc = run_eops(work, pend_psi, s);
+            if (!rc) rc = run_eops(lam, pend_lam, s);
+            pend_psi.clear();
+            pend_lam.clear();
+            for (int i : by_op[k]) {
+                if (rc) break;
+                EOp dop;
+                rc = deriv_eop(ops[k], derivs[i].dmatrix, dop, err);
+                if (rc) {
+                    rc = fail(rc, err);
+                    break;
+                }
+                double sres[2];
+                rc = sandwich(lam, work, dop, sres, scr, s);
+                if (!rc) grads[derivs[i].param] += derivs[i].scale * 2.0 * sres[0];
+            }
+        }
+        pend_lam.insert(pend_lam.end(), inv.begin(), inv.end());
+    }
+    cleanup();
+    return rc;
+}
+
+}  // namespace goqu
+
+using namespace goqu;
+
+extern "C" {
+
+int goqu_sv_adjoint(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops, const goqu_deriv_t *derivs,
+                    int32_t num_derivs, int32_t num_params, int32_t num_terms, const uint64_t *xmask,
+                    const uint64_t *zmask, const int32_t *num_y, const double *coeffs, double *energy,
+                    double *grads) {
+    if (!h || num_ops < 0 || num_derivs < 0 || num_params < 0 || num_terms < 0 || !energy ||
+        (num_ops && !ops) || (num_derivs && !derivs) || (num_params && !grads) ||
+        (num_terms && (!xmask || !zmask || !num_y || !coeffs)))
+        return fail(GOQU_ERR_INVALID_ARG, "bad argument");
+    return adjoint_sv(h->st, h->nq_user, ops, num_ops, derivs, num_derivs, num_params, num_terms, xmask, zmask, num_y,
+                      coeffs, energy, grads, h->scr, h->stream);
+}
+
+int goqu_dm_adjoint(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops, const goqu_deriv_t *derivs,
+                    int32_t num_derivs, int32_t num_params, int32_t num_terms, const uint64_t *xmask,
+                    const uint64_t *zmask, const int32_t *num_y, const double *coeffs, int32_t checkpoint_every,
+                    double *energy, double *grads) {
+    (void)h, (void)ops, (void)num_ops, (void)derivs, (void)num_derivs, (void)num_params, (void)num_terms;
+    (void)xmask, (void)zmask, (void)num_y, (void)coeffs, (void)checkpoint_every, (void)energy, (void)grads;
+    return fail(GOQU_ERR_UNSUPPORTED, "goqu_dm_adjoint: not built yet");
+}
+
+int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits, const int32_t *layers,
+                    const int64_t *edge_offsets, const int32_t *edges, const int64_t *param_offsets,
+                    const double *params, double *energies, double *grads) {
+    (void)num_instances, (void)num_qubits, (void)layers, (void)edge_offsets, (void)edges, (void)param_offsets;
+    (void)params, (void)energies, (void)grads;
+    return fail(GOQU_ERR_UNSUPPORTED, "goqu_qaoa_batch: not built yet");
+}
+
+}  // extern "C"

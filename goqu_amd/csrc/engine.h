@@ -1,0 +1,113 @@
+// engine.h — shared host/device structures of the tiled statevector engine.
+//
+// A circuit is lowered to elementary ops (EOp), which the planner groups into stages.
+// A stage is one sweep over the state: every workgroup loads one tile of 2^m amplitudes
+// (the tile qubits T = the L lowest qubits plus up to m-L chosen qubits), runs the
+// stage's instruction list on it in registers (re-distributing amplitudes between
+// threads through LDS when an op needs other qubits in registers), and stores it back.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace goqu {
+
+// Register slots per thread: each thread keeps 2^kRegBits amplitudes in registers.
+constexpr int kRegBits = 4;
+constexpr int kRegAmps = 1 << kRegBits;
+// Smallest internal qubit count: tiny circuits are padded with |0> qubits so that every
+// tile has at least kRegBits register qubits and a few thread qubits.
+constexpr int kMinInternalQubits = 6;
+constexpr int kMaxTileBits = 13;
+
+// ---- elementary ops ------------------------------------------------------------------
+enum EOpKind : int32_t {
+    EOP_DENSE = 0,  // k targets (k <= 4), dense 2^k x 2^k matrix, qubits[0] = MSB
+    EOP_DIAG = 1,   // k qubits anywhere (k <= 4), 2^k diagonal entries, qubits[0] = MSB
+    EOP_ANTI1 = 2,  // 1 target: [a0, a1] <- [p0 * a1, p1 * a0]   (X, Y, CNOT, CY, CCX ...)
+    EOP_SWAP2 = 3,  // 2 targets: |01> <- p * |10>, |10> <- p * |01>  (SWAP, iSWAP, CSWAP)
+    EOP_RESET = 4,  // 1 target: deterministic reset (a0 <- |(a0,a1)|, a1 <- 0)
+};
+
+struct EOp {
+    int32_t kind = EOP_DENSE;
+    int32_t k = 0;        // number of qubits in q[]
+    int32_t q[4] = {0, 0, 0, 0};
+    uint64_t cmask = 0;   // control qubits (all must be 1)
+    std::vector<double> m;  // interleaved complex payload (dense matrix / diag / phases)
+    bool pure_swap = false; // ANTI1 / SWAP2 with unit phases: move only
+    // which of the q[] are targets that must sit in registers (DENSE, ANTI1, SWAP2, RESET)
+    bool needs_regs() const { return kind != EOP_DIAG; }
+    uint64_t touch() const {
+        uint64_t t = cmask;
+        for (int i = 0; i < k; i++) t |= 1ull << q[i];
+        return t;
+    }
+    uint64_t targets() const {
+        if (!needs_regs()) return 0;
+        uint64_t t = 0;
+        for (int i = 0; i < k; i++) t |= 1ull << q[i];
+        return t;
+    }
+};
+
+// ---- device instructions ---------------------------------------------------------------
+enum InstrOp : int32_t {
+    I_DENSE = 0,  // dense on register slot mask `smask`; matrix at pool[mat] in slot order
+    I_DIAG = 1,   // diag on global qubits q[0..k-1]
+    I_ANTI1 = 2,  // slot s[0]
+    I_SWAP2 = 3,  // slots s[0], s[1]
+    I_RESET = 4,  // slot s[0]
+    I_REMAP = 5,  // new layout lay[0..m-1]: lay[0..3] register local bits, rest thread bits
+};
+
+struct alignas(16) Instr {
+    int32_t op;
+    int32_t k;
+    int32_t mat;      // offset (complex elements) into the stage pool
+    int32_t flags;    // bit0: pure swap (unit phases)
+    uint64_t cmask;   // control mask on global qubits
+    uint8_t s[4];     // register slots
+    uint8_t q[4];     // global qubits (I_DIAG), qubits[0] = MSB of the diag index
+    int32_t smask;    // I_DENSE: register slot mask (dense matrix is in slot-bit order)
+    int32_t pad;
+    uint8_t lay[24];  // I_REMAP: new layout, local tile bit per slot/thread bit
+};
+static_assert(sizeof(Instr) == 64, "Instr must stay 64 bytes");
+
+struct StageHeader {
+    int32_t m;            // tile bits
+    int32_t ninstr;
+    int32_t instr_off;    // index of first Instr in the program's instruction array
+    int32_t pool_off;     // complex-element offset of this stage's pool
+    uint8_t T[kMaxTileBits + 3];    // tile qubits, ascending (local bit j <-> global T[j])
+    uint8_t lay0[24];     // initial layout (local bits: 4 register slots then thread bits)
+    uint8_t layN[24];     // final layout
+};
+
+struct Program {
+    int nq = 0;           // internal qubit count
+    std::vector<StageHeader> stages;
+    std::vector<Instr> instrs;
+    std::vector<double> pool;  // interleaved complex
+    int64_t num_eops = 0;
+};
+
+struct PlanConfig {
+    int tile_bits = 12;   // m
+    int low_bits = 3;     // L: lowest qubits in every tile (contiguous runs of 2^L amps)
+};
+
+// Lowering of goqu_op_t records to elementary ops (statevector semantics).  Returns a
+// GOQU_* status and an error message.
+struct goqu_op_view;
+int lower_sv_ops(const void *ops /* goqu_op_t* */, int nops, int nq_user,
+                 std::vector<EOp> &out, std::string &err);
+
+// Lowering for a density matrix of n qubits stored as a 2n-qubit vector.
+int lower_dm_ops(const void *ops, int nops, int n, std::vector<EOp> &out, std::string &err);
+
+// Greedy stage planner + register allocation.
+Program plan_program(const std::vector<EOp> &eops, int nq_internal, const PlanConfig &cfg);
+
+}  // namespace goqu

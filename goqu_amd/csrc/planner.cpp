@@ -1,0 +1,635 @@
+// planner.cpp — lowering of goqu ops to elementary ops, and the stage planner.
+//
+// Lowering follows the reference's dispatch tiers (sim/statevector/sim.go:92-157,
+// kernel2q.go:24-114, kernel3q.go:10-45, kernel_controlled.go:11-50): every tier becomes
+// the elementary op with the same effect on the amplitudes (permutations move
+// amplitudes, diagonal gates multiply, controlled gates act only where all controls are
+// 1).  The planner then groups elementary ops into stages (one HBM sweep each) and, inside
+// a stage, assigns register slots and the LDS re-distributions between them.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+
+#include "../../include/goqu_hip.h"
+#include "engine.h"
+
+namespace goqu {
+
+using cd = std::complex<double>;
+
+static cd at(const double *m, int i) { return cd(m[2 * i], m[2 * i + 1]); }
+
+static void push(std::vector<double> &v, cd c) {
+    v.push_back(c.real());
+    v.push_back(c.imag());
+}
+
+static bool is_diagonal(const double *m, int dim) {
+    for (int r = 0; r < dim; r++)
+        for (int c = 0; c < dim; c++)
+            if (r != c && (m[2 * (r * dim + c)] != 0.0 || m[2 * (r * dim + c) + 1] != 0.0)) return false;
+    return true;
+}
+
+static EOp make_diag(const int *q, int k, uint64_t cmask, const double *m, int dim) {
+    EOp e;
+    e.kind = EOP_DIAG;
+    e.k = k;
+    for (int i = 0; i < k; i++) e.q[i] = q[i];
+    e.cmask = cmask;
+    for (int i = 0; i < dim; i++) push(e.m, at(m, i * dim + i));
+    return e;
+}
+
+static EOp make_dense(const int *q, int k, uint64_t cmask, const double *m, int dim) {
+    EOp e;
+    e.kind = EOP_DENSE;
+    e.k = k;
+    for (int i = 0; i < k; i++) e.q[i] = q[i];
+    e.cmask = cmask;
+    e.m.assign(m, m + 2 * dim * dim);
+    return e;
+}
+
+static EOp make_anti1(int t, uint64_t cmask, cd p0, cd p1) {
+    EOp e;
+    e.kind = EOP_ANTI1;
+    e.k = 1;
+    e.q[0] = t;
+    e.cmask = cmask;
+    push(e.m, p0);
+    push(e.m, p1);
+    e.pure_swap = (p0 == cd(1, 0) && p1 == cd(1, 0));
+    return e;
+}
+
+static EOp make_swap2(int a, int b, uint64_t cmask, cd p) {
+    EOp e;
+    e.kind = EOP_SWAP2;
+    e.k = 2;
+    e.q[0] = a;
+    e.q[1] = b;
+    e.cmask = cmask;
+    push(e.m, p);
+    e.pure_swap = (p == cd(1, 0));
+    return e;
+}
+
+// A matrix acting on k qubits (possibly under controls) -> the cheapest elementary op.
+static void lower_matrix(const int *q, int k, uint64_t cmask, const double *m, std::vector<EOp> &out) {
+    const int dim = 1 << k;
+    if (is_diagonal(m, dim)) {
+        out.push_back(make_diag(q, k, cmask, m, dim));
+        return;
+    }
+    if (k == 1 && m[0] == 0 && m[1] == 0 && m[6] == 0 && m[7] == 0) {
+        out.push_back(make_anti1(q[0], cmask, at(m, 1), at(m, 2)));
+        return;
+    }
+    out.push_back(make_dense(q, k, cmask, m, dim));
+}
+
+int lower_sv_ops(const void *ops_v, int nops, int nq, std::vector<EOp> &out, std::string &err) {
+    const goqu_op_t *ops = static_cast<const goqu_op_t *>(ops_v);
+    out.clear();
+    for (int i = 0; i < nops; i++) {
+        const goqu_op_t &op = ops[i];
+        if (op.kind == GOQU_OP_SKIP) continue;
+        if (op.num_qubits < 1 || op.num_qubits > GOQU_MAX_OP_QUBITS) {
+            err = "op " + std::to_string(i) + ": bad qubit count " + std::to_string(op.num_qubits);
+            return GOQU_ERR_INVALID_ARG;
+        }
+        uint64_t seen = 0;
+        for (int j = 0; j < op.num_qubits; j++) {
+            int qq = op.qubits[j];
+            if (qq < 0 || qq >= nq || (seen >> qq & 1)) {
+                err = "op " + std::to_string(i) + ": qubit " + std::to_string(qq) + " invalid or repeated";
+                return GOQU_ERR_INVALID_ARG;
+            }
+            seen |= 1ull << qq;
+        }
+        const int *q = op.qubits;
+        const double *m = op.matrix;
+        switch (op.kind) {
+        case GOQU_OP_RESET: {
+            EOp e;
+            e.kind = EOP_RESET;
+            e.k = 1;
+            e.q[0] = q[0];
+            out.push_back(e);
+            continue;
+        }
+        case GOQU_OP_CONTROLLED: {  // dispatchControlled, kernel_controlled.go:11-50
+            const int nc = op.num_controls, nt = op.num_qubits - nc;
+            if (nc < 0 || nt < 1) {
+                err = "op " + std::to_string(i) + ": bad control count";
+                return GOQU_ERR_INVALID_ARG;
+            }
+            if (!m) continue;  // opaque inner gate: identity
+            if (nt >= 3 && op.num_qubits > 10) {  // controlled.Matrix() cap, controlled.go:87-89
+                err = "controlled gate with " + std::to_string(op.num_qubits) + " qubits exceeds 10";
+                return GOQU_ERR_UNSUPPORTED;
+            }
+            if (nt > kRegBits) {
+                err = "controlled gate with more than 4 target qubits";
+                return GOQU_ERR_UNSUPPORTED;
+            }
+            uint64_t cm = 0;
+            for (int j = 0; j < nc; j++) cm |= 1ull << q[j];
+            lower_matrix(q + nc, nt, cm, m, out);
+            continue;
+        }
+        case GOQU_OP_GATE:
+            break;
+        default:
+            err = "op " + std::to_string(i) + ": kind " + std::to_string(op.kind) + " not valid for a statevector";
+            return GOQU_ERR_INVALID_ARG;
+        }
+        const int k = op.num_qubits;
+        if (k == 1) {
+            if (!m) {
+                err = "gate has no matrix representation";  // sim.go:139-142
+                return GOQU_ERR_NO_MATRIX;
+            }
+            lower_matrix(q, 1, 0, m, out);
+            continue;
+        }
+        if (k == 2) {  // dispatchGate2 tiers, kernel2q.go:24-114
+            const uint64_t c0 = 1ull << q[0];
+            switch (op.gate_id) {
+            case GOQU_GATE_CNOT:
+                out.push_back(make_anti1(q[1], c0, 1.0, 1.0));
+                continue;
+            case GOQU_GATE_CZ: {  // diag(1,-1) on q1 under control q0
+                const double z[8] = {1, 0, 0, 0, 0, 0, -1, 0};
+                out.push_back(make_diag(q + 1, 1, c0, z, 2));
+                continue;
+            }
+            case GOQU_GATE_SWAP:
+                out.push_back(make_swap2(q[0], q[1], 0, 1.0));
+                continue;
+            case GOQU_GATE_CY:  // |10> <- -i a3, |11> <- i a2
+                out.push_back(make_anti1(q[1], c0, cd(0, -1), cd(0, 1)));
+                continue;
+            case GOQU_GATE_ISWAP:
+                out.push_back(make_swap2(q[0], q[1], 0, cd(0, 1)));
+                continue;
+            case GOQU_GATE_SYCAMORE: {
+                cd d11 = m ? at(m, 15) : std::exp(cd(0, -M_PI / 6));
+                out.push_back(make_swap2(q[0], q[1], 0, cd(0, -1)));
+                EOp e;
+                e.kind = EOP_DIAG;
+                e.k = 1;
+                e.q[0] = q[1];
+                e.cmask = c0;
+                push(e.m, 1.0);
+                push(e.m, d11);
+                out.push_back(e);
+                continue;
+            }
+            default:
+                break;
+            }
+            if (!m) continue;  // opaque 2q gate: identity (kernel2q.go:106-108)
+            if (op.gate_id == GOQU_GATE_CTRL_DIAG) {
+                EOp e;
+                e.kind = EOP_DIAG;
+                e.k = 1;
+                e.q[0] = q[1];
+                e.cmask = c0;
+                push(e.m, at(m, 10));
+                push(e.m, at(m, 15));
+                out.push_back(e);
+                continue;
+            }
+            if (op.gate_id == GOQU_GATE_DIAG2) {
+                EOp e;
+                e.kind = EOP_DIAG;
+                e.k = 2;
+                e.q[0] = q[0];
+                e.q[1] = q[1];
+                push(e.m, at(m, 0));
+                push(e.m, at(m, 5));
+                push(e.m, at(m, 10));
+                push(e.m, at(m, 15));
+                out.push_back(e);
+                continue;
+            }
+            if (op.gate_id == GOQU_GATE_CTRL_U) {
+                const double sub[8] = {m[20], m[21], m[22], m[23], m[28], m[29], m[30], m[31]};
+                lower_matrix(q + 1, 1, c0, sub, out);
+                continue;
+            }
+            lower_matrix(q, 2, 0, m, out);
+            continue;
+        }
+        if (k == 3) {  // dispatchGate3, kernel3q.go:10-45
+            const uint64_t c01 = (1ull << q[0]) | (1ull << q[1]);
+            switch (op.gate_id) {
+            case GOQU_GATE_CCX:
+                out.push_back(make_anti1(q[2], c01, 1.0, 1.0));
+                continue;
+            case GOQU_GATE_CSWAP:
+                out.push_back(make_swap2(q[1], q[2], 1ull << q[0], 1.0));
+                continue;
+            case GOQU_GATE_CCZ: {
+                EOp e;
+                e.kind = EOP_DIAG;
+                e.k = 1;
+                e.q[0] = q[2];
+                e.cmask = c01;
+                push(e.m, 1.0);
+                push(e.m, -1.0);
+                out.push_back(e);
+                continue;
+            }
+            default:
+                break;
+            }
+            if (!m) continue;  // opaque 3q gate: identity (kernel3q.go:38-41)
+            lower_matrix(q, 3, 0, m, out);
+            continue;
+        }
+        err = "unsupported gate size: " + std::to_string(k) + " qubits";  // sim.go:152
+        return GOQU_ERR_UNSUPPORTED;
+    }
+    return GOQU_OK;
+}
+
+// ---- density matrix: rho as a 2n-qubit vector (row bits n..2n-1, column bits 0..n-1) -------------
+// U rho U^dagger = U on the row qubits, element-wise conj(U) on the column qubits
+// (sim/densitymatrix/kernel1q.go:10-67, kernel2q.go:7-110).  A Kraus channel sum_e E rho E^dag
+// is the superoperator sum_e E (x) conj(E) on (row qubits, column qubits)
+// (sim/densitymatrix/sim.go:220-255); reset is the channel {|0><0|, |0><1|} (sim.go:195-217).
+static std::vector<double> conj_of(const double *m, int n) {
+    std::vector<double> r(m, m + 2 * n);
+    for (int i = 0; i < n; i++) r[2 * i + 1] = -r[2 * i + 1];
+    return r;
+}
+
+int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std::string &err) {
+    const goqu_op_t *ops = static_cast<const goqu_op_t *>(ops_v);
+    out.clear();
+    for (int i = 0; i < nops; i++) {
+        const goqu_op_t &op = ops[i];
+        if (op.kind == GOQU_OP_SKIP) continue;
+        const int k = op.num_qubits;
+        if (k < 1 || k > 2) {
+            err = "op " + std::to_string(i) + ": density-matrix ops act on 1 or 2 qubits (decompose first)";
+            return k > 2 ? GOQU_ERR_UNSUPPORTED : GOQU_ERR_INVALID_ARG;
+        }
+        for (int j = 0; j < k; j++)
+            if (op.qubits[j] < 0 || op.qubits[j] >= n || (k == 2 && op.qubits[0] == op.qubits[1])) {
+                err = "op " + std::to_string(i) + ": qubit out of range or repeated";
+                return GOQU_ERR_INVALID_ARG;
+            }
+        const int rq[2] = {op.qubits[0] + n, k == 2 ? op.qubits[1] + n : 0};
+        const int cq[2] = {op.qubits[0], k == 2 ? op.qubits[1] : 0};
+        const int d = 1 << k;
+        if (op.kind == GOQU_OP_RESET) {
+            const double S[32] = {1, 0, 0, 0, 0, 0, 1, 0};  // rho00 <- rho00 + rho11
+            const int q2[2] = {rq[0], cq[0]};
+            out.push_back(make_dense(q2, 2, 0, S, 4));
+            continue;
+        }
+        const bool channel = op.kind == GOQU_OP_CHANNEL && op.num_kraus > 1;
+        if (op.kind != GOQU_OP_GATE && op.kind != GOQU_OP_CHANNEL) {
+            err = "op " + std::to_string(i) + ": kind not valid for a density matrix";
+            return GOQU_ERR_INVALID_ARG;
+        }
+        if (!op.matrix) continue;  // sim.go:147-150
+        if (!channel) {  // unitary gate or single-Kraus channel: two passes
+            lower_matrix(rq, k, 0, op.matrix, out);
+            std::vector<double> cm = conj_of(op.matrix, d * d);
+            lower_matrix(cq, k, 0, cm.data(), out);
+            continue;
+        }
+        // superoperator S[(r',c'),(r,c)] = sum_e E[r',r] conj(E[c',c]); index = r*d + c
+        const int D = d * d;
+        std::vector<cd> S((size_t)D * D, cd(0, 0));
+        for (int e = 0; e < op.num_kraus; e++) {
+            const double *E = op.matrix + (size_t)2 * d * d * e;
+            for (int r1 = 0; r1 < d; r1++)
+                for (int c1 = 0; c1 < d; c1++)
+                    for (int r0 = 0; r0 < d; r0++)
+                        for (int c0 = 0; c0 < d; c0++)
+                            S[(size_t)(r1 * d + c1) * D + (r0 * d + c0)] +=
+                                at(E, r1 * d + r0) * std::conj(at(E, c1 * d + c0));
+        }
+        std::vector<double> Sm(2 * (size_t)D * D);
+        for (size_t j = 0; j < S.size(); j++) {
+            Sm[2 * j] = S[j].real();
+            Sm[2 * j + 1] = S[j].imag();
+        }
+        int q4[4];
+        if (k == 1) {
+            q4[0] = rq[0];
+            q4[1] = cq[0];
+        } else {
+            q4[0] = rq[0];
+            q4[1] = rq[1];
+            q4[2] = cq[0];
+            q4[3] = cq[1];
+        }
+        lower_matrix(q4, 2 * k, 0, Sm.data(), out);
+    }
+    return GOQU_OK;
+}
+
+// ---- peephole fusion of uncontrolled single-qubit ops on the same qubit ------------------
+static void as_matrix2(const EOp &e, cd M[4]) {
+    if (e.kind == EOP_DIAG) {
+        M[0] = at(e.m.data(), 0);
+        M[1] = 0;
+        M[2] = 0;
+        M[3] = at(e.m.data(), 1);
+    } else if (e.kind == EOP_ANTI1) {
+        M[0] = 0;
+        M[1] = at(e.m.data(), 0);
+        M[2] = at(e.m.data(), 1);
+        M[3] = 0;
+    } else {
+        for (int i = 0; i < 4; i++) M[i] = at(e.m.data(), i);
+    }
+}
+
+static bool fusable1(const EOp &e) {
+    return e.cmask == 0 && e.k == 1 && (e.kind == EOP_DENSE || e.kind == EOP_DIAG || e.kind == EOP_ANTI1);
+}
+
+static std::vector<EOp> fuse_single_qubit(const std::vector<EOp> &in) {
+    std::vector<EOp> out;
+    out.reserve(in.size());
+    int last[64];
+    for (int i = 0; i < 64; i++) last[i] = -1;
+    for (const EOp &e : in) {
+        if (fusable1(e)) {
+            int l = last[e.q[0]];
+            if (l >= 0 && fusable1(out[l]) && out[l].q[0] == e.q[0]) {
+                cd A[4], B[4], P[4];
+                as_matrix2(out[l], A);
+                as_matrix2(e, B);
+                P[0] = B[0] * A[0] + B[1] * A[2];
+                P[1] = B[0] * A[1] + B[1] * A[3];
+                P[2] = B[2] * A[0] + B[3] * A[2];
+                P[3] = B[2] * A[1] + B[3] * A[3];
+                double pm[8];
+                for (int i = 0; i < 4; i++) {
+                    pm[2 * i] = P[i].real();
+                    pm[2 * i + 1] = P[i].imag();
+                }
+                std::vector<EOp> tmp;
+                lower_matrix(e.q, 1, 0, pm, tmp);
+                out[l] = tmp[0];
+                continue;
+            }
+        }
+        out.push_back(e);
+        const int idx = (int)out.size() - 1;
+        uint64_t t = e.touch();
+        while (t) {
+            int b = __builtin_ctzll(t);
+            t &= t - 1;
+            last[b] = idx;
+        }
+    }
+    return out;
+}
+
+// ---- stage planning -----------------------------------------------------------------------
+namespace {
+
+struct StageOps {
+    uint64_t T = 0;
+    std::vector<int> ops;
+};
+
+// Register allocation inside one stage.  Emits instructions into prog.
+struct Layout {
+    int m = 0;
+    int lay[24];  // lay[0..3] register local bits; lay[4..m-1] thread local bits
+};
+
+// Thread bits in ascending local order: lanes 0..2 then sit on the lowest tile qubits,
+// which keeps global loads/stores in contiguous runs and LDS accesses bank-spread.
+static Layout make_layout(int m, const int reg[4]) {
+    Layout L;
+    L.m = m;
+    for (int s = 0; s < kRegBits; s++) L.lay[s] = reg[s];
+    int u = kRegBits;
+    for (int b = 0; b < m; b++) {
+        bool in = false;
+        for (int s = 0; s < kRegBits; s++) in |= (reg[s] == b);
+        if (!in) L.lay[u++] = b;
+    }
+    return L;
+}
+
+}  // namespace
+
+Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &cfg) {
+    Program P;
+    P.nq = nq;
+    std::vector<EOp> eops = fuse_single_qubit(eops_in);
+    P.num_eops = (int64_t)eops.size();
+    const int m = std::min(cfg.tile_bits, nq);
+    const int L = std::min(cfg.low_bits, m - kRegBits);
+    const uint64_t all = (nq >= 64) ? ~0ull : ((1ull << nq) - 1);
+    const uint64_t lowmask = (1ull << L) - 1;
+
+    // 1. greedy stage formation
+    std::vector<char> done(eops.size(), 0);
+    std::vector<StageOps> stages;
+    size_t first_open = 0;
+    while (first_open < eops.size()) {
+        StageOps S;
+        S.T = lowmask;
+        uint64_t blocked = 0;
+        for (size_t i = first_open; i < eops.size(); i++) {
+            if (done[i]) continue;
+            const EOp &e = eops[i];
+            const uint64_t touch = e.touch(), tg = e.targets();
+            if (touch & blocked) {
+                blocked |= touch;
+            } else {
+                const uint64_t nt = S.T | tg;
+                if (__builtin_popcountll(nt) <= m) {
+                    S.T = nt;
+                    S.ops.push_back((int)i);
+                    done[i] = 1;
+                } else {
+                    blocked |= touch;
+                }
+            }
+            if ((blocked & all) == all) break;
+        }
+        // fill the tile up to m qubits with the lowest unused qubits (longer contiguous runs)
+        for (int b = 0; b < nq && __builtin_popcountll(S.T) < m; b++) S.T |= 1ull << b;
+        stages.push_back(S);
+        while (first_open < eops.size() && done[first_open]) first_open++;
+    }
+
+    // 2. per stage: local numbering, register allocation, instruction emission
+    for (const StageOps &S : stages) {
+        StageHeader H;
+        std::memset(&H, 0, sizeof(H));
+        H.m = m;
+        H.instr_off = (int32_t)P.instrs.size();
+        H.pool_off = (int32_t)(P.pool.size() / 2);
+        int loc[64];
+        for (int i = 0; i < 64; i++) loc[i] = -1;
+        {
+            int j = 0;
+            for (int b = 0; b < nq; b++)
+                if (S.T >> b & 1) {
+                    H.T[j] = (uint8_t)b;
+                    loc[b] = j++;
+                }
+        }
+        // targets (local bits) needed by each op in order, for the look-ahead
+        std::vector<uint32_t> need(S.ops.size(), 0);
+        for (size_t i = 0; i < S.ops.size(); i++) {
+            uint64_t t = eops[S.ops[i]].targets();
+            uint32_t l = 0;
+            while (t) {
+                int b = __builtin_ctzll(t);
+                t &= t - 1;
+                l |= 1u << loc[b];
+            }
+            need[i] = l;
+        }
+        // choose a register set containing `must`, filled with the bits needed soonest
+        // from position `from`, avoiding the lane bits 0..2 when possible
+        auto choose = [&](uint32_t must, size_t from, int out[4]) {
+            uint32_t set = must;
+            for (size_t i = from; i < S.ops.size() && __builtin_popcount(set) < kRegBits; i++) {
+                uint32_t cand = need[i] & ~set;
+                if (__builtin_popcount(set | cand) <= kRegBits) set |= cand;
+            }
+            for (int b = m - 1; b >= 0 && __builtin_popcount(set) < kRegBits; b--)
+                if (!(set >> b & 1) && b >= std::min(3, m - kRegBits)) set |= 1u << b;
+            for (int b = m - 1; b >= 0 && __builtin_popcount(set) < kRegBits; b--) set |= 1u << b;
+            int s = 0;
+            for (int b = 0; b < m; b++)
+                if (set >> b & 1) out[s++] = b;
+        };
+        // initial registers: look-ahead, but never on the lowest 3 local bits (coalescing)
+        int reg[4];
+        {
+            const uint32_t avoid = (m - kRegBits >= 3) ? 0x7u : 0u;
+            uint32_t set = 0;
+            for (size_t i = 0; i < S.ops.size() && __builtin_popcount(set) < kRegBits; i++) {
+                uint32_t cand = need[i] & ~set & ~avoid;
+                if (__builtin_popcount(set | cand) <= kRegBits) set |= cand;
+            }
+            for (int b = m - 1; b >= 0 && __builtin_popcount(set) < kRegBits; b--)
+                if (!(set >> b & 1) && !(avoid >> b & 1)) set |= 1u << b;
+            int s = 0;
+            for (int b = 0; b < m; b++)
+                if (set >> b & 1) reg[s++] = b;
+        }
+        Layout cur = make_layout(m, reg);
+        for (int i = 0; i < m; i++) H.lay0[i] = (uint8_t)cur.lay[i];
+
+        auto slot_of = [&](int localbit) {
+            for (int s = 0; s < kRegBits; s++)
+                if (cur.lay[s] == localbit) return s;
+            return -1;
+        };
+        auto emit_remap = [&](const int nreg[4]) {
+            cur = make_layout(m, nreg);
+            Instr I;
+            std::memset(&I, 0, sizeof(I));
+            I.op = I_REMAP;
+            for (int i = 0; i < m; i++) I.lay[i] = (uint8_t)cur.lay[i];
+            P.instrs.push_back(I);
+        };
+
+        for (size_t i = 0; i < S.ops.size(); i++) {
+            const EOp &e = eops[S.ops[i]];
+            uint32_t regset = 0;
+            for (int s = 0; s < kRegBits; s++) regset |= 1u << cur.lay[s];
+            if ((need[i] & ~regset) != 0) {
+                int nreg[4];
+                choose(need[i], i, nreg);
+                emit_remap(nreg);
+            }
+            Instr I;
+            std::memset(&I, 0, sizeof(I));
+            I.cmask = e.cmask;
+            I.k = e.k;
+            I.mat = (int32_t)(P.pool.size() / 2);
+            switch (e.kind) {
+            case EOP_DIAG: {
+                I.op = I_DIAG;
+                for (int j = 0; j < e.k; j++) I.q[j] = (uint8_t)e.q[j];
+                P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
+                break;
+            }
+            case EOP_ANTI1:
+            case EOP_RESET: {
+                I.op = e.kind == EOP_ANTI1 ? I_ANTI1 : I_RESET;
+                I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
+                I.flags = e.pure_swap ? 1 : 0;
+                P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
+                break;
+            }
+            case EOP_SWAP2: {
+                I.op = I_SWAP2;
+                I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
+                I.s[1] = (uint8_t)slot_of(loc[e.q[1]]);
+                I.flags = e.pure_swap ? 1 : 0;
+                P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
+                break;
+            }
+            case EOP_DENSE: {
+                I.op = I_DENSE;
+                const int k = e.k, dim = 1 << k;
+                int slot[4], smask = 0;
+                for (int j = 0; j < k; j++) {
+                    slot[j] = slot_of(loc[e.q[j]]);
+                    smask |= 1 << slot[j];
+                }
+                // rank of each gate qubit among the sorted slots: kernel index bit rank[j]
+                int rank[4];
+                for (int j = 0; j < k; j++) {
+                    rank[j] = 0;
+                    for (int t = 0; t < k; t++)
+                        if (slot[t] < slot[j]) rank[j]++;
+                }
+                auto perm = [&](int g) {
+                    int r = 0;
+                    for (int j = 0; j < k; j++)
+                        if (g >> (k - 1 - j) & 1) r |= 1 << rank[j];
+                    return r;
+                };
+                std::vector<double> km(2 * dim * dim);
+                for (int gr = 0; gr < dim; gr++)
+                    for (int gc = 0; gc < dim; gc++) {
+                        const int kr = perm(gr), kc = perm(gc);
+                        km[2 * (kr * dim + kc)] = e.m[2 * (gr * dim + gc)];
+                        km[2 * (kr * dim + kc) + 1] = e.m[2 * (gr * dim + gc) + 1];
+                    }
+                I.smask = smask;
+                P.pool.insert(P.pool.end(), km.begin(), km.end());
+                break;
+            }
+            }
+            P.instrs.push_back(I);
+        }
+        // leave the tile with lanes on the lowest local bits for coalesced stores
+        {
+            bool bad = false;
+            for (int s = 0; s < kRegBits; s++)
+                if (cur.lay[s] < 3 && m - kRegBits >= 3) bad = true;
+            if (bad) emit_remap(reg);
+        }
+        for (int i = 0; i < m; i++) H.layN[i] = (uint8_t)cur.lay[i];
+        H.ninstr = (int32_t)P.instrs.size() - H.instr_off;
+        P.stages.push_back(H);
+    }
+    return P;
+}
+
+}  // namespace goqu

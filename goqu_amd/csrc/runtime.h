@@ -1,0 +1,108 @@
+// runtime.h — handles, error plumbing and kernel launchers shared by the C ABI files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/goqu_hip.h"
+#include "engine.h"
+
+namespace goqu {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define GOQU_HIP(call)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return ::goqu::fail(e_ == hipErrorOutOfMemory ? GOQU_ERR_OUT_OF_MEMORY : GOQU_ERR_DEVICE, \
+                                std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cx {
+    R x, y;
+};
+
+// Device-side copy of a planned program.
+struct DeviceProgram {
+    void *blob = nullptr;  // headers | instrs | pool (pool as R)
+    size_t bytes = 0;
+    std::vector<StageHeader> stages;  // host copy
+    size_t hdr_off = 0, instr_off = 0, pool_off = 0;
+};
+
+// A state vector on the device: 2^nq internal amplitudes of type cx<R>.
+struct State {
+    int nq = 0;        // internal qubits (>= kMinInternalQubits)
+    int prec = GOQU_C128;
+    void *amps = nullptr;
+    size_t elem = 16;  // bytes per amplitude
+    uint64_t size() const { return 1ull << nq; }
+};
+
+struct Scratch {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    ~Scratch();
+};
+
+// ---- launchers (stream-ordered) -------------------------------------------------------------
+PlanConfig default_plan_config(int prec, int nq);
+int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp);
+int run_program(State &st, const DeviceProgram &dp, hipStream_t s);
+int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s);  // plan + upload + run
+
+int fill_basis(State &st, uint64_t index, hipStream_t s);  // |index>
+int copy_state(State &dst, const State &src, hipStream_t s);
+
+// Pauli expectations: sums S_t = sum_i conj(psi[i^x]) (-1)^{|i&z_t|} psi[i] (no coefficient,
+// no i^ny), all terms sharing one x mask.  out holds 2*nt doubles.
+int pauli_sums(State &st, uint64_t x, const uint64_t *z, int nt, double *out, Scratch &scr, hipStream_t s);
+// Re/Im of <a|b> in FP64.
+int inner_product(const State &a, const State &b, double out[2], Scratch &scr, hipStream_t s);
+// sum over amplitudes with bit q == 0 of |a|^2 (q < 0: all)
+int prob_zero(State &st, int q, double *out, Scratch &scr, hipStream_t s);
+int collapse(State &st, int q, int outcome, double scale, hipStream_t s);
+int sample_state(State &st, uint64_t nvalid, const double *u, int64_t shots, uint64_t *out, Scratch &scr,
+                 hipStream_t s);
+int sample_probs(const double *dprobs, uint64_t n, const double *u, int64_t shots, uint64_t *out, Scratch &scr,
+                 hipStream_t s);
+// lam += coeff * P psi over a group of terms sharing x: lam[i] += sum_t c_t s_t(i^x) psi[i^x]
+int apply_pauli_group(const State &psi, State &lam, uint64_t x, const uint64_t *z, const double *c, int nt,
+                      Scratch &scr, hipStream_t s);
+// <lam| (dM on targets, under cmask) |psi>, dM dense k-qubit (k<=4) in gate order
+int sandwich(const State &lam, const State &psi, const EOp &dop, double out[2], Scratch &scr, hipStream_t s);
+int scale_state(State &st, double re, double im, hipStream_t s);
+int axpy_state(State &y, const State &x, double re, double im, hipStream_t s);  // y += c x
+
+// density-matrix helpers (rho stored as a 2n-qubit vector, row bits high)
+int dm_diag(const State &rho, int n, double *dprobs_out, hipStream_t s);
+int dm_readout(double *dprobs, double *tmp, int n, int q, double p01, double p10, hipStream_t s);
+int dm_pauli(const State &rho, int n, uint64_t x, uint64_t z, double out[2], Scratch &scr, hipStream_t s);
+int dm_purity(const State &rho, int n, double *out, Scratch &scr, hipStream_t s);
+int dm_fidelity(const State &rho, int n, const void *dpsi, double *out, Scratch &scr, hipStream_t s);
+
+}  // namespace goqu
+
+// Opaque handle types of the C ABI.
+struct goqu_sv {
+    goqu::State st;
+    int nq_user = 0;
+    hipStream_t stream = nullptr;
+    goqu::Scratch scr;
+};
+
+struct goqu_dm {
+    goqu::State st;  // 2n-qubit vector, complex128
+    int n = 0;
+    hipStream_t stream = nullptr;
+    goqu::Scratch scr;
+    double *dprobs = nullptr, *dtmp = nullptr;
+};

@@ -1,0 +1,472 @@
+// sv_stage.hip — the tiled gate-application kernel for gfx950.
+//
+// One launch = one stage = one HBM sweep.  A workgroup of 2^(m-4) threads owns a tile of
+// 2^m amplitudes: the tile's qubits are T[0..m-1] (the lowest L qubits plus the stage's
+// chosen qubits); the remaining n-m qubit values are fixed per tile.  Each thread keeps 16
+// amplitudes in registers: register slot s <-> tile-local bit lay[s], thread-id bit u <->
+// tile-local bit lay[4+u].  Ops whose targets are register slots run entirely in
+// registers; controls and diagonal gates may sit on any qubit (they only read index bits).
+// I_REMAP re-distributes the tile through LDS so that other qubits become register slots.
+// Global loads/stores use a layout whose lanes 0..2 walk the tile's lowest qubits, so a
+// wave touches contiguous runs of >= 8 amplitudes (128 B at complex128).
+#include <atomic>
+#include <mutex>
+
+#include "runtime.h"
+
+namespace goqu {
+
+__host__ __device__ constexpr int dep4(int v, int mask) {
+    int r = 0, b = 0;
+    for (int i = 0; i < 4; i++)
+        if (mask >> i & 1) {
+            if (v >> b & 1) r |= 1 << i;
+            b++;
+        }
+    return r;
+}
+__host__ __device__ constexpr int popc4(int m) { return (m & 1) + (m >> 1 & 1) + (m >> 2 & 1) + (m >> 3 & 1); }
+
+// XOR swizzle of LDS element indices: lanes that differ in any three tile bits whose
+// positions are distinct mod 3 (mod 4 at complex64) hit distinct 16-byte bank groups.
+template <typename R>
+__device__ __forceinline__ uint32_t swz(uint32_t l) {
+    if constexpr (sizeof(R) == 8) {
+        uint32_t h = l >> 3;
+        return l ^ ((h ^ (h >> 3) ^ (h >> 6) ^ (h >> 9)) & 7u);
+    } else {
+        uint32_t h = l >> 4;
+        return l ^ ((h ^ (h >> 4) ^ (h >> 8)) & 15u);
+    }
+}
+
+struct Lay {
+    uint64_t tg;      // global index bits contributed by tile base + thread id
+    uint32_t tl;      // tile-local bits contributed by thread id
+    uint64_t rq[4];   // global bit of register slot s
+    uint32_t rl[4];   // tile-local bit of register slot s
+};
+
+__device__ __forceinline__ void set_layout(Lay &L, const uint8_t *__restrict__ lay, const uint8_t *__restrict__ T,
+                                           int m, uint32_t tid, uint64_t base) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int lb = lay[s];
+        L.rl[s] = 1u << lb;
+        L.rq[s] = 1ull << T[lb];
+    }
+    uint64_t tg = base;
+    uint32_t tl = 0;
+    for (int u = 0; u < m - 4; u++) {
+        const int lb = lay[4 + u];
+        const uint64_t gb = 1ull << T[lb];
+        if (tid >> u & 1) {
+            tl |= 1u << lb;
+            tg |= gb;
+        }
+    }
+    L.tg = tg;
+    L.tl = tl;
+}
+
+template <int r>
+__device__ __forceinline__ uint64_t rgof(const Lay &L) {
+    return ((r & 1) ? L.rq[0] : 0ull) | ((r & 2) ? L.rq[1] : 0ull) | ((r & 4) ? L.rq[2] : 0ull) |
+           ((r & 8) ? L.rq[3] : 0ull);
+}
+template <int r>
+__device__ __forceinline__ uint32_t rlof(const Lay &L) {
+    return ((r & 1) ? L.rl[0] : 0u) | ((r & 2) ? L.rl[1] : 0u) | ((r & 4) ? L.rl[2] : 0u) | ((r & 8) ? L.rl[3] : 0u);
+}
+
+// compile-time loop helper
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ cx<R> cmulp(cx<R> a, R br, R bi) {
+    return {a.x * br - a.y * bi, a.x * bi + a.y * br};
+}
+
+template <int MASK, typename R>
+__device__ __forceinline__ void op_dense(cx<R> (&a)[16], const R *__restrict__ M, uint64_t cm, const Lay &L) {
+    constexpr int K = popc4(MASK);
+    constexpr int D = 1 << K;
+    constexpr int OUTER = 16 >> K;
+    constexpr int OM = 15 & ~MASK;
+    static_for<0, OUTER>([&](auto oc) __attribute__((always_inline)) {
+        constexpr int base = dep4(decltype(oc)::value, OM);
+        bool ok = true;
+        if (cm) ok = ((L.tg | rgof<base>(L)) & cm) == cm;
+        cx<R> x[D];
+        static_for<0, D>([&](auto jc) __attribute__((always_inline)) { x[decltype(jc)::value] = a[base | dep4(decltype(jc)::value, MASK)]; });
+        static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
+            constexpr int i = decltype(ic)::value;
+            R re = 0, im = 0;
+            static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int j = decltype(jc)::value;
+                const R mr = M[2 * (i * D + j)], mi = M[2 * (i * D + j) + 1];
+                re = fma(mr, x[j].x, re);
+                re = fma(-mi, x[j].y, re);
+                im = fma(mr, x[j].y, im);
+                im = fma(mi, x[j].x, im);
+            });
+            constexpr int dst = base | dep4(i, MASK);
+            if (ok) a[dst] = cx<R>{re, im};
+        });
+    });
+}
+
+template <int S, typename R>
+__device__ __forceinline__ void op_anti1(cx<R> (&a)[16], R p0r, R p0i, R p1r, R p1i, bool pure, uint64_t cm,
+                                         const Lay &L) {
+    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (!(r >> S & 1)) {
+            constexpr int r1 = r | (1 << S);
+            bool ok = true;
+            if (cm) ok = ((L.tg | rgof<r>(L)) & cm) == cm;
+            cx<R> a0 = a[r], a1 = a[r1];
+            cx<R> n0 = pure ? a1 : cmulp(a1, p0r, p0i);
+            cx<R> n1 = pure ? a0 : cmulp(a0, p1r, p1i);
+            if (ok) {
+                a[r] = n0;
+                a[r1] = n1;
+            }
+        }
+    });
+}
+
+template <int S0, int S1, typename R>
+__device__ __forceinline__ void op_swap2(cx<R> (&a)[16], R pr, R pi, bool pure, uint64_t cm, const Lay &L) {
+    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (!(r >> S0 & 1) && !(r >> S1 & 1)) {
+            constexpr int i01 = r | (1 << S1), i10 = r | (1 << S0);
+            bool ok = true;
+            if (cm) ok = ((L.tg | rgof<r>(L)) & cm) == cm;
+            cx<R> a01 = a[i01], a10 = a[i10];
+            cx<R> n01 = pure ? a10 : cmulp(a10, pr, pi);
+            cx<R> n10 = pure ? a01 : cmulp(a01, pr, pi);
+            if (ok) {
+                a[i01] = n01;
+                a[i10] = n10;
+            }
+        }
+    });
+}
+
+template <int S, typename R>
+__device__ __forceinline__ void op_reset(cx<R> (&a)[16]) {
+    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (!(r >> S & 1)) {
+            constexpr int r1 = r | (1 << S);
+            cx<R> a0 = a[r], a1 = a[r1];
+            R nrm = sqrt(a0.x * a0.x + a0.y * a0.y + a1.x * a1.x + a1.y * a1.y);
+            a[r] = nrm > R(1e-15) ? cx<R>{nrm, R(0)} : cx<R>{R(0), R(0)};
+            a[r1] = cx<R>{R(0), R(0)};
+        }
+    });
+}
+
+template <typename R>
+__device__ __forceinline__ void op_diag(cx<R> (&a)[16], const Instr &I, const R *__restrict__ D, const Lay &L) {
+    const int k = I.k;
+    const uint64_t cm = I.cmask;
+    const int q0 = I.q[0], q1 = I.q[1], q2 = I.q[2], q3 = I.q[3];
+    if (k <= 2) {
+        const R d0r = D[0], d0i = D[1], d1r = D[2], d1i = D[3];
+        const R d2r = k == 2 ? D[4] : R(0), d2i = k == 2 ? D[5] : R(0);
+        const R d3r = k == 2 ? D[6] : R(0), d3i = k == 2 ? D[7] : R(0);
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            const uint64_t g = L.tg | rgof<r>(L);
+            const bool ok = (g & cm) == cm;
+            // bit-wise selection (a select chain on an index becomes a stack lookup table)
+            const bool bhi = (g >> q0) & 1;
+            const bool blo = (g >> q1) & 1;
+            R dr, di;
+            if (k == 2) {
+                const R lr = blo ? d1r : d0r, li = blo ? d1i : d0i;
+                const R hr = blo ? d3r : d2r, hi = blo ? d3i : d2i;
+                dr = bhi ? hr : lr;
+                di = bhi ? hi : li;
+            } else {
+                dr = bhi ? d1r : d0r;
+                di = bhi ? d1i : d0i;
+            }
+            cx<R> v = cmulp(a[r], dr, di);
+            if (ok) a[r] = v;
+        });
+    } else {
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            const uint64_t g = L.tg | rgof<r>(L);
+            const bool ok = (g & cm) == cm;
+            int idx = (int)((g >> q0) & 1);
+            idx = (idx << 1) | (int)((g >> q1) & 1);
+            idx = (idx << 1) | (int)((g >> q2) & 1);
+            if (k == 4) idx = (idx << 1) | (int)((g >> q3) & 1);
+            const R dr = D[2 * idx], di = D[2 * idx + 1];
+            cx<R> v = cmulp(a[r], dr, di);
+            if (ok) a[r] = v;
+        });
+    }
+}
+
+#define DENSE_CASE(MK) \
+    case MK:           \
+        op_dense<MK>(a, M, I.cmask, L); \
+        break;
+
+template <typename R>
+__global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+                                               const Instr *__restrict__ prog, const R *__restrict__ pool,
+                                               uint64_t ntiles) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<R> *lds = reinterpret_cast<cx<R> *>(smem_raw);
+    const int m = hdr->m;
+    const int ninstr = hdr->ninstr;
+    const uint32_t tid = threadIdx.x;
+    const Instr *__restrict__ P = prog + hdr->instr_off;
+
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        // deposit the tile index into the qubits outside the tile (T ascending)
+        uint64_t base = tile;
+        for (int j = 0; j < m; j++) {
+            const int t = hdr->T[j];
+            const uint64_t lo = base & ((1ull << t) - 1);
+            base = ((base >> t) << (t + 1)) | lo;
+        }
+        Lay L;
+        set_layout(L, hdr->lay0, hdr->T, m, tid, base);
+        cx<R> a[16];
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            a[r] = st[L.tg | rgof<r>(L)];
+        });
+
+        for (int pc = 0; pc < ninstr; pc++) {
+            const Instr &I = P[pc];
+            const int op = I.op;
+            if (op == I_REMAP) {
+                __syncthreads();
+                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                    constexpr int r = decltype(rc)::value;
+                    lds[swz<R>(L.tl | rlof<r>(L))] = a[r];
+                });
+                __syncthreads();
+                set_layout(L, I.lay, hdr->T, m, tid, base);
+                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                    constexpr int r = decltype(rc)::value;
+                    a[r] = lds[swz<R>(L.tl | rlof<r>(L))];
+                });
+            } else if (op == I_DENSE) {
+                const R *__restrict__ M = pool + 2 * I.mat;
+                switch (I.smask) {
+                    DENSE_CASE(1)
+                    DENSE_CASE(2)
+                    DENSE_CASE(3)
+                    DENSE_CASE(4)
+                    DENSE_CASE(5)
+                    DENSE_CASE(6)
+                    DENSE_CASE(7)
+                    DENSE_CASE(8)
+                    DENSE_CASE(9)
+                    DENSE_CASE(10)
+                    DENSE_CASE(11)
+                    DENSE_CASE(12)
+                    DENSE_CASE(13)
+                    DENSE_CASE(14)
+                    DENSE_CASE(15)
+                default:
+                    break;
+                }
+            } else if (op == I_DIAG) {
+                op_diag<R>(a, I, pool + 2 * I.mat, L);
+            } else if (op == I_ANTI1) {
+                const R *__restrict__ p = pool + 2 * I.mat;
+                const bool pure = I.flags & 1;
+                const R p0r = p[0], p0i = p[1], p1r = p[2], p1i = p[3];
+                switch (I.s[0]) {
+                case 0: op_anti1<0>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
+                case 1: op_anti1<1>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
+                case 2: op_anti1<2>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
+                default: op_anti1<3>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
+                }
+            } else if (op == I_SWAP2) {
+                const R *__restrict__ p = pool + 2 * I.mat;
+                const bool pure = I.flags & 1;
+                const R pr = p[0], pi = p[1];
+                const int s0 = I.s[0] < I.s[1] ? I.s[0] : I.s[1];
+                const int s1 = I.s[0] < I.s[1] ? I.s[1] : I.s[0];
+                switch (s0 * 4 + s1) {
+                case 1: op_swap2<0, 1>(a, pr, pi, pure, I.cmask, L); break;
+                case 2: op_swap2<0, 2>(a, pr, pi, pure, I.cmask, L); break;
+                case 3: op_swap2<0, 3>(a, pr, pi, pure, I.cmask, L); break;
+                case 6: op_swap2<1, 2>(a, pr, pi, pure, I.cmask, L); break;
+                case 7: op_swap2<1, 3>(a, pr, pi, pure, I.cmask, L); break;
+                default: op_swap2<2, 3>(a, pr, pi, pure, I.cmask, L); break;
+                }
+            } else if (op == I_RESET) {
+                switch (I.s[0]) {
+                case 0: op_reset<0>(a); break;
+                case 1: op_reset<1>(a); break;
+                case 2: op_reset<2>(a); break;
+                default: op_reset<3>(a); break;
+                }
+            }
+        }
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            st[L.tg | rgof<r>(L)] = a[r];
+        });
+        if (tile + gridDim.x < ntiles) __syncthreads();
+    }
+}
+
+// ---- profiling ---------------------------------------------------------------------------------
+static std::atomic<bool> g_profile{false};
+static std::mutex g_prof_mu;
+static double g_prof_ms = 0, g_prof_bytes = 0;
+static int64_t g_prof_launches = 0;
+
+static void profile_add(double ms, int64_t launches, double bytes) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_ms += ms;
+    g_prof_launches += launches;
+    g_prof_bytes += bytes;
+}
+
+// ---------------------------------------------------------------------------------------------
+PlanConfig default_plan_config(int prec, int nq) {
+    PlanConfig c;
+    c.tile_bits = 12;
+    c.low_bits = 3;
+    (void)prec;
+    (void)nq;
+    return c;
+}
+
+int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp) {
+    const size_t rsz = prec == GOQU_C128 ? sizeof(double) : sizeof(float);
+    const size_t nh = P.stages.size(), ni = P.instrs.size(), np = P.pool.size();
+    dp.hdr_off = 0;
+    dp.instr_off = (nh * sizeof(StageHeader) + 255) & ~size_t(255);
+    dp.pool_off = (dp.instr_off + ni * sizeof(Instr) + 255) & ~size_t(255);
+    const size_t bytes = dp.pool_off + np * rsz + 64;
+    std::vector<unsigned char> host(bytes, 0);
+    std::memcpy(host.data(), P.stages.data(), nh * sizeof(StageHeader));
+    std::memcpy(host.data() + dp.instr_off, P.instrs.data(), ni * sizeof(Instr));
+    if (prec == GOQU_C128) {
+        std::memcpy(host.data() + dp.pool_off, P.pool.data(), np * sizeof(double));
+    } else {
+        float *f = reinterpret_cast<float *>(host.data() + dp.pool_off);
+        for (size_t i = 0; i < np; i++) f[i] = (float)P.pool[i];
+    }
+    if (dp.bytes < bytes) {
+        if (dp.blob) GOQU_HIP(hipFree(dp.blob));
+        dp.blob = nullptr;
+        GOQU_HIP(hipMalloc(&dp.blob, bytes));
+        dp.bytes = bytes;
+    }
+    GOQU_HIP(hipMemcpyAsync(dp.blob, host.data(), bytes, hipMemcpyHostToDevice, s));
+    // the host buffer must outlive the async copy
+    GOQU_HIP(hipStreamSynchronize(s));
+    dp.stages = P.stages;
+    return GOQU_OK;
+}
+
+template <typename R>
+static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
+    const unsigned char *blob = static_cast<const unsigned char *>(dp.blob);
+    const StageHeader *hdrs = reinterpret_cast<const StageHeader *>(blob + dp.hdr_off);
+    const Instr *instrs = reinterpret_cast<const Instr *>(blob + dp.instr_off);
+    const R *pool = reinterpret_cast<const R *>(blob + dp.pool_off);
+    static bool attr_done = false;
+    if (!attr_done) {
+        GOQU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<R>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_done = true;
+    }
+    const bool prof = g_profile.load();
+    std::vector<hipEvent_t> ev;
+    for (size_t i = 0; i < dp.stages.size(); i++) {
+        const int m = dp.stages[i].m;
+        if (dp.stages[i].ninstr == 0) continue;
+        const uint64_t ntiles = 1ull << (st.nq - m);
+        const unsigned threads = 1u << (m - kRegBits);
+        const size_t lds = (size_t(1) << m) * sizeof(cx<R>);
+        const uint64_t cap = 1ull << 20;
+        const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
+        if (prof) {
+            ev.resize(ev.size() + 2);
+            GOQU_HIP(hipEventCreate(&ev[ev.size() - 2]));
+            GOQU_HIP(hipEventCreate(&ev[ev.size() - 1]));
+            GOQU_HIP(hipEventRecord(ev[ev.size() - 2], s));
+        }
+        hipLaunchKernelGGL(k_stage<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(st.amps),
+                           hdrs + i, instrs, pool, ntiles);
+        GOQU_HIP(hipGetLastError());
+        if (prof) GOQU_HIP(hipEventRecord(ev.back(), s));
+    }
+    if (prof && !ev.empty()) {
+        GOQU_HIP(hipStreamSynchronize(s));
+        double ms_total = 0;
+        for (size_t i = 0; i < ev.size(); i += 2) {
+            float ms = 0;
+            GOQU_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            ms_total += ms;
+            (void)hipEventDestroy(ev[i]);
+            (void)hipEventDestroy(ev[i + 1]);
+        }
+        profile_add(ms_total, (int64_t)(ev.size() / 2), (double)(ev.size() / 2) * 2.0 * st.size() * sizeof(cx<R>));
+    }
+    return GOQU_OK;
+}
+
+int run_program(State &st, const DeviceProgram &dp, hipStream_t s) {
+    return st.prec == GOQU_C128 ? run_program_t<double>(st, dp, s) : run_program_t<float>(st, dp, s);
+}
+
+int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s) {
+    if (eops.empty()) return GOQU_OK;
+    Program P = plan_program(eops, st.nq, default_plan_config(st.prec, st.nq));
+    DeviceProgram dp;
+    int rc = upload_program(P, st.prec, s, dp);
+    if (rc) return rc;
+    rc = run_program(st, dp, s);
+    hipError_t e = hipStreamSynchronize(s);
+    hipFree(dp.blob);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("stage kernels: ") + hipGetErrorString(e));
+    return GOQU_OK;
+}
+
+}  // namespace goqu
+
+extern "C" {
+int goqu_profile_enable(int32_t on) {
+    goqu::g_profile.store(on != 0);
+    return GOQU_OK;
+}
+int goqu_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(goqu::g_prof_mu);
+    goqu::g_prof_ms = goqu::g_prof_bytes = 0;
+    goqu::g_prof_launches = 0;
+    return GOQU_OK;
+}
+int goqu_profile_read(double *kernel_ms, int64_t *launches, double *bytes) {
+    std::lock_guard<std::mutex> lk(goqu::g_prof_mu);
+    if (kernel_ms) *kernel_ms = goqu::g_prof_ms;
+    if (launches) *launches = goqu::g_prof_launches;
+    if (bytes) *bytes = goqu::g_prof_bytes;
+    return GOQU_OK;
+}
+}

@@ -1,0 +1,260 @@
+"""Statevector simulator on MI355X — drop-in mirror of goqu's sim/statevector.Sim
+(sim/statevector/sim.go:21-383, dynamic.go:14-230) over the C ABI of libgoqu_hip.
+
+Differences from the CPU reference, by design:
+  * New accepts 1..34 qubits (one MI355X holds 33 complex128 qubits); the CPU reference
+    panics above 28 (sim.go:28-30).
+  * Sampling takes its uniforms from a caller-supplied array or numpy Generator, so runs are
+    reproducible; the reference draws from an unseeded PCG per Run (sim.go:51).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from . import gate as G
+from .circuit import Circuit, Operation, MaxControlFlowIterations
+from .pauli import PauliString, PauliSum, masks
+
+
+def format_bitstring(idx: int, n: int) -> str:
+    """formatBitstring (sim.go:306-316): qubit n-1 leftmost."""
+    return format(int(idx), f"0{n}b")
+
+
+def _uniforms(shots: int, uniforms=None, rng=None) -> np.ndarray:
+    if uniforms is not None:
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        if len(u) < shots:
+            raise ValueError("fewer uniforms than shots")
+        return u[:shots]
+    rng = rng if rng is not None else np.random.default_rng()
+    return rng.random(shots)
+
+
+class Sim:
+    """sim.Simulator implementation (sim/sim.go:9-22) backed by a device state vector."""
+
+    def __init__(self, num_qubits: int, precision: int = N.C128):
+        if num_qubits < 1 or num_qubits > 34:
+            raise ValueError(f"statevector: numQubits {num_qubits} out of range [1, 34]")
+        self.num_qubits = num_qubits
+        self.precision = precision
+        h = C.c_void_p()
+        N.check(N.lib().goqu_sv_create(num_qubits, precision, C.byref(h)))
+        self._h = h
+
+    # -- lifecycle --
+    def Close(self) -> None:
+        if self._h:
+            N.lib().goqu_sv_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.Close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.Close()
+
+    def Reset(self) -> None:
+        N.check(N.lib().goqu_sv_reset(self._h))
+
+    # -- evolution --
+    def Evolve(self, c: Circuit) -> None:
+        """Evolve (sim.go:82-85): reset to |0...0> and apply."""
+        self.Reset()
+        self.Apply(c)
+
+    def Apply(self, c: Circuit) -> None:
+        """Apply (sim.go:92-157)."""
+        if c.NumQubits() != self.num_qubits:
+            raise ValueError(f"circuit has {c.NumQubits()} qubits, simulator has {self.num_qubits}")
+        buf = N.OpBuffer()
+        for op in c.Ops():
+            g = op.Gate
+            if g is None or op.ControlFlow is not None:
+                continue
+            if isinstance(g, G.StatePrepGate):
+                self._flush(buf)
+                buf = N.OpBuffer()
+                self._state_prep(g, op.Qubits)
+                continue
+            N.add_sv_gate(buf, g, op.Qubits)
+        self._flush(buf)
+
+    def _flush(self, buf: "N.OpBuffer") -> None:
+        if len(buf):
+            arr = buf.array()
+            N.check(N.lib().goqu_sv_apply(self._h, arr, len(buf)))
+
+    def _state_prep(self, g: "G.StatePrepGate", qubits: Sequence[int]) -> None:
+        # fast path of sim.go:105-120: full-state copy on qubits [0..n-1]
+        if list(qubits) == list(range(self.num_qubits)):
+            self.SetStateVector(g.Amplitudes())
+            return
+        raise NotImplementedError("StatePrep on a qubit subset (Möttönen decomposition) is not supported")
+
+    def ApplyOps(self, buf: "N.OpBuffer") -> None:
+        """Apply pre-marshalled op records."""
+        self._flush(buf)
+
+    # -- state access --
+    def StateVector(self) -> np.ndarray:
+        """StateVector (sim.go:61-65): a host copy."""
+        n = 1 << self.num_qubits
+        out = np.empty(n, dtype=np.complex128)
+        N.check(N.lib().goqu_sv_get_state(self._h, N.dptr(out.view(np.float64)), 0, n))
+        return out
+
+    def StateVectorRange(self, offset: int, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=np.complex128)
+        N.check(N.lib().goqu_sv_get_state(self._h, N.dptr(out.view(np.float64)), offset, count))
+        return out
+
+    def SetStateVector(self, amps, offset: int = 0) -> None:
+        a = np.ascontiguousarray(amps, dtype=np.complex128)
+        N.check(N.lib().goqu_sv_set_state(self._h, N.dptr(a.view(np.float64)), offset, len(a)))
+
+    # -- measurement --
+    def Sample(self, uniforms: np.ndarray) -> np.ndarray:
+        """Inverse-CDF sampling (sim.go:274-298) for the given uniforms."""
+        u = np.ascontiguousarray(uniforms, dtype=np.float64)
+        out = np.empty(len(u), dtype=np.uint64)
+        if len(u):
+            N.check(N.lib().goqu_sv_sample(self._h, N.dptr(u), len(u), N.u64ptr(out)))
+        return out
+
+    def Run(self, c: Circuit, shots: int, uniforms=None, rng=None) -> Dict[str, int]:
+        """Run (sim.go:40-58); dynamic circuits go to RunDynamic."""
+        if c.IsDynamic():
+            return self.RunDynamic(c, shots, rng=rng)
+        self.Evolve(c)
+        counts: Dict[str, int] = {}
+        if shots <= 0:
+            return counts
+        idx = self.Sample(_uniforms(shots, uniforms, rng))
+        vals, cnt = np.unique(idx, return_counts=True)
+        for v, k in zip(vals, cnt):
+            counts[format_bitstring(v, self.num_qubits)] = int(k)
+        return counts
+
+    def MeasureQubit(self, q: int, u: float) -> int:
+        out = C.c_int32(0)
+        N.check(N.lib().goqu_sv_measure_qubit(self._h, q, float(u), C.byref(out)))
+        return out.value
+
+    # -- dynamic circuits (dynamic.go:14-230) --
+    def RunDynamic(self, c: Circuit, shots: int, rng=None) -> Dict[str, int]:
+        if c.NumQubits() != self.num_qubits:
+            raise ValueError(f"circuit has {c.NumQubits()} qubits, simulator has {self.num_qubits}")
+        if shots <= 0:
+            raise ValueError(f"shots must be positive, got {shots}")
+        rng = rng if rng is not None else np.random.default_rng()
+        counts: Dict[str, int] = {}
+        nc = c.NumClbits()
+        for _ in range(shots):
+            self.Reset()
+            clbits = [0] * nc
+            self._exec(c.Ops(), clbits, rng)
+            bs = "".join("1" if clbits[nc - 1 - i] else "0" for i in range(nc))
+            counts[bs] = counts.get(bs, 0) + 1
+        return counts
+
+    def _apply_one(self, g: G.Gate, qubits: Sequence[int]) -> None:
+        buf = N.OpBuffer()
+        N.add_sv_gate(buf, g, qubits)
+        self._flush(buf)
+
+    def _exec(self, ops: List[Operation], clbits: List[int], rng) -> None:
+        for op in ops:
+            cf = op.ControlFlow
+            if cf is not None:
+                self._control_flow(cf, clbits, rng)
+                continue
+            g = op.Gate
+            if g is None:
+                if op.Clbits and op.Qubits:
+                    clbits[op.Clbits[0]] = self.MeasureQubit(op.Qubits[0], rng.random())
+                continue
+            name = g.Name()
+            if name in ("barrier", "delay"):
+                continue
+            if name == "reset":  # measure, then X on outcome 1 (dynamic.go:79-85)
+                if self.MeasureQubit(op.Qubits[0], rng.random()) == 1:
+                    self._apply_one(G.X, op.Qubits)
+                continue
+            if op.Condition is not None and clbits[op.Condition.Clbit] != op.Condition.Value:
+                continue
+            self._apply_one(g, op.Qubits)
+
+    def _control_flow(self, cf, clbits, rng) -> None:
+        if cf.Type == "while":
+            for _ in range(MaxControlFlowIterations):
+                if clbits[cf.Condition.Clbit] != cf.Condition.Value:
+                    break
+                self._exec(cf.Bodies[0], clbits, rng)
+        elif cf.Type == "for":
+            start, end, step = cf.ForRange
+            for _ in range(start, end, step):
+                self._exec(cf.Bodies[0], clbits, rng)
+        elif cf.Type == "if_else":
+            if clbits[cf.Condition.Clbit] == cf.Condition.Value:
+                self._exec(cf.Bodies[0], clbits, rng)
+            elif len(cf.Bodies) > 1:
+                self._exec(cf.Bodies[1], clbits, rng)
+
+    # -- observables --
+    def _check_pauli(self, nq: int, what: str) -> None:
+        if nq != self.num_qubits:
+            raise ValueError(f"statevector: {what} has {nq} qubits, simulator has {self.num_qubits}")
+
+    def ExpectTerms(self, terms: Sequence[PauliString]) -> np.ndarray:
+        """Complex <psi|P_t|psi> (with coefficients) for each term; terms sharing an x mask
+        share one sweep of the state."""
+        x, z, ny, cf = masks(terms)
+        out = np.empty(2 * len(terms), dtype=np.float64)
+        N.check(N.lib().goqu_sv_expect_paulis(self._h, len(terms), N.u64ptr(x), N.u64ptr(z), N.i32ptr(ny),
+                                              N.dptr(cf), N.dptr(out)))
+        return out.view(np.complex128)
+
+    def ExpectPauliString(self, ps: PauliString) -> float:
+        """sim.go:346-353"""
+        self._check_pauli(ps.NumQubits(), "PauliString")
+        return float(self.ExpectTerms([ps])[0].real)
+
+    def ExpectPauliSum(self, ps: PauliSum) -> float:
+        """sim.go:355-363: sum over terms in order."""
+        self._check_pauli(ps.NumQubits(), "PauliSum")
+        vals = self.ExpectTerms(ps.Terms())
+        total = 0j
+        for v in vals:
+            total += v
+        return float(total.real)
+
+    def ExpectationValue(self, qubits: Sequence[int]) -> float:
+        """sim.go:368-383 (rounded to 1e-14)."""
+        mask = 0
+        for q in qubits:
+            mask |= 1 << q
+        out = C.c_double(0)
+        N.check(N.lib().goqu_sv_expectation_z(self._h, mask, C.byref(out)))
+        return round(out.value * 1e14) / 1e14
+
+    def Norm2(self) -> float:
+        out = C.c_double(0)
+        N.check(N.lib().goqu_sv_norm2(self._h, C.byref(out)))
+        return out.value
+
+
+def New(num_qubits: int, precision: int = N.C128) -> Sim:
+    """statevector.New (sim.go:27-35)."""
+    return Sim(num_qubits, precision)

@@ -1,0 +1,135 @@
+"""Seeded circuit generators for the parity tests and the benchmark workloads."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from goqu_amd import gate as G
+from goqu_amd.circuit import NewBuilder, NewVector
+
+TWO_PI = 2 * math.pi
+
+ONE_Q_FIXED = [G.I, G.H, G.X, G.Y, G.Z, G.S, G.Sdg, G.T, G.Tdg, G.SX]
+TWO_Q_FIXED = [G.CNOT, G.CZ, G.SWAP, G.CY, G.ISWAP, G.ECR, G.DCX, G.CH, G.CSX, G.Sycamore]
+THREE_Q_FIXED = [G.CCX, G.CSWAP, G.CCZ]
+
+
+def rand_1q(rng):
+    k = rng.integers(0, 12)
+    a = rng.uniform(0, TWO_PI, 3)
+    return [lambda: ONE_Q_FIXED[rng.integers(len(ONE_Q_FIXED))], lambda: G.RX(a[0]), lambda: G.RY(a[0]),
+            lambda: G.RZ(a[0]), lambda: G.Phase(a[0]), lambda: G.U3(*a), lambda: G.U1(a[0]),
+            lambda: G.U2(a[0], a[1]), lambda: G.Rot(*a), lambda: G.GPI(a[0]), lambda: G.GPI2(a[0]),
+            lambda: G.PhasedXZ(*a)][k]()
+
+
+def rand_2q(rng):
+    k = rng.integers(0, 13)
+    a = rng.uniform(0, TWO_PI, 2)
+    return [lambda: TWO_Q_FIXED[rng.integers(len(TWO_Q_FIXED))], lambda: G.CP(a[0]), lambda: G.CRZ(a[0]),
+            lambda: G.CRX(a[0]), lambda: G.CRY(a[0]), lambda: G.RXX(a[0]), lambda: G.RYY(a[0]),
+            lambda: G.RZZ(a[0]), lambda: G.ZZ(a[0]), lambda: G.MS(a[0], a[1]), lambda: G.FSim(a[0], a[1]),
+            lambda: G.PSwap(a[0]), lambda: G.Controlled(G.H, 1)][k]()
+
+
+def random_circuit(n: int, depth: int, rng, three_q: bool = True, controlled: bool = True):
+    """Every layer: a random 1q gate per qubit and random multi-qubit gates on random qubits."""
+    b = NewBuilder("random", n)
+    for _ in range(depth):
+        for q in range(n):
+            b.Apply(rand_1q(rng), q)
+        for _ in range(max(1, n // 2)):
+            if n >= 2:
+                qs = rng.choice(n, 2, replace=False)
+                b.Apply(rand_2q(rng), int(qs[0]), int(qs[1]))
+            if three_q and n >= 3 and rng.random() < 0.3:
+                qs = rng.choice(n, 3, replace=False)
+                if rng.random() < 0.5:
+                    b.Apply(THREE_Q_FIXED[rng.integers(3)], *map(int, qs))
+                else:
+                    b.Apply(G.Controlled(G.RY(rng.uniform(0, TWO_PI)), 2), *map(int, qs))
+            if controlled and n >= 4 and rng.random() < 0.2:
+                nc = int(rng.integers(3, min(n, 6)))
+                qs = [int(x) for x in rng.choice(n, nc + 1, replace=False)]
+                inner = [G.X, G.Z, G.H, G.RX(rng.uniform(0, TWO_PI)), G.Phase(rng.uniform(0, TWO_PI))][rng.integers(5)]
+                b.Apply(G.Controlled(inner, nc), *qs)
+    return b.Build()
+
+
+def config1_circuit(n: int = 20, layers: int = 20, seed: int = 1):
+    """20 layers of RX/RY/RZ per qubit (angles U[0,2pi)) then CNOT brickwork (even, then odd
+    pairs), MeasureAll."""
+    rng = np.random.default_rng(seed)
+    b = NewBuilder("config1", n)
+    for _ in range(layers):
+        for q in range(n):
+            a = rng.uniform(0, TWO_PI, 3)
+            b.RX(a[0], q).RY(a[1], q).RZ(a[2], q)
+        for q in range(0, n - 1, 2):
+            b.CNOT(q, q + 1)
+        for q in range(1, n - 1, 2):
+            b.CNOT(q, q + 1)
+    b.MeasureAll()
+    return b.Build()
+
+
+def config2_circuit(n: int = 30, layers: int = 40, seed: int = 2):
+    """Depth-40 random circuit: U3 (angles U[0,2pi)) on every qubit, then floor(n/2) disjoint
+    random pairs, at least a quarter of them spanning a low (<10) and a high (>=n-10) qubit;
+    each pair gets CZ, CNOT, SWAP or CP(U[0,2pi))."""
+    rng = np.random.default_rng(seed)
+    b = NewBuilder("config2", n)
+    lo_n = min(10, n // 2)
+    for _ in range(layers):
+        for q in range(n):
+            b.U3(*rng.uniform(0, TWO_PI, 3), q)
+        free = list(range(n))
+        pairs = []
+        n_span = max(1, (n // 2) // 4)
+        for _ in range(n_span):
+            lows = [q for q in free if q < lo_n]
+            highs = [q for q in free if q >= n - lo_n]
+            if not lows or not highs:
+                break
+            a, c = int(rng.choice(lows)), int(rng.choice(highs))
+            free.remove(a)
+            free.remove(c)
+            pairs.append((a, c))
+        rng.shuffle(free)
+        while len(free) >= 2:
+            pairs.append((free.pop(), free.pop()))
+        for a, c in pairs:
+            if rng.random() < 0.5:
+                a, c = c, a
+            k = rng.integers(0, 4)
+            if k == 0:
+                b.CZ(a, c)
+            elif k == 1:
+                b.CNOT(a, c)
+            elif k == 2:
+                b.SWAP(a, c)
+            else:
+                b.CP(rng.uniform(0, TWO_PI), a, c)
+    return b.Build()
+
+
+def efficient_su2(n: int, reps: int):
+    """algorithm/ansatz/efficientsu2.go:19-50 with Circular entanglement
+    (realamplitudes.go:51-70): reps+1 layers of SymRY(theta[2j]) SymRZ(theta[2j+1]) per qubit,
+    CNOT(i,i+1)...CNOT(n-1,0) between layers."""
+    nparams = 2 * n * (reps + 1)
+    th = NewVector("θ", nparams)
+    b = NewBuilder("efficientsu2", n)
+    p = 0
+    for layer in range(reps + 1):
+        for q in range(n):
+            b.SymRY(th.At(p), q)
+            b.SymRZ(th.At(p + 1), q)
+            p += 2
+        if layer < reps:
+            for q in range(n - 1):
+                b.CNOT(q, q + 1)
+            if n > 1:
+                b.CNOT(n - 1, 0)
+    return b.Build(), nparams

@@ -1,0 +1,44 @@
+"""ctypes binding of tests/cpp/stage_emulator.cpp (built with g++ on demand)."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from goqu_amd import _native as N
+
+HERE = Path(__file__).resolve().parent / "cpp"
+SRC = [HERE / "stage_emulator.cpp", HERE.parent.parent / "goqu_amd" / "csrc" / "planner.cpp"]
+LIB = HERE / "_build" / "libstage_emu.so"
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        deps = SRC + [HERE.parent.parent / "goqu_amd" / "csrc" / "engine.h"]
+        if not LIB.exists() or any(p.stat().st_mtime > LIB.stat().st_mtime for p in deps):
+            LIB.parent.mkdir(exist_ok=True)
+            subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", str(LIB), *map(str, SRC)],
+                           check=True)
+        L = C.CDLL(str(LIB))
+        L.emu_apply.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(N.GoquOp), C.c_int, C.c_int,
+                                C.c_int, C.c_int, C.POINTER(C.c_int64)]
+        L.emu_apply.restype = C.c_int
+        L.emu_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def apply(state: np.ndarray, nq_user: int, buf: "N.OpBuffer", tile_bits: int = 12, low_bits: int = 3,
+          dm: bool = False, nq_internal: int | None = None):
+    """Runs the planned program on `state` (complex128, 2^nq_internal entries) in place."""
+    nqi = nq_internal if nq_internal is not None else int(np.log2(len(state)))
+    ns = C.c_int64(0)
+    rc = lib().emu_apply(state.view(np.float64).ctypes.data_as(C.POINTER(C.c_double)), nqi, nq_user, buf.array(),
+                         len(buf), tile_bits, low_bits, int(dm), C.byref(ns))
+    if rc:
+        raise RuntimeError(f"emulator status {rc}: {lib().emu_error().decode()}")
+    return ns.value

@@ -1,0 +1,174 @@
+"""Statevector parity on the GPU: the HIP path against the CPU oracle on the same seeded
+circuits.  complex128: max |delta amplitude| <= 1e-12; complex64 (fp32 arithmetic, fp64
+reductions) against the fp64 oracle: <= 1e-5.  Sampled indices identical for the same
+uniforms."""
+import math
+
+import numpy as np
+import pytest
+
+from circuits import config1_circuit, config2_circuit, random_circuit
+from goqu_amd import _native as N
+from goqu_amd import gate as G
+from goqu_amd import statevector
+from goqu_amd.circuit import NewBuilder
+from goqu_amd.pauli import NewPauliString, NewPauliSum, Parse, X, Y, Z
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL128 = 1e-12
+TOL64 = 1e-5
+
+
+def _check(c, tol=TOL128, precision=N.C128):
+    sim = statevector.New(c.NumQubits(), precision)
+    sim.Evolve(c)
+    got = sim.StateVector()
+    want = O.sv_evolve(c)
+    err = float(np.max(np.abs(got - want)))
+    assert err <= tol, f"max |delta amplitude| = {err:.3e}"
+    return sim, want
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 9, 11, 13, 14, 16])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_random_circuits_all_gates(n, seed):
+    rng = np.random.default_rng(1000 * n + seed)
+    c = random_circuit(n, 6, rng)
+    print(f"random circuit n={n} seed={seed} ops={c.NumOps()}", flush=True)
+    _check(c)
+
+
+@pytest.mark.parametrize("n", [5, 12, 17, 20])
+def test_random_circuits_c64(n):
+    rng = np.random.default_rng(77 + n)
+    c = random_circuit(n, 4, rng)
+    _check(c, TOL64, N.C64)
+
+
+def test_config1_parity_and_sampling():
+    c = config1_circuit(20, 20, seed=11)
+    sim, want = _check(c)
+    u = np.random.default_rng(5).random(1024)
+    got = sim.Sample(u)
+    ref = O.sv_sample(want, u)
+    assert np.array_equal(got, ref)
+    counts = sim.Run(c, 1024, uniforms=u)
+    assert sum(counts.values()) == 1024
+    assert all(len(k) == 20 for k in counts)
+
+
+def test_config2_shape_22q():
+    c = config2_circuit(22, 8, seed=3)
+    _check(c)
+
+
+def test_high_qubit_gates_24q_unitarity():
+    """U then U^dagger returns |0...0> at a size where the oracle is still quick."""
+    n = 24
+    c = config2_circuit(n, 6, seed=4)
+    sim = statevector.New(n)
+    sim.Evolve(c)
+    b = NewBuilder("inv", n)
+    for op in reversed(c.Ops()):
+        if op.Gate is not None:
+            b.Apply(op.Gate.Inverse(), *op.Qubits)
+    sim.Apply(b.Build())
+    sv = sim.StateVectorRange(0, 4)
+    assert abs(sv[0] - 1) < 1e-11
+    assert abs(sim.Norm2() - 1) < 1e-12
+
+
+def test_expect_paulis_vs_oracle():
+    n = 12
+    rng = np.random.default_rng(3)
+    c = random_circuit(n, 4, rng)
+    sim, psi = _check(c)
+    terms = []
+    for t in range(60):
+        ops = {}
+        for q in rng.choice(n, int(rng.integers(0, 6)), replace=False):
+            ops[int(q)] = [X, Y, Z][rng.integers(3)]
+        terms.append(NewPauliString(complex(rng.normal(), rng.normal() * (t % 3 == 0)), ops, n))
+    # terms sharing one x mask (Z strings)
+    for t in range(40):
+        ops = {int(q): Z for q in rng.choice(n, int(rng.integers(1, n)), replace=False)}
+        terms.append(NewPauliString(rng.normal(), ops, n))
+    got = sim.ExpectTerms(terms)
+    for t, ps in enumerate(terms):
+        want = O.sv_expect(psi, ps)
+        assert abs(got[t] - want) < 1e-12, (t, str(ps), got[t], want)
+    H = NewPauliSum(terms)
+    assert abs(sim.ExpectPauliSum(H) - sum(O.sv_expect(psi, p) for p in terms).real) < 1e-11
+
+
+def test_expectation_value_rounding():
+    c = NewBuilder("bell", 2).H(0).CNOT(0, 1).Build()
+    sim = statevector.New(2)
+    sim.Evolve(c)
+    assert sim.ExpectationValue([0, 1]) == 1.0
+    assert sim.ExpectationValue([0]) == 0.0
+
+
+def test_reset_semantics():
+    """sim_test.go:666-746: Evolve's reset concentrates probability into |0>."""
+    s2 = 1 / math.sqrt(2)
+    cases = [
+        (NewBuilder("xr", 1).X(0).Reset(0).Build(), [1, 0]),
+        (NewBuilder("hr", 1).H(0).Reset(0).Build(), [1, 0]),
+        (NewBuilder("br", 2).H(0).CNOT(0, 1).Reset(0).Build(), [s2, 0, s2, 0]),
+        (NewBuilder("r0", 1).Reset(0).Build(), [1, 0]),
+    ]
+    for c, want in cases:
+        sim = statevector.New(c.NumQubits())
+        sim.Evolve(c)
+        assert np.max(np.abs(sim.StateVector() - np.array(want))) < 1e-12
+
+
+def test_measure_qubit_matches_oracle():
+    n = 8
+    rng = np.random.default_rng(9)
+    c = random_circuit(n, 3, rng, three_q=False, controlled=False)
+    sim, psi = _check(c)
+    ref = psi.copy()
+    for q, u in [(3, 0.3), (0, 0.9), (7, 0.5)]:
+        o_gpu = sim.MeasureQubit(q, u)
+        o_cpu = O.sv_measure_qubit(ref, q, u)
+        assert o_gpu == o_cpu
+    assert np.max(np.abs(sim.StateVector() - ref)) < 1e-12
+
+
+def test_errors_are_status_codes():
+    sim = statevector.New(3)
+    buf = N.OpBuffer()
+    buf.add(N.OP_GATE, [0, 1, 2], np.eye(8))
+    buf.add(N.OP_GATE, [0, 1, 5], np.eye(8))  # qubit out of range
+    with pytest.raises(N.GoquError) as e:
+        sim.ApplyOps(buf)
+    assert e.value.code == N.ERR_INVALID_ARG
+    buf = N.OpBuffer()
+    buf.add(N.OP_GATE, [0], None)  # 1q gate without matrix: sim.go:139-142
+    with pytest.raises(N.GoquError) as e:
+        sim.ApplyOps(buf)
+    assert e.value.code == N.ERR_NO_MATRIX
+
+
+def test_bell_counts_statistics():
+    """sim_test.go:135-167: Bell counts in [4000, 6000] of 10000."""
+    c = NewBuilder("bell", 2).H(0).CNOT(0, 1).MeasureAll().Build()
+    sim = statevector.New(2)
+    counts = sim.Run(c, 10000, rng=np.random.default_rng(1))
+    assert set(counts) <= {"00", "11"}
+    assert 4000 <= counts.get("00", 0) <= 6000
+
+
+def test_dynamic_teleportation_like():
+    """RunDynamic with mid-circuit measurement and feed-forward (dynamic.go)."""
+    b = NewBuilder("ff", 2).WithClbits(2)
+    b.X(0).Measure(0, 0).If(0, 1, G.X, 1).Measure(1, 1)
+    c = b.Build()
+    assert c.IsDynamic()
+    sim = statevector.New(2)
+    counts = sim.Run(c, 20, rng=np.random.default_rng(2))
+    assert counts == {"11": 20}
