@@ -183,6 +183,151 @@ int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goq
     return rc;
 }
 
+// ---- density matrix ------------------------------------------------------------------------------
+// E = Tr(H rho_N) = <h|rho_N> with h = vec(H) (h[r*dim + c] = H_rc, H Hermitian).  Going
+// backwards h_{k-1} = S_k^dag h_k, where S_k is op k's superoperator (the conjugate transpose
+// of every lowered op, in reverse order).  A gate U(theta) has S = U_row conj(U)_col and
+// dE/dtheta = 2 Re <h_k| (dU_row conj(U)_col) |rho_{k-1}>.  rho_{k-1} is recomputed from the
+// nearest stored checkpoint (every `every` ops), because channels cannot be uncomputed.
+
+// h[(c ^ x) * dim + c] += coef * i^ny * (-1)^{|c & z|}
+__global__ void k_add_pauli_matrix(cx<double> *__restrict__ h, uint64_t dim, uint64_t x, uint64_t z, double cr,
+                                   double ci) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < dim; c += (uint64_t)gridDim.x * blockDim.x) {
+        const bool neg = __popcll(c & z) & 1;
+        cx<double> &e = h[(c ^ x) * dim + c];
+        e.x += neg ? -cr : cr;
+        e.y += neg ? -ci : ci;
+    }
+}
+
+static int alloc_like(State &s, const State &ref) {
+    s.nq = ref.nq;
+    s.prec = ref.prec;
+    s.elem = ref.elem;
+    if (hipMalloc(&s.amps, ref.size() * ref.elem) != hipSuccess) {
+        (void)hipGetLastError();
+        s.amps = nullptr;
+        return fail(GOQU_ERR_OUT_OF_MEMORY, "adjoint: cannot allocate a density-matrix buffer");
+    }
+    return GOQU_OK;
+}
+
+int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_deriv_t *derivs, int nd, int np,
+               int nt, const uint64_t *x, const uint64_t *z, const int32_t *ny, const double *coeffs, int every,
+               double *energy, double *grads, Scratch &scr, hipStream_t s) {
+    if (every < 1) every = 1;
+    const uint64_t valid = (1ull << n) - 1;
+    for (int t = 0; t < nt; t++)
+        if ((x[t] & ~valid) || (z[t] & ~valid)) return fail(GOQU_ERR_INVALID_ARG, "Pauli mask beyond the qubit count");
+    std::vector<std::vector<int>> by_op(nops);
+    for (int i = 0; i < nd; i++) {
+        const goqu_deriv_t &d = derivs[i];
+        if (d.op < 0 || d.op >= nops || d.param < 0 || d.param >= np || !d.dmatrix)
+            return fail(GOQU_ERR_INVALID_ARG, "derivative entry " + std::to_string(i) + " out of range");
+        const goqu_op_t &op = ops[d.op];
+        if (op.kind != GOQU_OP_GATE || op.num_qubits > 2 || !op.matrix)
+            return fail(GOQU_ERR_UNSUPPORTED, "density-matrix derivatives need 1- or 2-qubit gates with a matrix");
+        by_op[d.op].push_back(i);
+    }
+    std::vector<std::vector<EOp>> low(nops);
+    std::string err;
+    for (int k = 0; k < nops; k++) {
+        int rc = lower_dm_ops(&ops[k], 1, n, low[k], err);
+        if (rc) return fail(rc, "op " + std::to_string(k) + ": " + err);
+    }
+    // forward with checkpoints
+    const int nck = (nops + every - 1) / every;
+    std::vector<State> ck(std::max(nck, 0));
+    State h, work;
+    auto cleanup = [&]() {
+        for (State &c : ck)
+            if (c.amps) (void)hipFree(c.amps);
+        if (h.amps) (void)hipFree(h.amps);
+        if (work.amps) (void)hipFree(work.amps);
+    };
+    int rc = fill_basis(rho, 0, s);
+    for (int c = 0; c < nck && !rc; c++) {
+        rc = alloc_like(ck[c], rho);
+        if (!rc) rc = copy_state(ck[c], rho, s);
+        std::vector<EOp> seg;
+        for (int k = c * every; k < std::min(nops, (c + 1) * every); k++) seg.insert(seg.end(), low[k].begin(), low[k].end());
+        if (!rc) rc = run_eops(rho, seg, s);
+    }
+    if (!rc) rc = alloc_like(h, rho);
+    if (!rc) rc = alloc_like(work, rho);
+    if (!rc) rc = fill_basis(h, ~0ull, s);
+    const uint64_t dim = 1ull << n;
+    for (int t = 0; t < nt && !rc; t++) {
+        double yr, yi;
+        ipow(ny[t], yr, yi);
+        const double cr = coeffs[2 * t] * yr - coeffs[2 * t + 1] * yi;
+        const double ci = coeffs[2 * t] * yi + coeffs[2 * t + 1] * yr;
+        const unsigned grid = (unsigned)std::min<uint64_t>((dim + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_add_pauli_matrix, dim3(grid), dim3(256), 0, s, static_cast<cx<double> *>(h.amps), dim,
+                           x[t], z[t], cr, ci);
+        if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "pauli matrix kernel launch failed");
+        // terms with the same x touch the same entries: keep launches ordered
+    }
+    double e2[2] = {0, 0};
+    if (!rc) rc = inner_product(h, rho, e2, scr, s);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    *energy = e2[0];
+    for (int p = 0; p < np; p++) grads[p] = 0;
+
+    std::vector<EOp> pend;
+    for (int k = nops - 1; k >= 0 && !rc; k--) {
+        if (!by_op[k].empty()) {
+            rc = run_eops(h, pend, s);
+            pend.clear();
+            const int c = k / every;
+            if (!rc) rc = copy_state(work, ck[c], s);
+            std::vector<EOp> seg;
+            for (int j = c * every; j < k; j++) seg.insert(seg.end(), low[j].begin(), low[j].end());
+            if (!rc) rc = run_eops(work, seg, s);
+            const goqu_op_t &op = ops[k];
+            const int kg = op.num_qubits, d = 1 << kg, D = d * d;
+            for (int i : by_op[k]) {
+                if (rc) break;
+                // dop = dU (x) conj(U) on (row qubits, column qubits)
+                EOp dop;
+                dop.kind = EOP_DENSE;
+                dop.k = 2 * kg;
+                for (int j = 0; j < kg; j++) {
+                    dop.q[j] = op.qubits[j] + n;
+                    dop.q[kg + j] = op.qubits[j];
+                }
+                dop.m.assign(2 * (size_t)D * D, 0.0);
+                const double *dU = derivs[i].dmatrix, *U = op.matrix;
+                for (int r1 = 0; r1 < d; r1++)
+                    for (int r0 = 0; r0 < d; r0++)
+                        for (int c1 = 0; c1 < d; c1++)
+                            for (int c0 = 0; c0 < d; c0++) {
+                                const double ar = dU[2 * (r1 * d + r0)], ai = dU[2 * (r1 * d + r0) + 1];
+                                const double br = U[2 * (c1 * d + c0)], bi = -U[2 * (c1 * d + c0) + 1];
+                                const size_t idx = (size_t)(r1 * d + c1) * D + (r0 * d + c0);
+                                dop.m[2 * idx] = ar * br - ai * bi;
+                                dop.m[2 * idx + 1] = ar * bi + ai * br;
+                            }
+                double sres[2];
+                rc = sandwich(h, work, dop, sres, scr, s);
+                if (!rc) grads[derivs[i].param] += derivs[i].scale * 2.0 * sres[0];
+            }
+        }
+        for (int j = (int)low[k].size() - 1; j >= 0 && !rc; j--) {
+            EOp a;
+            rc = invert_eop(low[k][j], a, err);  // conjugate transpose = adjoint superoperator
+            if (rc) rc = fail(rc, err);
+            pend.push_back(a);
+        }
+    }
+    cleanup();
+    return rc;
+}
+
 }  // namespace goqu
 
 using namespace goqu;
@@ -205,17 +350,12 @@ int goqu_dm_adjoint(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops, const g
                     int32_t num_derivs, int32_t num_params, int32_t num_terms, const uint64_t *xmask,
                     const uint64_t *zmask, const int32_t *num_y, const double *coeffs, int32_t checkpoint_every,
                     double *energy, double *grads) {
-    (void)h, (void)ops, (void)num_ops, (void)derivs, (void)num_derivs, (void)num_params, (void)num_terms;
-    (void)xmask, (void)zmask, (void)num_y, (void)coeffs, (void)checkpoint_every, (void)energy, (void)grads;
-    return fail(GOQU_ERR_UNSUPPORTED, "goqu_dm_adjoint: not built yet");
-}
-
-int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits, const int32_t *layers,
-                    const int64_t *edge_offsets, const int32_t *edges, const int64_t *param_offsets,
-                    const double *params, double *energies, double *grads) {
-    (void)num_instances, (void)num_qubits, (void)layers, (void)edge_offsets, (void)edges, (void)param_offsets;
-    (void)params, (void)energies, (void)grads;
-    return fail(GOQU_ERR_UNSUPPORTED, "goqu_qaoa_batch: not built yet");
+    if (!h || num_ops < 0 || num_derivs < 0 || num_params < 0 || num_terms < 0 || !energy ||
+        (num_ops && !ops) || (num_derivs && !derivs) || (num_params && !grads) ||
+        (num_terms && (!xmask || !zmask || !num_y || !coeffs)))
+        return fail(GOQU_ERR_INVALID_ARG, "bad argument");
+    return adjoint_dm(h->st, h->n, ops, num_ops, derivs, num_derivs, num_params, num_terms, xmask, zmask, num_y,
+                      coeffs, checkpoint_every, energy, grads, h->scr, h->stream);
 }
 
 }  // extern "C"

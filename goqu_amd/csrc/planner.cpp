@@ -275,15 +275,54 @@ int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std:
         const goqu_op_t &op = ops[i];
         if (op.kind == GOQU_OP_SKIP) continue;
         const int k = op.num_qubits;
-        if (k < 1 || k > 2) {
-            err = "op " + std::to_string(i) + ": density-matrix ops act on 1 or 2 qubits (decompose first)";
-            return k > 2 ? GOQU_ERR_UNSUPPORTED : GOQU_ERR_INVALID_ARG;
+        if (k < 1 || k > GOQU_MAX_OP_QUBITS) {
+            err = "op " + std::to_string(i) + ": bad qubit count";
+            return GOQU_ERR_INVALID_ARG;
         }
-        for (int j = 0; j < k; j++)
-            if (op.qubits[j] < 0 || op.qubits[j] >= n || (k == 2 && op.qubits[0] == op.qubits[1])) {
-                err = "op " + std::to_string(i) + ": qubit out of range or repeated";
+        {
+            uint64_t seen = 0;
+            for (int j = 0; j < k; j++) {
+                const int qq = op.qubits[j];
+                if (qq < 0 || qq >= n || (seen >> qq & 1)) {
+                    err = "op " + std::to_string(i) + ": qubit out of range or repeated";
+                    return GOQU_ERR_INVALID_ARG;
+                }
+                seen |= 1ull << qq;
+            }
+        }
+        // Gates on 3+ qubits (and controlled gates) act exactly: U on the row qubits and
+        // conj(U) on the column qubits, controls as masks on the row / column bits.
+        if ((op.kind == GOQU_OP_GATE && k > 2) || op.kind == GOQU_OP_CONTROLLED) {
+            const int nc = op.kind == GOQU_OP_CONTROLLED ? op.num_controls : 0, nt = k - nc;
+            if (nc < 0 || nt < 1) {
+                err = "op " + std::to_string(i) + ": bad control count";
                 return GOQU_ERR_INVALID_ARG;
             }
+            if (nt > kRegBits) {
+                err = "op " + std::to_string(i) + ": more than 4 target qubits";
+                return GOQU_ERR_UNSUPPORTED;
+            }
+            if (!op.matrix) continue;
+            uint64_t rc = 0, cc = 0;
+            int rt[4], ct[4];
+            for (int j = 0; j < nc; j++) {
+                rc |= 1ull << (op.qubits[j] + n);
+                cc |= 1ull << op.qubits[j];
+            }
+            for (int j = 0; j < nt; j++) {
+                rt[j] = op.qubits[nc + j] + n;
+                ct[j] = op.qubits[nc + j];
+            }
+            const int d = 1 << nt;
+            lower_matrix(rt, nt, rc, op.matrix, out);
+            std::vector<double> cm = conj_of(op.matrix, d * d);
+            lower_matrix(ct, nt, cc, cm.data(), out);
+            continue;
+        }
+        if (k > 2) {
+            err = "op " + std::to_string(i) + ": channels act on 1 or 2 qubits";
+            return GOQU_ERR_UNSUPPORTED;
+        }
         const int rq[2] = {op.qubits[0] + n, k == 2 ? op.qubits[1] + n : 0};
         const int cq[2] = {op.qubits[0], k == 2 ? op.qubits[1] : 0};
         const int d = 1 << k;

@@ -3,12 +3,12 @@
 
 rho lives on the device as a 2n-qubit vector (row bits high): a gate is U on the row qubits
 and conj(U) on the column qubits, a Kraus channel is its superoperator on both, each applied
-by the tiled engine in fused sweeps.  Noise lookups and the >2-qubit decompositions run on
-the host exactly as in the reference (sim.go:182-188, 321-518).
+by the tiled engine in fused sweeps.  Noise lookups run on the host as in the reference
+(sim.go:182-188).  Gates on 3+ qubits are applied exactly, where the reference decomposes
+controlled gates (sim.go:321-518) and rejects other 3-qubit gates.
 """
 from __future__ import annotations
 
-import cmath
 import ctypes as C
 import math
 from typing import Dict, List, Optional, Sequence
@@ -23,102 +23,58 @@ from .pauli import PauliString, PauliSum, masks
 from .statevector import _uniforms, format_bitstring
 
 
-# ---- >2-qubit decomposition (sim/densitymatrix/sim.go:321-518) --------------------------------------
-def _near_zero(x: float) -> bool:
-    return abs(math.remainder(x, 2 * math.pi)) < 1e-10
+def _add_channel(buf: "N.OpBuffer", ch: NZ.Channel, qubits: Sequence[int]) -> None:
+    if ch.Qubits() not in (1, 2):
+        return  # applyChannel only handles 1q/2q channels (sim.go:228-233)
+    ks = ch.Kraus()
+    buf.add(N.OP_CHANNEL, list(qubits)[: ch.Qubits()], np.concatenate(ks), num_kraus=len(ks))
 
 
-def _mcx(controls: List[int], target: int) -> List[Operation]:
-    n = len(controls)
-    if n == 1:
-        return [Operation(G.CNOT, [controls[0], target])]
-    if n == 2:
-        c0, c1 = controls
-        seq = [(G.H, [target]), (G.CNOT, [c1, target]), (G.Tdg, [target]), (G.CNOT, [c0, target]),
-               (G.T, [target]), (G.CNOT, [c1, target]), (G.Tdg, [target]), (G.CNOT, [c0, target]),
-               (G.T, [c1]), (G.T, [target]), (G.CNOT, [c0, c1]), (G.H, [target]), (G.T, [c0]),
-               (G.Tdg, [c1]), (G.CNOT, [c0, c1])]
-        return [Operation(g, q) for g, q in seq]
-    last, rest = controls[-1], controls[:-1]
-    ops = _c1q(G.SX, rest, target)
-    ops.append(Operation(G.CNOT, [last, target]))
-    ops += _c1q(G.SX.Inverse(), rest, target)
-    ops.append(Operation(G.CNOT, [last, target]))
-    ops += _c1q(G.S, rest, last)
-    return ops
-
-
-def _c1q(u: G.Gate, controls: List[int], target: int) -> List[Operation]:
-    if len(controls) == 1:
-        return [Operation(G.Controlled(u, 1), [controls[0], target])]
-    if u is G.X or u.Name() == "X":
-        return _mcx(controls, target)
-    return _general(u, controls, target)
-
-
-def _general(u: G.Gate, controls: List[int], target: int) -> List[Operation]:
-    m = u.Matrix()
-    det = m[0] * m[3] - m[1] * m[2]
-    phase = cmath.log(det).imag / 2
-    factor = cmath.exp(complex(0, -phase))
-    a, b = m[0] * factor, m[1] * factor
-    beta = 2 * math.acos(min(max(abs(a), 0.0), 1.0))
-    if abs(b) < 1e-10:
-        alpha, gamma = -2 * cmath.phase(a), 0.0
-    elif abs(a) < 1e-10:
-        alpha, beta, gamma = -2 * cmath.phase(-b), math.pi, 0.0
-    else:
-        apg, amg = cmath.phase(a), cmath.phase(-b)
-        alpha, gamma = -(apg + amg), -(apg - amg)
-    ops: List[Operation] = []
-    if not _near_zero(gamma - alpha):
-        ops.append(Operation(G.RZ((gamma - alpha) / 2), [target]))
-    ops += _mcx(controls, target)
-    if not _near_zero(alpha + gamma):
-        ops.append(Operation(G.RZ(-(alpha + gamma) / 2), [target]))
-    if not _near_zero(beta):
-        ops.append(Operation(G.RY(-beta / 2), [target]))
-    ops += _mcx(controls, target)
-    if not _near_zero(beta):
-        ops.append(Operation(G.RY(beta / 2), [target]))
-    if not _near_zero(alpha):
-        ops.append(Operation(G.RZ(alpha), [target]))
-    if not _near_zero(phase):
-        n = len(controls)
-        if n == 1:
-            ops.append(Operation(G.Phase(phase), [controls[0]]))
-        else:
-            ops += _c1q(G.Phase(phase), controls[:-1], controls[-1])
-    return ops
-
-
-def decompose_for_density(op: Operation) -> Optional[List[Operation]]:
-    """decomposeForDensity (sim.go:321-352)."""
+def _marshal_op(buf: "N.OpBuffer", op: Operation, noise: Optional[NZ.NoiseModel]) -> None:
+    """One op of DM Apply (sim.go:87-191) -> op records, with the noise lookups."""
     g = op.Gate
-    applied = g.Decompose(op.Qubits)
-    if applied is not None:
-        out: List[Operation] = []
-        for a in applied:
-            if a.Gate.Qubits() <= 2:
-                out.append(a)
-            else:
-                inner = decompose_for_density(a)
-                if inner is None:
-                    return None
-                out += inner
-        return out
-    if isinstance(g, G.ControlledGate):
-        nc = g.NumControls()
-        controls, targets = op.Qubits[:nc], op.Qubits[nc:]
-        inner = g.Inner()
-        if nc == 1 and inner.Qubits() == 1:
-            return [op]
-        if inner.Qubits() != 1:
-            return None
-        if inner is G.X or inner.Name() == "X":
-            return _mcx(list(controls), targets[0])
-        return _general(inner, list(controls), targets[0])
-    return None
+    if g is None or op.ControlFlow is not None or g.Name() == "barrier":
+        return
+    if g.Name() == "delay":
+        if noise is not None:
+            ch = noise.Lookup("delay", op.Qubits)
+            if ch is not None:
+                _add_channel(buf, ch, op.Qubits)
+        return
+    if g.Name() == "reset":
+        buf.add(N.OP_RESET, op.Qubits[:1])
+        if noise is not None:
+            ch = noise.Lookup("reset", op.Qubits)
+            if ch is not None:
+                _add_channel(buf, ch, op.Qubits)
+        return
+    if isinstance(g, G.StatePrepGate):
+        raise NotImplementedError("StatePrep on a density matrix: use SetDensityMatrix")
+    k = g.Qubits()
+    if k >= 3 and isinstance(g, G.ControlledGate):
+        # exact controlled gate; goqu decomposes it on the host instead (sim.go:321-518),
+        # and its multi-controlled X decomposition for 3+ controls leaves the target unchanged
+        inner = g.Inner().Matrix()
+        if inner is None:
+            return
+        buf.add(N.OP_CONTROLLED, op.Qubits, inner, num_controls=g.NumControls())
+    else:
+        m = g.Matrix()
+        if m is None:
+            return  # sim.go:147-150 (no noise either)
+        buf.add(N.OP_GATE, op.Qubits, m)
+    if noise is not None:
+        ch = noise.Lookup(g.Name(), op.Qubits)
+        if ch is not None:
+            _add_channel(buf, ch, op.Qubits)
+
+
+def marshal(c: Circuit, noise: Optional[NZ.NoiseModel] = None) -> "N.OpBuffer":
+    """The whole circuit as density-matrix op records."""
+    buf = N.OpBuffer()
+    for op in c.Ops():
+        _marshal_op(buf, op, noise)
+    return buf
 
 
 class Sim:
@@ -154,63 +110,10 @@ class Sim:
         self.Reset()
         self.Apply(c)
 
-    # -- op marshalling (sim.go:87-191) --
-    def _add_channel(self, buf: "N.OpBuffer", ch: NZ.Channel, qubits: Sequence[int]) -> None:
-        ks = ch.Kraus()
-        if ch.Qubits() not in (1, 2):
-            return  # applyChannel only handles 1q/2q channels (sim.go:228-233)
-        buf.add(N.OP_CHANNEL, list(qubits)[: ch.Qubits()], np.concatenate(ks), num_kraus=len(ks))
-
-    def _add_gate(self, buf: "N.OpBuffer", g: G.Gate, qubits: Sequence[int]) -> None:
-        m = g.Matrix()
-        if m is None:
-            return
-        buf.add(N.OP_GATE, qubits, m)
-
     def marshal(self, c: Circuit) -> "N.OpBuffer":
         if c.NumQubits() != self.num_qubits:
             raise ValueError(f"circuit has {c.NumQubits()} qubits, simulator has {self.num_qubits}")
-        buf = N.OpBuffer()
-        for op in c.Ops():
-            self._marshal_op(buf, op)
-        return buf
-
-    def _marshal_op(self, buf: "N.OpBuffer", op: Operation) -> None:
-        g = op.Gate
-        if g is None or op.ControlFlow is not None or g.Name() == "barrier":
-            return
-        if g.Name() == "delay":
-            if self.noise is not None:
-                ch = self.noise.Lookup("delay", op.Qubits)
-                if ch is not None:
-                    self._add_channel(buf, ch, op.Qubits)
-            return
-        if g.Name() == "reset":
-            buf.add(N.OP_RESET, op.Qubits[:1])
-            if self.noise is not None:
-                ch = self.noise.Lookup("reset", op.Qubits)
-                if ch is not None:
-                    self._add_channel(buf, ch, op.Qubits)
-            return
-        if isinstance(g, G.StatePrepGate):
-            raise NotImplementedError("StatePrep on a density matrix: use SetDensityMatrix")
-        k = g.Qubits()
-        if k <= 2:
-            if g.Matrix() is None:
-                return  # sim.go:147-150 (no noise either)
-            self._add_gate(buf, g, op.Qubits)
-        else:
-            subs = decompose_for_density(op)
-            if subs is None:
-                raise ValueError(f"densitymatrix: unsupported gate size: {k} qubits")
-            for sub in subs:
-                if sub.Gate.Qubits() > 2:
-                    raise ValueError(f"densitymatrix: decomposition produced {sub.Gate.Qubits()}-qubit gate")
-                self._add_gate(buf, sub.Gate, sub.Qubits)
-        if self.noise is not None:
-            ch = self.noise.Lookup(g.Name(), op.Qubits)
-            if ch is not None:
-                self._add_channel(buf, ch, op.Qubits)
+        return marshal(c, self.noise)
 
     def Apply(self, c: Circuit) -> None:
         buf = self.marshal(c)

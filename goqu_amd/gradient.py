@@ -1,0 +1,250 @@
+"""Gradients — host-side mirror of goqu's algorithm/gradient (gradient.go:25-86) plus the
+adjoint method on the device.
+
+CostFunc, ParameterShift and FiniteDifference follow the reference exactly (they are
+combinators over an objective).  Adjoint computes <H> and every partial derivative with one
+forward and one reverse sweep on the GPU (goqu_sv_adjoint / goqu_dm_adjoint): each
+parameterised op contributes scale * 2 Re <lambda_k| dU_k |psi_{k-1}>, where dU_k is the
+derivative of the gate matrix with respect to its argument and scale the derivative of the
+argument expression with respect to the parameter (chain rule for reused or scaled
+parameters, e.g. QAOA's 2*gamma*coeff).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _native as N
+from . import gate as G
+from .circuit import Bind, Circuit, FreeParameters, SymbolicGate, CONSTRUCTORS
+from .pauli import PauliSum, masks
+
+ObjectiveFunc = Callable[[Sequence[float]], float]
+GradientFunc = Callable[[Sequence[float]], np.ndarray]
+
+_PX = np.array([[0, 1], [1, 0]], dtype=complex)
+_PY = np.array([[0, -1j], [1j, 0]], dtype=complex)
+_PZ = np.diag([1, -1]).astype(complex)
+_GEN = {"RX": _PX, "RY": _PY, "RZ": _PZ, "RXX": np.kron(_PX, _PX), "RYY": np.kron(_PY, _PY),
+        "RZZ": np.kron(_PZ, _PZ), "ZZ": np.kron(_PZ, _PZ)}
+_CGEN = {"CRX": _PX, "CRY": _PY, "CRZ": _PZ}
+
+
+def _mat(name: str, args: Sequence[float]) -> np.ndarray:
+    g = CONSTRUCTORS[name](*args)
+    d = 1 << g.Qubits()
+    return g.Matrix().reshape(d, d)
+
+
+def gate_derivative(name: str, args: Sequence[float], j: int) -> np.ndarray:
+    """d(matrix)/d(args[j]) for the parameterised gate constructors."""
+    M = _mat(name, args)
+    if name in _GEN:  # exp(-i theta/2 G): dM = -i/2 G M
+        return -0.5j * _GEN[name] @ M
+    if name in _CGEN:
+        R = M[2:, 2:]
+        out = np.zeros((4, 4), dtype=complex)
+        out[2:, 2:] = -0.5j * _CGEN[name] @ R
+        return out
+    if name in ("P", "U1", "CP"):
+        out = np.zeros_like(M)
+        out[-1, -1] = 1j * M[-1, -1]
+        return out
+    if name == "GlobalPhase":
+        return 1j * M
+    if name == "U3":
+        th, ph, la = args
+        c, s = math.cos(th / 2), math.sin(th / 2)
+        e = lambda a: complex(math.cos(a), math.sin(a))  # noqa: E731
+        if j == 0:
+            return np.array([[-s / 2, -e(la) * c / 2], [e(ph) * c / 2, -e(ph + la) * s / 2]])
+        if j == 1:
+            return np.array([[0, 0], [1j * e(ph) * s, 1j * e(ph + la) * c]])
+        return np.array([[0, -1j * e(la) * s], [0, 1j * e(ph + la) * c]])
+    # other gates: Richardson-extrapolated central difference (error ~1e-12)
+    def D(h):
+        a1, a0 = list(args), list(args)
+        a1[j] += h
+        a0[j] -= h
+        return (_mat(name, a1) - _mat(name, a0)) / (2 * h)
+
+    h = 1e-3
+    return (4 * D(h / 2) - D(h)) / 3
+
+
+# ---- reference combinators (algorithm/gradient/gradient.go) ------------------------------------
+def CostFunc(circuit: Circuit, hamiltonian: PauliSum, param_names: Sequence[str], sim=None,
+             precision: int = N.C128) -> ObjectiveFunc:
+    """gradient.go:25-43: x -> Bind -> Evolve -> Re<H>.  A reusable simulator may be passed;
+    otherwise each call creates one, as the reference does."""
+    from . import statevector
+
+    names = list(param_names)
+
+    def f(x):
+        bound = Bind(circuit, {n: float(v) for n, v in zip(names, x)})
+        s = sim if sim is not None else statevector.New(bound.NumQubits(), precision)
+        try:
+            s.Evolve(bound)
+            return s.ExpectPauliSum(hamiltonian)
+        finally:
+            if sim is None:
+                s.Close()
+
+    return f
+
+
+def ParameterShift(f: ObjectiveFunc) -> GradientFunc:
+    """gradient.go:49-66: g_i = (f(x + pi/2 e_i) - f(x - pi/2 e_i)) / 2."""
+
+    def g(x):
+        x = np.asarray(x, dtype=float)
+        out = np.zeros(len(x))
+        for i in range(len(x)):
+            xp, xm = x.copy(), x.copy()
+            xp[i] += 0.5 * math.pi
+            xm[i] -= 0.5 * math.pi
+            out[i] = (f(xp) - f(xm)) / 2
+        return out
+
+    return g
+
+
+def FiniteDifference(f: ObjectiveFunc, h: float) -> GradientFunc:
+    """gradient.go:70-86: central difference."""
+
+    def g(x):
+        x = np.asarray(x, dtype=float)
+        out = np.zeros(len(x))
+        for i in range(len(x)):
+            xp, xm = x.copy(), x.copy()
+            xp[i] += h
+            xm[i] -= h
+            out[i] = (f(xp) - f(xm)) / (2 * h)
+        return out
+
+    return g
+
+
+# ---- adjoint -------------------------------------------------------------------------------------
+class AdjointProgram:
+    """Marshals a parameterised circuit for goqu_sv_adjoint / goqu_dm_adjoint."""
+
+    def __init__(self, circuit: Circuit, param_names: Sequence[str], density: bool = False, noise=None):
+        self.circuit = circuit
+        self.names = list(param_names)
+        self.index = {n: i for i, n in enumerate(self.names)}
+        self.density = density
+        self.noise = noise
+
+    def build(self, x: Sequence[float]):
+        b = {n: float(v) for n, v in zip(self.names, x)}
+        buf = N.OpBuffer()
+        derivs: List[Tuple[int, int, float, np.ndarray]] = []
+        for op in self.circuit.Ops():
+            g = op.Gate
+            if g is None or op.ControlFlow is not None:
+                continue
+            sym = g if isinstance(g, SymbolicGate) else None
+            bound = sym.Bind(b) if sym is not None else g
+            if self.density:
+                from .densitymatrix import _marshal_op
+                from .circuit import Operation
+
+                start = len(buf)
+                _marshal_op(buf, Operation(bound, op.Qubits), self.noise)
+                k = start  # the gate record precedes any noise channel
+            else:
+                k = N.add_sv_gate(buf, bound, op.Qubits)
+            if sym is None:
+                continue
+            args = [e.eval(b) for e in sym.exprs]
+            for j, e in enumerate(sym.exprs):
+                free = e.free()
+                if not free:
+                    continue
+                dm = np.ascontiguousarray(gate_derivative(sym.ctor_name, args, j), dtype=np.complex128)
+                for p in dict.fromkeys(free):
+                    if p not in self.index:
+                        continue
+                    derivs.append((k, self.index[p], e.deriv(p, b), dm))
+        arr = (N.GoquDeriv * max(1, len(derivs)))()
+        keep = []
+        for i, (k, p, sc, dm) in enumerate(derivs):
+            flat = dm.reshape(-1).view(np.float64)
+            keep.append(flat)
+            arr[i].op = k
+            arr[i].param = p
+            arr[i].scale = sc
+            arr[i].dmatrix = N.dptr(flat)
+        return buf, arr, len(derivs), keep
+
+
+def Adjoint(circuit: Circuit, hamiltonian: PauliSum, param_names: Sequence[str], sim=None,
+            precision: int = N.C128):
+    """A GradientFunc-compatible constructor beside ParameterShift: x -> dE/dx via one
+    forward and one reverse sweep on the device.  The returned function has a
+    `.value_and_grad(x)` attribute returning (E, grads)."""
+    from . import statevector
+
+    prog = AdjointProgram(circuit, param_names)
+    terms = hamiltonian.Terms()
+    xm, zm, ny, cf = masks(terms)
+    own = sim is None
+    state = {"sim": sim}
+
+    def value_and_grad(x):
+        s = state["sim"]
+        if s is None:
+            s = statevector.New(circuit.NumQubits(), precision)
+            if not own:
+                state["sim"] = s
+        buf, darr, nd, keep = prog.build(x)
+        grads = np.zeros(len(prog.names))
+        e = C.c_double(0)
+        try:
+            N.check(N.lib().goqu_sv_adjoint(s._h, buf.array(), len(buf), darr, nd, len(prog.names), len(terms),
+                                             N.u64ptr(xm), N.u64ptr(zm), N.i32ptr(ny), N.dptr(cf), C.byref(e),
+                                             N.dptr(grads)))
+        finally:
+            if own:
+                s.Close()
+        return e.value, grads
+
+    def g(x):
+        return value_and_grad(x)[1]
+
+    g.value_and_grad = value_and_grad
+    return g
+
+
+def AdjointDensity(circuit: Circuit, hamiltonian: PauliSum, param_names: Sequence[str], noise=None,
+                   checkpoint_every: int = 16):
+    """Adjoint gradient of Tr(H rho(theta)) on a noisy density matrix (checkpointed)."""
+    from . import densitymatrix
+
+    prog = AdjointProgram(circuit, param_names, density=True, noise=noise)
+    terms = hamiltonian.Terms()
+    xm, zm, ny, cf = masks(terms)
+
+    def value_and_grad(x):
+        s = densitymatrix.New(circuit.NumQubits())
+        buf, darr, nd, keep = prog.build(x)
+        grads = np.zeros(len(prog.names))
+        e = C.c_double(0)
+        try:
+            N.check(N.lib().goqu_dm_adjoint(s._h, buf.array(), len(buf), darr, nd, len(prog.names), len(terms),
+                                             N.u64ptr(xm), N.u64ptr(zm), N.i32ptr(ny), N.dptr(cf),
+                                             int(checkpoint_every), C.byref(e), N.dptr(grads)))
+        finally:
+            s.Close()
+        return e.value, grads
+
+    def g(x):
+        return value_and_grad(x)[1]
+
+    g.value_and_grad = value_and_grad
+    return g

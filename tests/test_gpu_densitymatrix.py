@@ -1,0 +1,113 @@
+"""Density-matrix parity on the GPU against the oracle (goqu's two-pass / Kraus-loop
+algorithm restated in C), max |delta rho| <= 1e-12, plus the reference's channel KATs."""
+import math
+
+import numpy as np
+import pytest
+
+from circuits import random_circuit
+from goqu_amd import densitymatrix as D
+from goqu_amd import gate as G
+from goqu_amd import noise as NZ
+from goqu_amd.circuit import NewBuilder
+from goqu_amd.pauli import NewPauliString, X, Y, Z
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def noise_model():
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(1e-2))
+    nm.AddDefaultError(2, NZ.Depolarizing2Q(2e-2))
+    nm.AddGateError("H", NZ.AmplitudeDamping(0.05))
+    nm.AddGateError("delay", NZ.PhaseDamping(0.03))
+    nm.AddGateQubitError("X", [0], NZ.GeneralizedAmplitudeDamping(0.3, 0.1))
+    nm.AddReadoutError(0, NZ.NewReadoutError(0.02, 0.05))
+    return nm
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7, 9])
+def test_noisy_dm_parity(n):
+    rng = np.random.default_rng(500 + n)
+    c = random_circuit(n, 4, rng, three_q=False, controlled=False)
+    b = NewBuilder("withdelay", n)
+    b.Compose(c)
+    for q in range(n):
+        b.Delay(q, 1e-6)
+    if n > 1:
+        b.Reset(1)
+    c = b.Build()
+    nm = noise_model()
+    dm = D.New(n).WithNoise(nm)
+    dm.Evolve(c)
+    got = dm.DensityMatrix()
+    want = O.dm_zero(n)
+    assert O.dm_apply_ops(want, n, D.marshal(c, nm)) == 0
+    assert np.max(np.abs(got - want)) <= TOL
+    p01 = np.full(n, -1.0)
+    p10 = np.full(n, -1.0)
+    p01[0], p10[0] = 0.02, 0.05
+    assert np.max(np.abs(dm.Probabilities() - O.dm_probs(want, n, p01, p10))) <= TOL
+    for _ in range(6):
+        ops = {int(q): [X, Y, Z][rng.integers(3)] for q in rng.choice(n, int(rng.integers(1, n + 1)), replace=False)}
+        ps = NewPauliString(rng.normal(), ops, n)
+        assert abs(dm.ExpectPauliString(ps) - O.dm_expect(want, n, ps).real) <= TOL
+    assert abs(dm.Purity() - O.dm_purity(want, n)) <= 1e-12
+    assert dm.ExpectationValue([0]) == O.dm_expectation_value(want, n, [0])
+    u = rng.random(200)
+    counts = dm.Run(c, 200, uniforms=u)
+    assert sum(counts.values()) == 200
+
+
+def test_dm_kats():
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(0.75))
+    dm = D.New(1).WithNoise(nm)
+    dm.Evolve(NewBuilder("x", 1).X(0).Build())
+    rho = dm.DensityMatrix()
+    assert abs(rho[0].real - 0.5) < 1e-10 and abs(rho[3].real - 0.5) < 1e-10
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(0.5))
+    dm = D.New(1).WithNoise(nm)
+    dm.Evolve(NewBuilder("h", 1).H(0).Build())
+    assert round(dm.Purity(), 4) == 0.5556
+
+
+@pytest.mark.parametrize("n", [4, 6])
+def test_dm_multiqubit_gates_exact(n):
+    """3+ qubit and multi-controlled gates act exactly: rho = |psi><psi| of the statevector."""
+    rng = np.random.default_rng(n)
+    c = random_circuit(n, 3, rng, three_q=True, controlled=True)
+    dm = D.New(n)
+    dm.Evolve(c)
+    psi = O.sv_evolve(c)
+    assert np.max(np.abs(dm.DensityMatrix() - np.outer(psi, psi.conj()).reshape(-1))) <= TOL
+    assert abs(dm.Fidelity(psi) - 1) < 1e-12
+
+
+def test_dm_14q_trace_and_hermitian():
+    """At the 14-qubit configuration size: trace 1 and Hermiticity through noisy layers."""
+    n = 14
+    rng = np.random.default_rng(14)
+    b = NewBuilder("c5", n)
+    for _ in range(2):
+        for q in range(n):
+            b.U3(*rng.uniform(0, 2 * math.pi, 3), q)
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            b.CNOT(int(perm[i]), int(perm[i + 1]))
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(1e-3))
+    nm.AddDefaultError(2, NZ.Depolarizing2Q(1e-2))
+    dm = D.New(n).WithNoise(nm)
+    dm.Evolve(b.Build())
+    p = dm.DiagonalProbs()
+    assert abs(p.sum() - 1) < 1e-10
+    blk = np.empty((64, 64), dtype=complex)
+    full = dm.DensityMatrix()
+    dim = 1 << n
+    for r in range(64):
+        blk[r] = full[r * dim: r * dim + 64]
+    assert np.max(np.abs(blk - blk.conj().T)) < 1e-13

@@ -1,0 +1,114 @@
+"""Adjoint differentiation on the GPU, checked against the reference's parameter-shift rule
+(exact when each parameter enters once as exp(-i theta P / 2)), central finite differences,
+and the oracle-based energy."""
+import math
+
+import numpy as np
+import pytest
+
+from circuits import efficient_su2
+from goqu_amd import gradient as GR
+from goqu_amd import statevector
+from goqu_amd.circuit import Bind, FreeParameters, NewBuilder, NewVector, Literal, Mul
+from goqu_amd.pauli import NewPauliString, NewPauliSum, X, Y, Z
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def random_hamiltonian(n, terms, rng, wmax=4):
+    out = []
+    for _ in range(terms):
+        w = int(rng.integers(1, min(wmax, n) + 1))
+        ops = {int(q): [X, Y, Z][rng.integers(3)] for q in rng.choice(n, w, replace=False)}
+        out.append(NewPauliString(rng.normal(), ops, n))
+    return NewPauliSum(out)
+
+
+def oracle_energy(c, names, x, H):
+    psi = O.sv_evolve(Bind(c, dict(zip(names, x))))
+    return sum(O.sv_expect(psi, t) for t in H.Terms()).real
+
+
+def test_gradient_closed_form():
+    """algorithm/gradient/gradient_test.go:15-87: E = cos(theta), dE/dtheta = -sin(theta)."""
+    th = NewVector("t", 1)
+    b = NewBuilder("ry", 1)
+    b.SymRY(th.At(0), 0)
+    c = b.Build()
+    H = NewPauliSum([NewPauliString(1, {0: Z}, 1)])
+    names = FreeParameters(c)
+    for x in [0.3, 1.2, -2.0]:
+        e, g = GR.Adjoint(c, H, names).value_and_grad([x])
+        assert abs(e - math.cos(x)) < 1e-12
+        assert abs(g[0] + math.sin(x)) < 1e-12
+        ps = GR.ParameterShift(GR.CostFunc(c, H, names))([x])
+        assert abs(ps[0] + math.sin(x)) < 1e-10
+
+
+@pytest.mark.parametrize("n,reps", [(4, 2), (8, 3), (12, 2)])
+def test_adjoint_vs_parameter_shift_efficient_su2(n, reps):
+    rng = np.random.default_rng(n)
+    c, npar = efficient_su2(n, reps)
+    names = FreeParameters(c)
+    assert len(names) == npar
+    H = random_hamiltonian(n, 20, rng)
+    x = rng.uniform(0, 2 * math.pi, npar)
+    e, g = GR.Adjoint(c, H, names).value_and_grad(x)
+    assert abs(e - oracle_energy(c, names, x, H)) < 1e-11
+    sim = statevector.New(n)
+    ps = GR.ParameterShift(GR.CostFunc(c, H, names, sim=sim))(x)
+    assert np.max(np.abs(g - ps)) < 1e-10
+
+
+def test_adjoint_vs_finite_difference_reused_scaled_params():
+    """QAOA-style reuse: the same parameter in many gates with scale factors (parameter shift
+    is inexact here; central differences are the check)."""
+    n = 6
+    gam, bet = NewVector("g", 2), NewVector("b", 2)
+    b = NewBuilder("qaoa", n)
+    for q in range(n):
+        b.H(q)
+    edges = [(0, 1), (1, 2), (2, 3), (3, 4), (4, 5), (0, 5), (1, 4)]
+    for k in range(2):
+        for (i, j) in edges:
+            b.SymRZZ(Mul(Literal(1.0), gam.At(k)), i, j)
+        for q in range(n):
+            b.SymRX(Mul(Literal(2.0), bet.At(k)), q)
+    c = b.Build()
+    names = FreeParameters(c)
+    terms = []
+    for (i, j) in edges:
+        terms.append(NewPauliString(-0.5, {}, n))
+        terms.append(NewPauliString(0.5, {i: Z, j: Z}, n))
+    H = NewPauliSum(terms)
+    x = np.array([0.4, 0.9, 1.3, 0.2])
+    e, g = GR.Adjoint(c, H, names).value_and_grad(x)
+    fd = GR.FiniteDifference(GR.CostFunc(c, H, names), 1e-5)(x)
+    assert np.max(np.abs(g - fd)) < 1e-8
+    assert abs(e - oracle_energy(c, names, x, H)) < 1e-11
+
+
+def test_adjoint_all_parameterised_gates():
+    """Every differentiable constructor: adjoint against 4th-order central differences."""
+    n = 3
+    th = NewVector("p", 16)
+    b = NewBuilder("mix", n)
+    b.H(0).H(1).H(2)
+    b.SymRX(th.At(0), 0).SymRY(th.At(1), 1).SymRZ(th.At(2), 2)
+    b.SymPhase(th.At(3), 0).SymU3(th.At(4), th.At(5), th.At(6), 1)
+    b.SymCP(th.At(7), 0, 1).SymRXX(th.At(8), 1, 2).SymRYY(th.At(9), 0, 2).SymRZZ(th.At(10), 0, 1)
+    b.SymRot(th.At(11), th.At(12), th.At(13), 2).SymFSim(th.At(14), th.At(15), 0, 2)
+    c = b.Build()
+    names = FreeParameters(c)
+    rng = np.random.default_rng(3)
+    H = random_hamiltonian(n, 10, rng, 3)
+    x = rng.uniform(0, 2 * math.pi, len(names))
+    _, g = GR.Adjoint(c, H, names).value_and_grad(x)
+    f = lambda v: oracle_energy(c, names, v, H)  # noqa: E731
+    h = 1e-3
+    for i in range(len(x)):
+        d = np.zeros(len(x))
+        d[i] = 1
+        fd = (-f(x + 2 * h * d) + 8 * f(x + h * d) - 8 * f(x - h * d) + f(x - 2 * h * d)) / (12 * h)
+        assert abs(g[i] - fd) < 1e-9, (names[i], g[i], fd)

@@ -1,0 +1,173 @@
+"""Pins the gate library and the CPU oracle with mathematics that does not share their code:
+gate matrices against matrix exponentials of their generators and textbook closed forms,
+and the oracle's kernels against full 2^n x 2^n unitaries built by brute force in numpy
+(qubits[0] = most significant bit of the gate-local index)."""
+import math
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from circuits import random_circuit
+from goqu_amd import gate as G
+from goqu_amd.circuit import NewBuilder
+from oracle import oracle as O
+
+PX = np.array([[0, 1], [1, 0]], dtype=complex)
+PY = np.array([[0, -1j], [1j, 0]], dtype=complex)
+PZ = np.diag([1, -1]).astype(complex)
+I2 = np.eye(2, dtype=complex)
+
+
+def mat(g):
+    d = 1 << g.Qubits()
+    return g.Matrix().reshape(d, d)
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.3, 1.7, -2.9, 5.5])
+def test_rotation_gates_are_exponentials(theta):
+    assert np.allclose(mat(G.RX(theta)), expm(-0.5j * theta * PX), atol=1e-14)
+    assert np.allclose(mat(G.RY(theta)), expm(-0.5j * theta * PY), atol=1e-14)
+    assert np.allclose(mat(G.RZ(theta)), expm(-0.5j * theta * PZ), atol=1e-14)
+    assert np.allclose(mat(G.RXX(theta)), expm(-0.5j * theta * np.kron(PX, PX)), atol=1e-14)
+    assert np.allclose(mat(G.RYY(theta)), expm(-0.5j * theta * np.kron(PY, PY)), atol=1e-14)
+    assert np.allclose(mat(G.RZZ(theta)), expm(-0.5j * theta * np.kron(PZ, PZ)), atol=1e-14)
+    # controlled rotations: identity block then the rotation (controls are the MSB)
+    for cg, g in [(G.CRX, G.RX), (G.CRY, G.RY), (G.CRZ, G.RZ)]:
+        want = np.block([[I2, np.zeros((2, 2))], [np.zeros((2, 2)), mat(g(theta))]])
+        assert np.allclose(mat(cg(theta)), want, atol=1e-14)
+    assert np.allclose(mat(G.CP(theta)), np.diag([1, 1, 1, np.exp(1j * theta)]), atol=1e-14)
+
+
+@pytest.mark.parametrize("a", [(0.1, 0.2, 0.3), (2.0, -1.0, 4.0), (math.pi, 0.0, math.pi / 2)])
+def test_u3_family_closed_forms(a):
+    th, ph, la = a
+    # U3 = e^{i(phi+lambda)/2} RZ(phi) RY(theta) RZ(lambda)
+    want = np.exp(0.5j * (ph + la)) * mat(G.RZ(ph)) @ mat(G.RY(th)) @ mat(G.RZ(la))
+    assert np.allclose(mat(G.U3(th, ph, la)), want, atol=1e-14)
+    assert np.allclose(mat(G.U2(ph, la)), mat(G.U3(math.pi / 2, ph, la)), atol=1e-14)
+    assert np.allclose(mat(G.U1(la)), mat(G.Phase(la)), atol=1e-14)
+    # goqu's Rot(phi, theta, omega) matrix (parameterized.go:459-475) is RZ(phi) RY(theta) RZ(omega)
+    assert np.allclose(mat(G.Rot(ph, th, la)), mat(G.RZ(ph)) @ mat(G.RY(th)) @ mat(G.RZ(la)), atol=1e-14)
+
+
+def test_fixed_gates():
+    s2 = 1 / math.sqrt(2)
+    assert np.allclose(mat(G.H), s2 * (PX + PZ))
+    assert np.allclose(mat(G.SX) @ mat(G.SX), PX)
+    assert np.allclose(mat(G.S) @ mat(G.S), PZ)
+    assert np.allclose(mat(G.T) @ mat(G.T), mat(G.S))
+    cx = np.eye(4)[[0, 1, 3, 2]]
+    assert np.allclose(mat(G.CNOT), cx)
+    assert np.allclose(mat(G.SWAP), np.eye(4)[[0, 2, 1, 3]])
+    assert np.allclose(mat(G.CZ), np.diag([1, 1, 1, -1]))
+    assert np.allclose(mat(G.CY), np.block([[I2, 0 * I2], [0 * I2, PY]]))
+    assert np.allclose(mat(G.ISWAP), expm(0.25j * math.pi * (np.kron(PX, PX) + np.kron(PY, PY))), atol=1e-14)
+    assert np.allclose(mat(G.CCX), np.eye(8)[[0, 1, 2, 3, 4, 5, 7, 6]])
+    assert np.allclose(mat(G.CSWAP), np.eye(8)[[0, 1, 2, 3, 4, 6, 5, 7]])
+    for g in [G.ECR, G.DCX, G.CH, G.CSX, G.Sycamore, G.MS(0.3, 0.9), G.FSim(0.4, 1.1), G.PSwap(0.7)]:
+        m = mat(g)
+        assert np.allclose(m @ m.conj().T, np.eye(4), atol=1e-14), g.Name()
+    # FSim(pi/2, pi/6) is the Sycamore gate
+    assert np.allclose(mat(G.FSim(math.pi / 2, math.pi / 6)), mat(G.Sycamore), atol=1e-14)
+
+
+def full_unitary(n, g, qubits):
+    """Brute force: U[i', i] = M[l(i'), l(i)] when i and i' agree outside the op's qubits."""
+    k = len(qubits)
+    M = g.Matrix().reshape(1 << k, 1 << k)
+    N = 1 << n
+    opmask = sum(1 << q for q in qubits)
+    U = np.zeros((N, N), dtype=complex)
+    loc = np.zeros(N, dtype=int)
+    for i in range(N):
+        loc[i] = sum(((i >> q) & 1) << (k - 1 - j) for j, q in enumerate(qubits))
+    for i in range(N):
+        for ip in range(N):
+            if (i & ~opmask) == (ip & ~opmask):
+                U[ip, i] = M[loc[ip], loc[i]]
+    return U
+
+
+@pytest.mark.parametrize("n", [1, 3, 5, 7])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_oracle_equals_bruteforce(n, seed):
+    rng = np.random.default_rng(100 * n + seed)
+    c = random_circuit(n, 3, rng)
+    psi = np.zeros(1 << n, dtype=complex)
+    psi[0] = 1
+    for op in c.Ops():
+        if op.Gate is None:
+            continue
+        psi = full_unitary(n, op.Gate, op.Qubits) @ psi
+    got = O.sv_evolve(c)
+    assert np.max(np.abs(got - psi)) < 1e-12
+
+
+def test_oracle_pauli_expect_bruteforce():
+    from goqu_amd.pauli import NewPauliString, X, Y, Z
+
+    n = 5
+    rng = np.random.default_rng(4)
+    psi = O.sv_evolve(random_circuit(n, 3, rng))
+    for _ in range(20):
+        ops = {int(q): [X, Y, Z][rng.integers(3)] for q in rng.choice(n, int(rng.integers(1, n + 1)), replace=False)}
+        ps = NewPauliString(complex(rng.normal(), rng.normal()), ops, n)
+        P = np.array([[1]], dtype=complex)
+        for q in range(n - 1, -1, -1):  # qubit n-1 is the most significant kron factor
+            P = np.kron(P, {0: I2, X: PX, Y: PY, Z: PZ}[ps.Op(q)])
+        want = ps.coeff * (psi.conj() @ P @ psi)
+        assert abs(O.sv_expect(psi, ps) - want) < 1e-12
+
+
+def test_oracle_sampling_definition():
+    """First index with u < cum_i, else the last index (sim.go:288-298)."""
+    probs = np.array([0.1, 0.0, 0.25, 0.65])
+    u = np.array([0.0, 0.0999, 0.1, 0.34, 0.35, 0.999999, 1.0])
+    got = O.sample_probs(probs, u)
+    assert list(got) == [0, 0, 2, 2, 3, 3, 3]
+
+
+def test_oracle_dm_noiseless_equals_pure():
+    """DM noiseless == |psi><psi| (densitymatrix/sim_test.go:46-120)."""
+    from goqu_amd import densitymatrix as D
+
+    n = 4
+    rng = np.random.default_rng(8)
+    c = random_circuit(n, 3, rng, three_q=False, controlled=False)
+    psi = O.sv_evolve(c)
+    rho = O.dm_zero(n)
+    assert O.dm_apply_ops(rho, n, D.marshal(c)) == 0
+    assert np.max(np.abs(rho - np.outer(psi, psi.conj()).reshape(-1))) < 1e-12
+
+
+def test_oracle_dm_channels_bruteforce():
+    """Kraus channels against sum_k K rho K^dag built in numpy."""
+    from goqu_amd import densitymatrix as D
+    from goqu_amd import noise as NZ
+
+    n = 3
+    rng = np.random.default_rng(12)
+    c = random_circuit(n, 2, rng, three_q=False, controlled=False)
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(0.07))
+    nm.AddDefaultError(2, NZ.Depolarizing2Q(0.05))
+    nm.AddGateError("H", NZ.AmplitudeDamping(0.2))
+    rho_ref = np.zeros((1 << n, 1 << n), dtype=complex)
+    rho_ref[0, 0] = 1
+    for op in c.Ops():
+        if op.Gate is None:
+            continue
+        U = full_unitary(n, op.Gate, op.Qubits)
+        rho_ref = U @ rho_ref @ U.conj().T
+        ch = nm.Lookup(op.Gate.Name(), op.Qubits)
+        if ch is not None:
+            acc = np.zeros_like(rho_ref)
+            for K in ch.Kraus():
+                Kg = G.Unitary("K", K, ch.Qubits())
+                KU = full_unitary(n, Kg, op.Qubits[: ch.Qubits()])
+                acc += KU @ rho_ref @ KU.conj().T
+            rho_ref = acc
+    rho = O.dm_zero(n)
+    assert O.dm_apply_ops(rho, n, D.marshal(c, nm)) == 0
+    assert np.max(np.abs(rho - rho_ref.reshape(-1))) < 1e-12

@@ -41,3 +41,32 @@ def test_planner_high_qubits_and_stage_count():
     assert np.max(np.abs(s - want)) < 1e-12
     gates = sum(1 for op in c.Ops() if op.Gate is not None)
     assert stages < gates / 4, (stages, gates)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6])
+def test_dm_lowering_matches_oracle(n):
+    """Density-matrix lowering (U on row qubits, conj(U) on column qubits, Kraus channels as
+    superoperators, reset) through the planner, against the oracle's two-pass Kraus loop."""
+    from goqu_amd import densitymatrix as D
+    from goqu_amd import noise as NZ
+    from goqu_amd.circuit import NewBuilder
+
+    rng = np.random.default_rng(70 + n)
+    c0 = random_circuit(n, 3, rng, three_q=False, controlled=False)
+    b = NewBuilder("dm", n).Compose(c0)
+    for q in range(n):
+        b.Delay(q, 1.0)
+    b.Reset(0)
+    c = b.Build()
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(0.03))
+    nm.AddDefaultError(2, NZ.Depolarizing2Q(0.04))
+    nm.AddGateError("delay", NZ.AmplitudeDamping(0.1))
+    buf = D.marshal(c, nm)
+    nqi = max(2 * n, 6)
+    s = np.zeros(1 << nqi, dtype=np.complex128)
+    s[0] = 1
+    E.apply(s, n, buf, tile_bits=min(12, nqi), low_bits=3, dm=True)
+    want = O.dm_zero(n)
+    assert O.dm_apply_ops(want, n, buf) == 0
+    assert np.max(np.abs(s[: 1 << (2 * n)] - want)) < 1e-12
