@@ -79,6 +79,9 @@ def main() -> int:
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run one read-only sweep (norm) and one write-only sweep (reset) of the state, "
+                         "whose HBM bytes are known, to scale PMC byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -113,6 +116,9 @@ def main() -> int:
     kms, launches, kbytes = N.profile_read()
     N.profile(False)
     norm = sim.Norm2()
+    if args.calib:  # k_prob0 reads 16 B x 2^n, k_fill_basis writes 16 B x 2^n
+        sim.Norm2()
+        sim.Reset()
     sim.Close()
 
     ms_step = dt / args.steps * 1e3

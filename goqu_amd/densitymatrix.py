@@ -160,9 +160,87 @@ class Sim:
                                       N.dptr(p10) if p10 is not None else None, N.dptr(out)))
         return out
 
+    # -- dynamic circuits (sim/densitymatrix/dynamic.go:13-255) --
+    def MeasureQubit(self, q: int, u: float) -> int:
+        """measureQubitDM (dynamic.go:211-255): P0 from the diagonal, outcome = (u >= P0),
+        rho <- P rho P / p_outcome (skipped when p_outcome < 1e-15)."""
+        p = self.DiagonalProbs()
+        idx = np.arange(self.dim)
+        prob0 = float(p[(idx >> q) & 1 == 0].sum())
+        outcome = 1 if u >= prob0 else 0
+        pout = prob0 if outcome == 0 else 1 - prob0
+        if pout < 1e-15:
+            return outcome
+        e = np.zeros(4, dtype=np.complex128)
+        e[3 if outcome else 0] = 1 / np.sqrt(pout)
+        buf = N.OpBuffer()
+        buf.add(N.OP_CHANNEL, [q], e, num_kraus=1)
+        self.ApplyOps(buf)
+        return outcome
+
+    def RunDynamic(self, c: Circuit, shots: int, rng=None) -> Dict[str, int]:
+        if c.NumQubits() != self.num_qubits:
+            raise ValueError(f"circuit has {c.NumQubits()} qubits, simulator has {self.num_qubits}")
+        if shots <= 0:
+            raise ValueError(f"shots must be positive, got {shots}")
+        rng = rng if rng is not None else np.random.default_rng()
+        counts: Dict[str, int] = {}
+        nc = c.NumClbits()
+        for _ in range(shots):
+            self.Reset()
+            clbits = [0] * nc
+            self._exec(c.Ops(), clbits, rng)
+            bs = "".join("1" if clbits[nc - 1 - i] else "0" for i in range(nc))
+            counts[bs] = counts.get(bs, 0) + 1
+        return counts
+
+    def _exec(self, ops: List[Operation], clbits: List[int], rng) -> None:
+        for op in ops:
+            cf = op.ControlFlow
+            if cf is not None:
+                if cf.Type == "while":
+                    from .circuit import MaxControlFlowIterations
+
+                    for _ in range(MaxControlFlowIterations):
+                        if clbits[cf.Condition.Clbit] != cf.Condition.Value:
+                            break
+                        self._exec(cf.Bodies[0], clbits, rng)
+                elif cf.Type == "for":
+                    start, end, step = cf.ForRange
+                    for _ in range(start, end, step):
+                        self._exec(cf.Bodies[0], clbits, rng)
+                elif cf.Type == "if_else":
+                    if clbits[cf.Condition.Clbit] == cf.Condition.Value:
+                        self._exec(cf.Bodies[0], clbits, rng)
+                    elif len(cf.Bodies) > 1:
+                        self._exec(cf.Bodies[1], clbits, rng)
+                continue
+            g = op.Gate
+            if g is None:
+                if op.Clbits and op.Qubits:
+                    clbits[op.Clbits[0]] = self.MeasureQubit(op.Qubits[0], rng.random())
+                continue
+            if g.Name() == "reset":  # measure, X on outcome 1, then the "reset" channel
+                buf = N.OpBuffer()
+                if self.MeasureQubit(op.Qubits[0], rng.random()) == 1:
+                    buf.add(N.OP_GATE, op.Qubits[:1], G.X.Matrix())
+                if self.noise is not None:
+                    ch = self.noise.Lookup("reset", op.Qubits)
+                    if ch is not None:
+                        _add_channel(buf, ch, op.Qubits)
+                self.ApplyOps(buf)
+                continue
+            if op.Condition is not None and clbits[op.Condition.Clbit] != op.Condition.Value:
+                continue
+            buf = N.OpBuffer()
+            _marshal_op(buf, op, self.noise)
+            self.ApplyOps(buf)
+
     def Run(self, c: Circuit, shots: int, uniforms=None, rng=None) -> Dict[str, int]:
+        """Run (sim.go:51-67): dynamic circuits go to RunDynamic; otherwise evolve, apply the
+        readout error and sample."""
         if c.IsDynamic():
-            raise NotImplementedError("densitymatrix RunDynamic is not supported on the device path")
+            return self.RunDynamic(c, shots, rng=rng)
         self.Evolve(c)
         probs = self.Probabilities()
         counts: Dict[str, int] = {}

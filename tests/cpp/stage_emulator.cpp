@@ -218,3 +218,22 @@ int emu_apply(double *state, int nq_internal, int nq_user, const goqu_op_t *ops,
 }
 const char *emu_error(void) { return g_err.c_str(); }
 }
+
+extern "C" int emu_plan_stats(const goqu_op_t *ops, int nops, int nq, int tile_bits, int low_bits, int64_t *stages,
+                              int64_t *remaps, int64_t *instrs, int64_t *eops) {
+    std::vector<EOp> e;
+    std::string err;
+    int rc = lower_sv_ops(ops, nops, nq, e, err);
+    if (rc) return rc;
+    PlanConfig cfg;
+    cfg.tile_bits = tile_bits;
+    cfg.low_bits = low_bits;
+    Program P = plan_program(e, nq, cfg);
+    *stages = (int64_t)P.stages.size();
+    *instrs = (int64_t)P.instrs.size();
+    *eops = P.num_eops;
+    int64_t r = 0;
+    for (const Instr &I : P.instrs) r += I.op == I_REMAP;
+    *remaps = r;
+    return 0;
+}

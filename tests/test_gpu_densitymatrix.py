@@ -87,6 +87,23 @@ def test_dm_multiqubit_gates_exact(n):
     assert abs(dm.Fidelity(psi) - 1) < 1e-12
 
 
+def test_dm_dynamic_feed_forward_and_reset():
+    """densitymatrix RunDynamic (dynamic.go): measurement collapse, feed-forward and reset."""
+    b = NewBuilder("ff", 2).WithClbits(2)
+    b.X(0).Measure(0, 0).If(0, 1, G.X, 1).Measure(1, 1)
+    c = b.Build()
+    dm = D.New(2)
+    assert dm.Run(c, 10, rng=np.random.default_rng(1)) == {"11": 10}
+    b = NewBuilder("rs", 1).WithClbits(1)
+    b.H(0).Reset(0).Measure(0, 0)
+    assert D.New(1).Run(b.Build(), 10, rng=np.random.default_rng(2)) == {"0": 10}
+    # collapse renormalises: after measuring qubit 0 of a Bell pair, qubit 1 agrees
+    b = NewBuilder("bell", 2).WithClbits(2)
+    b.H(0).CNOT(0, 1).Measure(0, 0).H(1).H(1).Measure(1, 1)
+    counts = D.New(2).Run(b.Build(), 200, rng=np.random.default_rng(3))
+    assert set(counts) <= {"00", "11"} and 60 <= counts.get("00", 0) <= 140
+
+
 def test_dm_14q_trace_and_hermitian():
     """At the 14-qubit configuration size: trace 1 and Hermiticity through noisy layers."""
     n = 14
