@@ -70,6 +70,56 @@ def cpu_baseline(circ, n: int, budget_s: float) -> dict:
             "sample": f"first {done} of {len(ops)} gates of the same {n}-qubit circuit, one sweep per gate"}
 
 
+def adjoint_section(qubits: int, reps: int, terms: int, ps_params: int, seed: int = 3) -> dict:
+    """Adjoint gradients vs the reference's parameter-shift rule on the same GPU.
+
+    Workload: EfficientSU2(qubits, reps, Circular) with theta ~ U[0, 2pi) and H = `terms` random
+    Pauli strings of weight 1..4 with N(0,1) coefficients.  Parameter shift (2 Evolves +
+    <H> per parameter, gradient.go:49-66) is timed on the first `ps_params` parameters with a
+    reused simulator and extrapolated to all of them; both paths run on the GPU."""
+    import math
+
+    from goqu_amd import gradient as GR
+    from goqu_amd import qaoa as Q
+    from goqu_amd import statevector
+    from goqu_amd.circuit import FreeParameters
+    from goqu_amd.pauli import NewPauliString, NewPauliSum, X, Y, Z
+
+    rng = np.random.default_rng(seed)
+    c, npar = Q.EfficientSU2(qubits, reps)
+    names = FreeParameters(c)
+    hs = []
+    for _ in range(terms):
+        w = int(rng.integers(1, 5))
+        ops = {int(q): [X, Y, Z][rng.integers(3)] for q in rng.choice(qubits, w, replace=False)}
+        hs.append(NewPauliString(rng.normal(), ops, qubits))
+    H = NewPauliSum(hs)
+    x = rng.uniform(0, 2 * math.pi, npar)
+    sim = statevector.New(qubits)
+    adj = GR.Adjoint(c, H, names, sim=sim)
+    adj.value_and_grad(x)  # warm-up
+    t0 = time.perf_counter()
+    e, g = adj.value_and_grad(x)
+    t_adj = time.perf_counter() - t0
+    f = GR.CostFunc(c, H, names, sim=sim)
+    f(x)
+    k = min(ps_params, npar)
+    t0 = time.perf_counter()
+    ps = np.zeros(k)
+    for i in range(k):
+        xp, xm = x.copy(), x.copy()
+        xp[i] += 0.5 * math.pi
+        xm[i] -= 0.5 * math.pi
+        ps[i] = (f(xp) - f(xm)) / 2
+    t_ps = (time.perf_counter() - t0) / k * npar
+    sim.Close()
+    return {"workload": f"EfficientSU2({qubits},{reps},Circular) {npar} params, H={terms} Pauli strings, c128",
+            "adjoint_s": round(t_adj, 4), "adjoint_grads_per_s": round(1.0 / t_adj, 3),
+            "param_shift_s": round(t_ps, 3), "param_shift_grads_per_s": round(1.0 / t_ps, 5),
+            "param_shift_sample": f"first {k} of {npar} parameters, extrapolated",
+            "speedup": round(t_ps / t_adj, 1), "max_abs_diff_vs_param_shift": float(np.max(np.abs(g[:k] - ps)))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +129,9 @@ def main() -> int:
     ap.add_argument("--layers", type=int, default=40)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-qubits", type=int, default=24)
+    ap.add_argument("--ps-params", type=int, default=16)
     ap.add_argument("--calib", action="store_true",
                     help="also run one read-only sweep (norm) and one write-only sweep (reset) of the state, "
                          "whose HBM bytes are known, to scale PMC byte counters")
@@ -129,7 +182,7 @@ def main() -> int:
     avg_launch_ms = kms / max(launches, 1)
     achieved = sweep_bytes / (avg_launch_ms / 1e3) / 1e9
     out = {
-        "metric": "30q c128 gate-apply GB/s vs HBM peak",
+        "metric": "30q c128 gate-apply GB/s vs HBM peak; adjoint grads/s vs param-shift",
         "value": round(value, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -149,6 +202,8 @@ def main() -> int:
                      "kernel": "k_stage<double>", "avg_launch_ms": round(avg_launch_ms, 4)},
         "cpu_baseline": None,
     }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_section(args.adj_qubits, 7, 200, args.ps_params)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(circ, n, args.cpu_budget)
     print(json.dumps(out), flush=True)

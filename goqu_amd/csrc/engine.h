@@ -49,6 +49,10 @@ struct EOp {
         for (int i = 0; i < k; i++) t |= 1ull << q[i];
         return t;
     }
+    // qubits acted on non-diagonally (targets) and only diagonally (controls, diagonal gates):
+    // two ops commute when every qubit they share is diagonal for both
+    uint64_t nondiag() const { return targets(); }
+    uint64_t diag() const { return touch() & ~targets(); }
 };
 
 // ---- device instructions ---------------------------------------------------------------
@@ -61,6 +65,7 @@ enum InstrOp : int32_t {
     I_REMAP = 5,  // new layout lay[0..m-1]: lay[0..3] register local bits, rest thread bits
 };
 
+// Byte arrays below are read by the kernel as aligned 32-bit words (scalar loads).
 struct alignas(16) Instr {
     int32_t op;
     int32_t k;
@@ -71,19 +76,23 @@ struct alignas(16) Instr {
     uint8_t q[4];     // global qubits (I_DIAG), qubits[0] = MSB of the diag index
     int32_t smask;    // I_DENSE: register slot mask (dense matrix is in slot-bit order)
     int32_t pad;
-    uint8_t lay[24];  // I_REMAP: new layout, local tile bit per slot/thread bit
+    uint8_t lay[16];  // I_REMAP: new layout, tile-local bit of register slots 0..3 then thread bits
+    uint8_t gq[16];   // I_REMAP: the global qubit of each lay[] entry
+    uint8_t pad2[16];
 };
-static_assert(sizeof(Instr) == 64, "Instr must stay 64 bytes");
+static_assert(sizeof(Instr) == 96, "Instr must stay 96 bytes");
 
-struct StageHeader {
+struct alignas(16) StageHeader {
     int32_t m;            // tile bits
     int32_t ninstr;
     int32_t instr_off;    // index of first Instr in the program's instruction array
     int32_t pool_off;     // complex-element offset of this stage's pool
-    uint8_t T[kMaxTileBits + 3];    // tile qubits, ascending (local bit j <-> global T[j])
-    uint8_t lay0[24];     // initial layout (local bits: 4 register slots then thread bits)
-    uint8_t layN[24];     // final layout
+    uint8_t T[16];        // tile qubits, ascending (local bit j <-> global T[j])
+    uint8_t lay0[16];     // initial layout (local bits: 4 register slots then thread bits)
+    uint8_t gq0[16];      // global qubits of lay0
+    uint8_t layN[16];     // final layout
 };
+static_assert(sizeof(StageHeader) == 80, "StageHeader layout");
 
 struct Program {
     int nq = 0;           // internal qubit count

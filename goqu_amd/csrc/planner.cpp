@@ -484,24 +484,29 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
     while (first_open < eops.size()) {
         StageOps S;
         S.T = lowmask;
-        uint64_t blocked = 0;
+        // qubits on which deferred ops act non-diagonally / only diagonally: a later op may pass
+        // a deferred one when every qubit they share is diagonal for both (they commute)
+        uint64_t b_nd = 0, b_dg = 0;
         for (size_t i = first_open; i < eops.size(); i++) {
             if (done[i]) continue;
             const EOp &e = eops[i];
-            const uint64_t touch = e.touch(), tg = e.targets();
-            if (touch & blocked) {
-                blocked |= touch;
-            } else {
+            const uint64_t nd = e.nondiag(), dg = e.diag(), tg = e.targets();
+            const bool conflict = (nd & (b_nd | b_dg)) || (dg & b_nd);
+            bool take = false;
+            if (!conflict) {
                 const uint64_t nt = S.T | tg;
                 if (__builtin_popcountll(nt) <= m) {
                     S.T = nt;
                     S.ops.push_back((int)i);
                     done[i] = 1;
-                } else {
-                    blocked |= touch;
+                    take = true;
                 }
             }
-            if ((blocked & all) == all) break;
+            if (!take) {
+                b_nd |= nd;
+                b_dg |= dg;
+            }
+            if ((b_nd & all) == all) break;
         }
         // fill the tile up to m qubits with the lowest unused qubits (longer contiguous runs)
         for (int b = 0; b < nq && __builtin_popcountll(S.T) < m; b++) S.T |= 1ull << b;
@@ -526,9 +531,10 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                     loc[b] = j++;
                 }
         }
-        // targets (local bits) needed by each op in order, for the look-ahead
-        std::vector<uint32_t> need(S.ops.size(), 0);
-        for (size_t i = 0; i < S.ops.size(); i++) {
+        // targets (local bits) needed by each op, for register choice and look-ahead
+        const size_t nops = S.ops.size();
+        std::vector<uint32_t> need(nops, 0);
+        for (size_t i = 0; i < nops; i++) {
             uint64_t t = eops[S.ops[i]].targets();
             uint32_t l = 0;
             while (t) {
@@ -538,38 +544,48 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             }
             need[i] = l;
         }
-        // choose a register set containing `must`, filled with the bits needed soonest
-        // from position `from`, avoiding the lane bits 0..2 when possible
-        auto choose = [&](uint32_t must, size_t from, int out[4]) {
-            uint32_t set = must;
-            for (size_t i = from; i < S.ops.size() && __builtin_popcount(set) < kRegBits; i++) {
-                uint32_t cand = need[i] & ~set;
-                if (__builtin_popcount(set | cand) <= kRegBits) set |= cand;
-            }
+        // List scheduling: an op is ready when it commutes with every earlier op of the stage
+        // still waiting; ready ops whose targets are register slots run first, and a remap is
+        // emitted only when no ready op can run.
+        std::vector<char> fin(nops, 0);
+        auto commute = [&](size_t a, size_t b) {
+            const EOp &x = eops[S.ops[a]], &y = eops[S.ops[b]];
+            const uint64_t shared = x.touch() & y.touch();
+            return (shared & (x.nondiag() | y.nondiag())) == 0;
+        };
+        auto ready = [&](size_t i) {
+            for (size_t j = 0; j < i; j++)
+                if (!fin[j] && !commute(i, j)) return false;
+            return true;
+        };
+        // new register set: targets of the first ready op that needs registers, then of the
+        // other ready ops, then of the waiting ops in order; never the lowest 3 local bits
+        // unless an op needs them
+        auto choose = [&](int out[4], bool initial) {
+            const uint32_t avoid = (m - kRegBits >= 3) ? 0x7u : 0u;
+            uint32_t set = 0;
+            for (int pass = 0; pass < 2 && __builtin_popcount(set) < kRegBits; pass++)
+                for (size_t i = 0; i < nops && __builtin_popcount(set) < kRegBits; i++) {
+                    if (fin[i] || !need[i]) continue;
+                    if (pass == 0 && !ready(i)) continue;
+                    const uint32_t cand = need[i] & ~set & (initial ? ~avoid : ~0u);
+                    if (initial && (need[i] & avoid)) continue;
+                    if (__builtin_popcount(set | cand) <= kRegBits) set |= cand;
+                }
             for (int b = m - 1; b >= 0 && __builtin_popcount(set) < kRegBits; b--)
-                if (!(set >> b & 1) && b >= std::min(3, m - kRegBits)) set |= 1u << b;
+                if (!(set >> b & 1) && !(avoid >> b & 1)) set |= 1u << b;
             for (int b = m - 1; b >= 0 && __builtin_popcount(set) < kRegBits; b--) set |= 1u << b;
             int s = 0;
             for (int b = 0; b < m; b++)
                 if (set >> b & 1) out[s++] = b;
         };
-        // initial registers: look-ahead, but never on the lowest 3 local bits (coalescing)
         int reg[4];
-        {
-            const uint32_t avoid = (m - kRegBits >= 3) ? 0x7u : 0u;
-            uint32_t set = 0;
-            for (size_t i = 0; i < S.ops.size() && __builtin_popcount(set) < kRegBits; i++) {
-                uint32_t cand = need[i] & ~set & ~avoid;
-                if (__builtin_popcount(set | cand) <= kRegBits) set |= cand;
-            }
-            for (int b = m - 1; b >= 0 && __builtin_popcount(set) < kRegBits; b--)
-                if (!(set >> b & 1) && !(avoid >> b & 1)) set |= 1u << b;
-            int s = 0;
-            for (int b = 0; b < m; b++)
-                if (set >> b & 1) reg[s++] = b;
-        }
+        choose(reg, true);
         Layout cur = make_layout(m, reg);
-        for (int i = 0; i < m; i++) H.lay0[i] = (uint8_t)cur.lay[i];
+        for (int i = 0; i < m; i++) {
+            H.lay0[i] = (uint8_t)cur.lay[i];
+            H.gq0[i] = H.T[cur.lay[i]];
+        }
 
         auto slot_of = [&](int localbit) {
             for (int s = 0; s < kRegBits; s++)
@@ -581,19 +597,33 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             Instr I;
             std::memset(&I, 0, sizeof(I));
             I.op = I_REMAP;
-            for (int i = 0; i < m; i++) I.lay[i] = (uint8_t)cur.lay[i];
+            for (int i = 0; i < m; i++) {
+                I.lay[i] = (uint8_t)cur.lay[i];
+                I.gq[i] = H.T[cur.lay[i]];
+            }
             P.instrs.push_back(I);
         };
 
-        for (size_t i = 0; i < S.ops.size(); i++) {
-            const EOp &e = eops[S.ops[i]];
+        size_t left = nops;
+        while (left) {
+            // pick the first ready op that runs on the current registers
             uint32_t regset = 0;
             for (int s = 0; s < kRegBits; s++) regset |= 1u << cur.lay[s];
-            if ((need[i] & ~regset) != 0) {
+            size_t i = nops;
+            for (size_t j = 0; j < nops; j++)
+                if (!fin[j] && (need[j] & ~regset) == 0 && ready(j)) {
+                    i = j;
+                    break;
+                }
+            if (i == nops) {
                 int nreg[4];
-                choose(need[i], i, nreg);
+                choose(nreg, false);
                 emit_remap(nreg);
+                continue;
             }
+            fin[i] = 1;
+            left--;
+            const EOp &e = eops[S.ops[i]];
             Instr I;
             std::memset(&I, 0, sizeof(I));
             I.cmask = e.cmask;

@@ -47,22 +47,32 @@ struct Lay {
     uint32_t rl[4];   // tile-local bit of register slot s
 };
 
-__device__ __forceinline__ void set_layout(Lay &L, const uint8_t *__restrict__ lay, const uint8_t *__restrict__ T,
+// byte j of four little-endian words (compile-time j after unrolling: scalar shifts)
+__device__ __forceinline__ uint32_t byte_of(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int j) {
+    const uint32_t w = j < 4 ? w0 : j < 8 ? w1 : j < 12 ? w2 : w3;
+    return (w >> (8 * (j & 3))) & 255u;
+}
+
+// Layout from 16-byte arrays of tile-local bits and global qubits, read as four aligned words
+// each (uniform addresses: scalar loads, no per-entry memory round trips).
+__device__ __forceinline__ void set_layout(Lay &L, const uint8_t *__restrict__ lay, const uint8_t *__restrict__ gq,
                                            int m, uint32_t tid, uint64_t base) {
+    const uint32_t *lw = reinterpret_cast<const uint32_t *>(lay);
+    const uint32_t *gw = reinterpret_cast<const uint32_t *>(gq);
+    const uint32_t l0 = lw[0], l1 = lw[1], l2 = lw[2], l3 = lw[3];
+    const uint32_t g0 = gw[0], g1 = gw[1], g2 = gw[2], g3 = gw[3];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-        const int lb = lay[s];
-        L.rl[s] = 1u << lb;
-        L.rq[s] = 1ull << T[lb];
+        L.rl[s] = 1u << byte_of(l0, l1, l2, l3, s);
+        L.rq[s] = 1ull << byte_of(g0, g1, g2, g3, s);
     }
     uint64_t tg = base;
     uint32_t tl = 0;
-    for (int u = 0; u < m - 4; u++) {
-        const int lb = lay[4 + u];
-        const uint64_t gb = 1ull << T[lb];
-        if (tid >> u & 1) {
-            tl |= 1u << lb;
-            tg |= gb;
+#pragma unroll
+    for (int u = 0; u < 12; u++) {
+        if (u < m - 4 && (tid >> u & 1)) {
+            tl |= 1u << byte_of(l0, l1, l2, l3, 4 + u);
+            tg |= 1ull << byte_of(g0, g1, g2, g3, 4 + u);
         }
     }
     L.tg = tg;
@@ -93,16 +103,39 @@ __device__ __forceinline__ cx<R> cmulp(cx<R> a, R br, R bi) {
     return {a.x * br - a.y * bi, a.x * bi + a.y * br};
 }
 
+// Register-slot pattern of a global qubit: bit r set iff register index r has that qubit = 1
+// (uniform across the wavefront; zero when the qubit is not a register slot).
+__device__ __forceinline__ uint32_t reg_pattern(const Lay &L, uint64_t qb) {
+    return (L.rq[0] == qb ? 0xAAAAu : 0u) | (L.rq[1] == qb ? 0xCCCCu : 0u) | (L.rq[2] == qb ? 0xF0F0u : 0u) |
+           (L.rq[3] == qb ? 0xFF00u : 0u);
+}
+
+// Controls split into a uniform pattern over register indices and one per-thread flag:
+// the control holds for register index r iff tok && (pat >> r & 1).
+struct Ctrl {
+    uint32_t pat;
+    bool tok;
+};
+__device__ __forceinline__ Ctrl ctrl_split(uint64_t cm, const Lay &L) {
+    uint32_t pat = 0xFFFFu;
+    if (L.rq[0] & cm) pat &= 0xAAAAu;
+    if (L.rq[1] & cm) pat &= 0xCCCCu;
+    if (L.rq[2] & cm) pat &= 0xF0F0u;
+    if (L.rq[3] & cm) pat &= 0xFF00u;
+    const uint64_t cmt = cm & ~(L.rq[0] | L.rq[1] | L.rq[2] | L.rq[3]);
+    return {pat, (L.tg & cmt) == cmt};
+}
+
 template <int MASK, typename R>
 __device__ __forceinline__ void op_dense(cx<R> (&a)[16], const R *__restrict__ M, uint64_t cm, const Lay &L) {
     constexpr int K = popc4(MASK);
     constexpr int D = 1 << K;
     constexpr int OUTER = 16 >> K;
     constexpr int OM = 15 & ~MASK;
+    const Ctrl c = ctrl_split(cm, L);
     static_for<0, OUTER>([&](auto oc) __attribute__((always_inline)) {
         constexpr int base = dep4(decltype(oc)::value, OM);
-        bool ok = true;
-        if (cm) ok = ((L.tg | rgof<base>(L)) & cm) == cm;
+        const bool ok = c.tok && ((c.pat >> base) & 1u);
         cx<R> x[D];
         static_for<0, D>([&](auto jc) __attribute__((always_inline)) { x[decltype(jc)::value] = a[base | dep4(decltype(jc)::value, MASK)]; });
         static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
@@ -125,12 +158,12 @@ __device__ __forceinline__ void op_dense(cx<R> (&a)[16], const R *__restrict__ M
 template <int S, typename R>
 __device__ __forceinline__ void op_anti1(cx<R> (&a)[16], R p0r, R p0i, R p1r, R p1i, bool pure, uint64_t cm,
                                          const Lay &L) {
+    const Ctrl c = ctrl_split(cm, L);
     static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
         constexpr int r = decltype(rc)::value;
         if constexpr (!(r >> S & 1)) {
             constexpr int r1 = r | (1 << S);
-            bool ok = true;
-            if (cm) ok = ((L.tg | rgof<r>(L)) & cm) == cm;
+            const bool ok = c.tok && ((c.pat >> r) & 1u);
             cx<R> a0 = a[r], a1 = a[r1];
             cx<R> n0 = pure ? a1 : cmulp(a1, p0r, p0i);
             cx<R> n1 = pure ? a0 : cmulp(a0, p1r, p1i);
@@ -144,12 +177,12 @@ __device__ __forceinline__ void op_anti1(cx<R> (&a)[16], R p0r, R p0i, R p1r, R 
 
 template <int S0, int S1, typename R>
 __device__ __forceinline__ void op_swap2(cx<R> (&a)[16], R pr, R pi, bool pure, uint64_t cm, const Lay &L) {
+    const Ctrl c = ctrl_split(cm, L);
     static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
         constexpr int r = decltype(rc)::value;
         if constexpr (!(r >> S0 & 1) && !(r >> S1 & 1)) {
             constexpr int i01 = r | (1 << S1), i10 = r | (1 << S0);
-            bool ok = true;
-            if (cm) ok = ((L.tg | rgof<r>(L)) & cm) == cm;
+            const bool ok = c.tok && ((c.pat >> r) & 1u);
             cx<R> a01 = a[i01], a10 = a[i10];
             cx<R> n01 = pure ? a10 : cmulp(a10, pr, pi);
             cx<R> n10 = pure ? a01 : cmulp(a01, pr, pi);
@@ -179,31 +212,53 @@ template <typename R>
 __device__ __forceinline__ void op_diag(cx<R> (&a)[16], const Instr &I, const R *__restrict__ D, const Lay &L) {
     const int k = I.k;
     const uint64_t cm = I.cmask;
-    const int q0 = I.q[0], q1 = I.q[1], q2 = I.q[2], q3 = I.q[3];
+    const uint32_t qw = *reinterpret_cast<const uint32_t *>(I.q);
+    const int q0 = qw & 255u, q1 = (qw >> 8) & 255u, q2 = (qw >> 16) & 255u, q3 = qw >> 24;
     if (k <= 2) {
+        // index bits split like the controls: a uniform register pattern per diag qubit and a
+        // per-thread part folded into a per-thread table e[ri] = D[tidx | ri]
+        const uint32_t m0 = reg_pattern(L, 1ull << q0);
+        const uint32_t m1 = k == 2 ? reg_pattern(L, 1ull << q1) : 0u;
+        const bool t0 = (L.tg >> q0) & 1, t1 = k == 2 ? ((L.tg >> q1) & 1) : false;
+        const Ctrl c = ctrl_split(cm, L);
         const R d0r = D[0], d0i = D[1], d1r = D[2], d1i = D[3];
         const R d2r = k == 2 ? D[4] : R(0), d2i = k == 2 ? D[5] : R(0);
         const R d3r = k == 2 ? D[6] : R(0), d3i = k == 2 ? D[7] : R(0);
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            const uint64_t g = L.tg | rgof<r>(L);
-            const bool ok = (g & cm) == cm;
-            // bit-wise selection (a select chain on an index becomes a stack lookup table)
-            const bool bhi = (g >> q0) & 1;
-            const bool blo = (g >> q1) & 1;
-            R dr, di;
-            if (k == 2) {
-                const R lr = blo ? d1r : d0r, li = blo ? d1i : d0i;
-                const R hr = blo ? d3r : d2r, hi = blo ? d3i : d2i;
-                dr = bhi ? hr : lr;
-                di = bhi ? hi : li;
+        R e0r, e0i, e1r, e1i, e2r = 0, e2i = 0, e3r = 0, e3i = 0;
+        if (k == 2) {  // bit 1 <- q0, bit 0 <- q1
+            e0r = t0 ? (t1 ? d3r : d2r) : (t1 ? d1r : d0r);
+            e0i = t0 ? (t1 ? d3i : d2i) : (t1 ? d1i : d0i);
+            e1r = t0 ? d3r : d1r;
+            e1i = t0 ? d3i : d1i;
+            e2r = t1 ? d3r : d2r;
+            e2i = t1 ? d3i : d2i;
+            e3r = d3r;
+            e3i = d3i;
+        } else {
+            e0r = t0 ? d1r : d0r;
+            e0i = t0 ? d1i : d0i;
+            e1r = d1r;
+            e1i = d1i;
+        }
+        if (c.tok) {  // per-lane predicate; everything below branches on uniform values
+            if ((m0 | m1) == 0) {  // no diag qubit in registers: one factor per thread
+                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                    constexpr int r = decltype(rc)::value;
+                    if ((c.pat >> r) & 1u) a[r] = cmulp(a[r], e0r, e0i);
+                });
             } else {
-                dr = bhi ? d1r : d0r;
-                di = bhi ? d1i : d0i;
+                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                    constexpr int r = decltype(rc)::value;
+                    if ((c.pat >> r) & 1u) {
+                        const uint32_t ri = (k == 2) ? ((((m0 >> r) & 1u) << 1) | ((m1 >> r) & 1u)) : ((m0 >> r) & 1u);
+                        if (ri == 0) a[r] = cmulp(a[r], e0r, e0i);
+                        else if (ri == 1) a[r] = cmulp(a[r], e1r, e1i);
+                        else if (ri == 2) a[r] = cmulp(a[r], e2r, e2i);
+                        else a[r] = cmulp(a[r], e3r, e3i);
+                    }
+                });
             }
-            cx<R> v = cmulp(a[r], dr, di);
-            if (ok) a[r] = v;
-        });
+        }
     } else {
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
@@ -236,16 +291,22 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
     const uint32_t tid = threadIdx.x;
     const Instr *__restrict__ P = prog + hdr->instr_off;
 
+    const uint32_t *Tw = reinterpret_cast<const uint32_t *>(hdr->T);
+    const uint32_t T0 = Tw[0], T1 = Tw[1], T2 = Tw[2], T3 = Tw[3];
+
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         // deposit the tile index into the qubits outside the tile (T ascending)
         uint64_t base = tile;
-        for (int j = 0; j < m; j++) {
-            const int t = hdr->T[j];
-            const uint64_t lo = base & ((1ull << t) - 1);
-            base = ((base >> t) << (t + 1)) | lo;
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            if (j < m) {
+                const uint32_t t = byte_of(T0, T1, T2, T3, j);
+                const uint64_t lo = base & ((1ull << t) - 1);
+                base = ((base >> t) << (t + 1)) | lo;
+            }
         }
         Lay L;
-        set_layout(L, hdr->lay0, hdr->T, m, tid, base);
+        set_layout(L, hdr->lay0, hdr->gq0, m, tid, base);
         cx<R> a[16];
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
@@ -255,6 +316,8 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
         for (int pc = 0; pc < ninstr; pc++) {
             const Instr &I = P[pc];
             const int op = I.op;
+            // slots and diag qubits as one word each
+            const uint32_t sw = *reinterpret_cast<const uint32_t *>(I.s);
             if (op == I_REMAP) {
                 __syncthreads();
                 static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
@@ -262,7 +325,7 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
                     lds[swz<R>(L.tl | rlof<r>(L))] = a[r];
                 });
                 __syncthreads();
-                set_layout(L, I.lay, hdr->T, m, tid, base);
+                set_layout(L, I.lay, I.gq, m, tid, base);
                 static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
                     constexpr int r = decltype(rc)::value;
                     a[r] = lds[swz<R>(L.tl | rlof<r>(L))];
@@ -294,7 +357,7 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
                 const R *__restrict__ p = pool + 2 * I.mat;
                 const bool pure = I.flags & 1;
                 const R p0r = p[0], p0i = p[1], p1r = p[2], p1i = p[3];
-                switch (I.s[0]) {
+                switch (sw & 255u) {
                 case 0: op_anti1<0>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
                 case 1: op_anti1<1>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
                 case 2: op_anti1<2>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
@@ -304,8 +367,8 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
                 const R *__restrict__ p = pool + 2 * I.mat;
                 const bool pure = I.flags & 1;
                 const R pr = p[0], pi = p[1];
-                const int s0 = I.s[0] < I.s[1] ? I.s[0] : I.s[1];
-                const int s1 = I.s[0] < I.s[1] ? I.s[1] : I.s[0];
+                const int sa = sw & 255u, sb = (sw >> 8) & 255u;
+                const int s0 = sa < sb ? sa : sb, s1 = sa < sb ? sb : sa;
                 switch (s0 * 4 + s1) {
                 case 1: op_swap2<0, 1>(a, pr, pi, pure, I.cmask, L); break;
                 case 2: op_swap2<0, 2>(a, pr, pi, pure, I.cmask, L); break;
@@ -315,7 +378,7 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
                 default: op_swap2<2, 3>(a, pr, pi, pure, I.cmask, L); break;
                 }
             } else if (op == I_RESET) {
-                switch (I.s[0]) {
+                switch (sw & 255u) {
                 case 0: op_reset<0>(a); break;
                 case 1: op_reset<1>(a); break;
                 case 2: op_reset<2>(a); break;
@@ -438,12 +501,13 @@ int run_program(State &st, const DeviceProgram &dp, hipStream_t s) {
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s) {
     if (eops.empty()) return GOQU_OK;
     Program P = plan_program(eops, st.nq, default_plan_config(st.prec, st.nq));
-    DeviceProgram dp;
+    // grow-only program buffer per host thread: every call ends synchronised, so the next call
+    // on this thread may overwrite it
+    static thread_local DeviceProgram dp;
     int rc = upload_program(P, st.prec, s, dp);
     if (rc) return rc;
     rc = run_program(st, dp, s);
     hipError_t e = hipStreamSynchronize(s);
-    hipFree(dp.blob);
     if (rc) return rc;
     if (e != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("stage kernels: ") + hipGetErrorString(e));
     return GOQU_OK;

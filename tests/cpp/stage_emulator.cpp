@@ -24,18 +24,24 @@ struct Lay {
     uint32_t rl[4];
 };
 
-void set_layout(Lay &L, const uint8_t *lay, const uint8_t *T, int m, uint32_t tid, uint64_t base) {
+// reads the global qubit of each entry from gq[] (as the kernel does) and checks it
+// against the tile's qubit list
+bool set_layout(Lay &L, const uint8_t *lay, const uint8_t *gq, const uint8_t *T, int m, uint32_t tid,
+                uint64_t base) {
+    bool ok = true;
+    for (int i = 0; i < m; i++) ok &= gq[i] == T[lay[i]];
     for (int s = 0; s < 4; s++) {
         L.rl[s] = 1u << lay[s];
-        L.rq[s] = 1ull << T[lay[s]];
+        L.rq[s] = 1ull << gq[s];
     }
     L.tg = base;
     L.tl = 0;
     for (int u = 0; u < m - 4; u++)
         if (tid >> u & 1) {
             L.tl |= 1u << lay[4 + u];
-            L.tg |= 1ull << T[lay[4 + u]];
+            L.tg |= 1ull << gq[4 + u];
         }
+    return ok;
 }
 
 uint64_t rg(const Lay &L, int r) {
@@ -84,7 +90,10 @@ int run(const Program &P, cd *st, int nq, std::string &err) {
             std::vector<Lay> L(nthr);
             std::vector<cd> a(size_t(nthr) * 16);
             for (int t = 0; t < nthr; t++) {
-                set_layout(L[t], H.lay0, H.T, m, t, base);
+                if (!set_layout(L[t], H.lay0, H.gq0, H.T, m, t, base)) {
+                    err = "layout global qubits disagree with the tile list";
+                    return 1;
+                }
                 for (int r = 0; r < 16; r++) {
                     const uint64_t g = L[t].tg | rg(L[t], r);
                     if (g >= N) {
@@ -110,7 +119,10 @@ int run(const Program &P, cd *st, int nq, std::string &err) {
                             lds[l] = a[t * 16 + r];
                         }
                     for (int t = 0; t < nthr; t++) {
-                        set_layout(L[t], I.lay, H.T, m, t, base);
+                        if (!set_layout(L[t], I.lay, I.gq, H.T, m, t, base)) {
+                            err = "remap global qubits disagree with the tile list";
+                            return 1;
+                        }
                         for (int r = 0; r < 16; r++) a[t * 16 + r] = lds[L[t].tl | rlo(L[t], r)];
                     }
                     continue;
