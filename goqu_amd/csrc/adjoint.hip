@@ -3,8 +3,10 @@
 //
 // Statevector: psi = U_N..U_1|0>, lam = H psi, E = Re<psi|lam>.  Going backwards,
 // psi_{k-1} = U_k^dag psi_k and lam_{k-1} = U_k^dag lam_k; for every parameter entering U_k,
-// dE/dtheta += scale * 2 Re <lam_k| dU_k |psi_{k-1}>.  Runs of ops without parameters are
-// uncomputed as one fused program (a few stage sweeps) instead of op by op.
+// dE/dtheta += scale * 2 Re <lam_k| dU_k |psi_{k-1}>.  The reverse sweep is one planned
+// program run by the paired stage kernel, which moves psi and lam through the same tiles and
+// reduces every derivative inside the stage where its op is uncomputed.  Derivatives on more
+// than two target qubits fall back to uncomputing op by op with one reduction each.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -15,51 +17,6 @@
 namespace goqu {
 
 using cd = std::complex<double>;
-
-static int invert_eop(const EOp &e, EOp &inv, std::string &err) {
-    inv = e;
-    switch (e.kind) {
-    case EOP_DENSE: {
-        const int d = 1 << e.k;
-        for (int r = 0; r < d; r++)
-            for (int c = 0; c < d; c++) {
-                inv.m[2 * (r * d + c)] = e.m[2 * (c * d + r)];
-                inv.m[2 * (r * d + c) + 1] = -e.m[2 * (c * d + r) + 1];
-            }
-        return GOQU_OK;
-    }
-    case EOP_DIAG:
-        for (size_t i = 1; i < inv.m.size(); i += 2) inv.m[i] = -inv.m[i];
-        return GOQU_OK;
-    case EOP_ANTI1:  // n0 = p0 a1, n1 = p1 a0  =>  a0 = conj(p1) n1, a1 = conj(p0) n0
-        inv.m = {e.m[2], -e.m[3], e.m[0], -e.m[1]};
-        return GOQU_OK;
-    case EOP_SWAP2:
-        inv.m = {e.m[0], -e.m[1]};
-        return GOQU_OK;
-    default:
-        err = "adjoint differentiation needs unitary ops (reset is not invertible)";
-        return GOQU_ERR_UNSUPPORTED;
-    }
-}
-
-// The op's derivative as a dense k-qubit matrix on its targets (controls stay controls).
-static int deriv_eop(const goqu_op_t &op, const double *dm, EOp &out, std::string &err) {
-    out = EOp();
-    out.kind = EOP_DENSE;
-    int nc = op.kind == GOQU_OP_CONTROLLED ? op.num_controls : 0;
-    const int nt = op.num_qubits - nc;
-    if (nt < 1 || nt > 4) {
-        err = "derivative of an op with more than 4 target qubits";
-        return GOQU_ERR_UNSUPPORTED;
-    }
-    out.k = nt;
-    for (int j = 0; j < nt; j++) out.q[j] = op.qubits[nc + j];
-    for (int j = 0; j < nc; j++) out.cmask |= 1ull << op.qubits[j];
-    const int d = 1 << nt;
-    out.m.assign(dm, dm + 2 * d * d);
-    return GOQU_OK;
-}
 
 static void ipow(int n, double &re, double &im) {
     switch (((n % 4) + 4) % 4) {
@@ -146,6 +103,26 @@ int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goq
     *energy = e2[0];
     for (int p = 0; p < np; p++) grads[p] = 0;
 
+    // Fused reverse pass: one program, run on (psi, lam) together, in which every derivative is
+    // an EOP_GRAD op placed right before the inverse of its op.
+    std::vector<EOp> rev;
+    bool fused = false;
+    rc = build_sv_reverse(ops, nops, nq_user, derivs, nd, rev, fused, err);
+    if (!rc && fused) {
+        std::vector<double> vals;
+        rc = run_pair(work, lam, rev, nd, vals, s);
+        if (!rc)
+            for (int i = 0; i < nd; i++) grads[derivs[i].param] += derivs[i].scale * 2.0 * vals[i];
+    } else if (rc) {
+        rc = fail(rc, err);
+    }
+    if (rc || fused) {
+        cleanup();
+        return rc;
+    }
+
+    // Op by op (derivatives on more than two targets): uncompute pending runs, then one
+    // sandwich reduction per derivative.
     std::vector<EOp> pend_psi, pend_lam;
     for (int k = nops - 1; k >= 0 && !rc; k--) {
         std::vector<EOp> inv;

@@ -10,6 +10,8 @@
 #include <string>
 #include <vector>
 
+#include "../../include/goqu_hip.h"
+
 namespace goqu {
 
 // Register slots per thread: each thread keeps 2^kRegBits amplitudes in registers.
@@ -27,7 +29,11 @@ enum EOpKind : int32_t {
     EOP_ANTI1 = 2,  // 1 target: [a0, a1] <- [p0 * a1, p1 * a0]   (X, Y, CNOT, CY, CCX ...)
     EOP_SWAP2 = 3,  // 2 targets: |01> <- p * |10>, |10> <- p * |01>  (SWAP, iSWAP, CSWAP)
     EOP_RESET = 4,  // 1 target: deterministic reset (a0 <- |(a0,a1)|, a1 <- 0)
+    EOP_GRAD = 5,   // paired states only: Re <lambda| D |psi> on k targets into gradient slot `slot`
 };
+
+// Gradient slots a single stage may hold (per-wave partials live in LDS).
+constexpr int kMaxGradPerStage = 64;
 
 struct EOp {
     int32_t kind = EOP_DENSE;
@@ -36,8 +42,10 @@ struct EOp {
     uint64_t cmask = 0;   // control qubits (all must be 1)
     std::vector<double> m;  // interleaved complex payload (dense matrix / diag / phases)
     bool pure_swap = false; // ANTI1 / SWAP2 with unit phases: move only
+    int32_t slot = -1;      // EOP_GRAD: global gradient slot
     // which of the q[] are targets that must sit in registers (DENSE, ANTI1, SWAP2, RESET)
     bool needs_regs() const { return kind != EOP_DIAG; }
+    bool is_grad() const { return kind == EOP_GRAD; }
     uint64_t touch() const {
         uint64_t t = cmask;
         for (int i = 0; i < k; i++) t |= 1ull << q[i];
@@ -63,6 +71,7 @@ enum InstrOp : int32_t {
     I_SWAP2 = 3,  // slots s[0], s[1]
     I_RESET = 4,  // slot s[0]
     I_REMAP = 5,  // new layout lay[0..m-1]: lay[0..3] register local bits, rest thread bits
+    I_GRAD = 6,   // like I_DENSE, but accumulates Re <b| M |a> into local gradient slot `pad`
 };
 
 // Byte arrays below are read by the kernel as aligned 32-bit words (scalar loads).
@@ -74,8 +83,8 @@ struct alignas(16) Instr {
     uint64_t cmask;   // control mask on global qubits
     uint8_t s[4];     // register slots
     uint8_t q[4];     // global qubits (I_DIAG), qubits[0] = MSB of the diag index
-    int32_t smask;    // I_DENSE: register slot mask (dense matrix is in slot-bit order)
-    int32_t pad;
+    int32_t smask;    // I_DENSE / I_GRAD: register slot mask (matrix in slot-bit order)
+    int32_t pad;      // I_GRAD: gradient slot within the stage
     uint8_t lay[16];  // I_REMAP: new layout, tile-local bit of register slots 0..3 then thread bits
     uint8_t gq[16];   // I_REMAP: the global qubit of each lay[] entry
     uint8_t pad2[16];
@@ -91,8 +100,11 @@ struct alignas(16) StageHeader {
     uint8_t lay0[16];     // initial layout (local bits: 4 register slots then thread bits)
     uint8_t gq0[16];      // global qubits of lay0
     uint8_t layN[16];     // final layout
+    int32_t grad_base;    // global gradient slot of local slot 0
+    int32_t ngrad;        // gradient slots of this stage
+    int32_t pad[2];
 };
-static_assert(sizeof(StageHeader) == 80, "StageHeader layout");
+static_assert(sizeof(StageHeader) == 96, "StageHeader layout");
 
 struct Program {
     int nq = 0;           // internal qubit count
@@ -100,6 +112,8 @@ struct Program {
     std::vector<Instr> instrs;
     std::vector<double> pool;  // interleaved complex
     int64_t num_eops = 0;
+    int32_t num_grad = 0;      // gradient slots over all stages
+    std::vector<int32_t> grad_slots;  // program gradient slot -> the EOp's `slot`
 };
 
 struct PlanConfig {
@@ -118,5 +132,18 @@ int lower_dm_ops(const void *ops, int nops, int n, std::vector<EOp> &out, std::s
 
 // Greedy stage planner + register allocation.
 Program plan_program(const std::vector<EOp> &eops, int nq_internal, const PlanConfig &cfg);
+
+// ---- adjoint differentiation programs (adjoint_plan.cpp) ----------------------------------
+// conjugate transpose of a unitary EOp (RESET has none)
+int invert_eop(const EOp &e, EOp &inv, std::string &err);
+// dense derivative op dU of a goqu_op_t on its targets (controls kept), from `dm`
+int deriv_eop(const goqu_op_t &op, const double *dm, EOp &out, std::string &err);
+// EOP_GRAD op D = dU U^dag of one derivative (false: more than two targets or no matrix)
+bool grad_eop(const goqu_op_t &op, const double *dm, int slot, EOp &out);
+// The reverse sweep of a statevector adjoint as one program on (psi, lam): for k = nops-1..0,
+// the EOP_GRAD ops of op k's derivatives (slot = derivative index), then the inverse of op k.
+// fused = false (and rev empty) when some derivative does not fit an EOP_GRAD op.
+int build_sv_reverse(const goqu_op_t *ops, int nops, int nq_user, const goqu_deriv_t *derivs, int nd,
+                     std::vector<EOp> &rev, bool &fused, std::string &err);
 
 }  // namespace goqu

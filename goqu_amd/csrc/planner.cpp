@@ -442,6 +442,7 @@ namespace {
 struct StageOps {
     uint64_t T = 0;
     std::vector<int> ops;
+    int ngrad = 0;
 };
 
 // Register allocation inside one stage.  Emits instructions into prog.
@@ -493,11 +494,12 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             const uint64_t nd = e.nondiag(), dg = e.diag(), tg = e.targets();
             const bool conflict = (nd & (b_nd | b_dg)) || (dg & b_nd);
             bool take = false;
-            if (!conflict) {
+            if (!conflict && !(e.is_grad() && S.ngrad >= kMaxGradPerStage)) {
                 const uint64_t nt = S.T | tg;
                 if (__builtin_popcountll(nt) <= m) {
                     S.T = nt;
                     S.ops.push_back((int)i);
+                    S.ngrad += e.is_grad();
                     done[i] = 1;
                     take = true;
                 }
@@ -521,6 +523,8 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         H.m = m;
         H.instr_off = (int32_t)P.instrs.size();
         H.pool_off = (int32_t)(P.pool.size() / 2);
+        H.grad_base = P.num_grad;
+        H.ngrad = 0;
         int loc[64];
         for (int i = 0; i < 64; i++) loc[i] = -1;
         {
@@ -652,8 +656,14 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
                 break;
             }
+            case EOP_GRAD:
             case EOP_DENSE: {
-                I.op = I_DENSE;
+                I.op = e.kind == EOP_GRAD ? I_GRAD : I_DENSE;
+                if (e.kind == EOP_GRAD) {
+                    I.pad = H.ngrad++;
+                    P.grad_slots.push_back(e.slot);
+                    P.num_grad++;
+                }
                 const int k = e.k, dim = 1 << k;
                 int slot[4], smask = 0;
                 for (int j = 0; j < k; j++) {

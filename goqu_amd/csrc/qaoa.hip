@@ -463,10 +463,12 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     size_t lds_small = 0;
     for (int id : small_ids)
         lds_small = std::max<size_t>(lds_small, (size_t(2) << insts[id].n) * sizeof(c2d) + (kQMaxEdges + 1) * sizeof(c2d));
-    GOQU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_qaoa_small),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    // dynamic LDS above the 64 KiB default (static LDS of the kernel comes on top)
+    if (lds_small)
+        GOQU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_qaoa_small),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_small));
     GOQU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_qaoa_phase),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_large));
     // capture every launch in one graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

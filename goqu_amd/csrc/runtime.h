@@ -20,7 +20,8 @@ int fail(int code, const std::string &msg);
     do {                                                                                      \
         hipError_t e_ = (call);                                                               \
         if (e_ != hipSuccess)                                                                 \
-            return ::goqu::fail(e_ == hipErrorOutOfMemory ? GOQU_ERR_OUT_OF_MEMORY : GOQU_ERR_DEVICE, \
+            return (void)hipGetLastError(), /* clear the sticky per-thread error */          \
+                   ::goqu::fail(e_ == hipErrorOutOfMemory ? GOQU_ERR_OUT_OF_MEMORY : GOQU_ERR_DEVICE, \
                                 std::string(#call) + ": " + hipGetErrorString(e_));          \
     } while (0)
 
@@ -58,6 +59,9 @@ PlanConfig default_plan_config(int prec, int nq);
 int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp);
 int run_program(State &st, const DeviceProgram &dp, hipStream_t s);
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s);  // plan + upload + run
+// ops applied to psi and lam together; EOP_GRAD ops accumulate Re <lam| D |psi> per slot
+int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, std::vector<double> &slot_vals,
+             hipStream_t s);
 
 int fill_basis(State &st, uint64_t index, hipStream_t s);  // |index>
 int copy_state(State &dst, const State &src, hipStream_t s);

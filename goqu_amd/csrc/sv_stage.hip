@@ -275,24 +275,61 @@ __device__ __forceinline__ void op_diag(cx<R> (&a)[16], const Instr &I, const R 
     }
 }
 
-#define DENSE_CASE(MK) \
-    case MK:           \
-        op_dense<MK>(a, M, I.cmask, L); \
-        break;
+// Re sum over the op's groups of conj(b) (M a): the derivative term of the adjoint method.
+template <int MASK, typename R>
+__device__ __forceinline__ double op_grad(const cx<R> (&a)[16], const cx<R> (&b)[16], const R *__restrict__ M,
+                                          uint64_t cm, const Lay &L) {
+    constexpr int K = popc4(MASK);
+    constexpr int D = 1 << K;
+    constexpr int OUTER = 16 >> K;
+    constexpr int OM = 15 & ~MASK;
+    const Ctrl c = ctrl_split(cm, L);
+    double acc = 0;
+    static_for<0, OUTER>([&](auto oc) __attribute__((always_inline)) {
+        constexpr int base = dep4(decltype(oc)::value, OM);
+        if (c.tok && ((c.pat >> base) & 1u)) {
+            static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
+                constexpr int i = decltype(ic)::value;
+                R re = 0, im = 0;
+                static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    const cx<R> x = a[base | dep4(j, MASK)];
+                    const R mr = M[2 * (i * D + j)], mi = M[2 * (i * D + j) + 1];
+                    re = fma(mr, x.x, re);
+                    re = fma(-mi, x.y, re);
+                    im = fma(mr, x.y, im);
+                    im = fma(mi, x.x, im);
+                });
+                const cx<R> bb = b[base | dep4(i, MASK)];
+                acc += (double)bb.x * (double)re + (double)bb.y * (double)im;
+            });
+        }
+    });
+    return acc;
+}
 
-template <typename R>
-__global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
-                                               const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                               uint64_t ntiles) {
+// One stage over one state (PAIR = false) or over psi and lambda together (PAIR = true: every op
+// acts on both, and I_GRAD accumulates Re <lambda| M |psi> per gradient slot).
+template <typename R, bool PAIR>
+__device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__restrict__ st2,
+                                           const StageHeader *__restrict__ hdr, const Instr *__restrict__ prog,
+                                           const R *__restrict__ pool, uint64_t ntiles, double *__restrict__ gpart,
+                                           double *gacc) {
     extern __shared__ unsigned char smem_raw[];
     cx<R> *lds = reinterpret_cast<cx<R> *>(smem_raw);
     const int m = hdr->m;
     const int ninstr = hdr->ninstr;
     const uint32_t tid = threadIdx.x;
     const Instr *__restrict__ P = prog + hdr->instr_off;
-
     const uint32_t *Tw = reinterpret_cast<const uint32_t *>(hdr->T);
     const uint32_t T0 = Tw[0], T1 = Tw[1], T2 = Tw[2], T3 = Tw[3];
+    const int wave = tid >> 6, lane = tid & 63;
+    int ngrad = 0;
+    if constexpr (PAIR) {
+        ngrad = hdr->ngrad;
+        for (int i = tid; i < ngrad * 8; i += blockDim.x) gacc[i] = 0;
+        __syncthreads();
+    }
 
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         // deposit the tile index into the qubits outside the tile (T ascending)
@@ -307,10 +344,11 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
         }
         Lay L;
         set_layout(L, hdr->lay0, hdr->gq0, m, tid, base);
-        cx<R> a[16];
+        cx<R> a[16], b[PAIR ? 16 : 1];
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             a[r] = st[L.tg | rgof<r>(L)];
+            if constexpr (PAIR) b[r] = st2[L.tg | rgof<r>(L)];
         });
 
         for (int pc = 0; pc < ninstr; pc++) {
@@ -325,13 +363,32 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
                     lds[swz<R>(L.tl | rlof<r>(L))] = a[r];
                 });
                 __syncthreads();
-                set_layout(L, I.lay, I.gq, m, tid, base);
+                Lay Ln;
+                set_layout(Ln, I.lay, I.gq, m, tid, base);
                 static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
                     constexpr int r = decltype(rc)::value;
-                    a[r] = lds[swz<R>(L.tl | rlof<r>(L))];
+                    a[r] = lds[swz<R>(Ln.tl | rlof<r>(Ln))];
                 });
+                if constexpr (PAIR) {
+                    __syncthreads();
+                    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                        constexpr int r = decltype(rc)::value;
+                        lds[swz<R>(L.tl | rlof<r>(L))] = b[r];
+                    });
+                    __syncthreads();
+                    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                        constexpr int r = decltype(rc)::value;
+                        b[r] = lds[swz<R>(Ln.tl | rlof<r>(Ln))];
+                    });
+                }
+                L = Ln;
             } else if (op == I_DENSE) {
                 const R *__restrict__ M = pool + 2 * I.mat;
+#define DENSE_CASE(MK)                                     \
+    case MK:                                               \
+        op_dense<MK>(a, M, I.cmask, L);                    \
+        if constexpr (PAIR) op_dense<MK>(b, M, I.cmask, L); \
+        break;
                 switch (I.smask) {
                     DENSE_CASE(1)
                     DENSE_CASE(2)
@@ -351,32 +408,50 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
                 default:
                     break;
                 }
+#undef DENSE_CASE
             } else if (op == I_DIAG) {
                 op_diag<R>(a, I, pool + 2 * I.mat, L);
+                if constexpr (PAIR) op_diag<R>(b, I, pool + 2 * I.mat, L);
             } else if (op == I_ANTI1) {
                 const R *__restrict__ p = pool + 2 * I.mat;
                 const bool pure = I.flags & 1;
                 const R p0r = p[0], p0i = p[1], p1r = p[2], p1i = p[3];
+#define ANTI_CASE(S)                                                                      \
+    case S:                                                                               \
+        op_anti1<S>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L);                             \
+        if constexpr (PAIR) op_anti1<S>(b, p0r, p0i, p1r, p1i, pure, I.cmask, L);         \
+        break;
                 switch (sw & 255u) {
-                case 0: op_anti1<0>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
-                case 1: op_anti1<1>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
-                case 2: op_anti1<2>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
-                default: op_anti1<3>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L); break;
+                    ANTI_CASE(0)
+                    ANTI_CASE(1)
+                    ANTI_CASE(2)
+                    ANTI_CASE(3)
+                default:
+                    break;
                 }
+#undef ANTI_CASE
             } else if (op == I_SWAP2) {
                 const R *__restrict__ p = pool + 2 * I.mat;
                 const bool pure = I.flags & 1;
                 const R pr = p[0], pi = p[1];
                 const int sa = sw & 255u, sb = (sw >> 8) & 255u;
                 const int s0 = sa < sb ? sa : sb, s1 = sa < sb ? sb : sa;
+#define SWAP_CASE(A, B)                                                    \
+    case A * 4 + B:                                                        \
+        op_swap2<A, B>(a, pr, pi, pure, I.cmask, L);                       \
+        if constexpr (PAIR) op_swap2<A, B>(b, pr, pi, pure, I.cmask, L);   \
+        break;
                 switch (s0 * 4 + s1) {
-                case 1: op_swap2<0, 1>(a, pr, pi, pure, I.cmask, L); break;
-                case 2: op_swap2<0, 2>(a, pr, pi, pure, I.cmask, L); break;
-                case 3: op_swap2<0, 3>(a, pr, pi, pure, I.cmask, L); break;
-                case 6: op_swap2<1, 2>(a, pr, pi, pure, I.cmask, L); break;
-                case 7: op_swap2<1, 3>(a, pr, pi, pure, I.cmask, L); break;
-                default: op_swap2<2, 3>(a, pr, pi, pure, I.cmask, L); break;
+                    SWAP_CASE(0, 1)
+                    SWAP_CASE(0, 2)
+                    SWAP_CASE(0, 3)
+                    SWAP_CASE(1, 2)
+                    SWAP_CASE(1, 3)
+                    SWAP_CASE(2, 3)
+                default:
+                    break;
                 }
+#undef SWAP_CASE
             } else if (op == I_RESET) {
                 switch (sw & 255u) {
                 case 0: op_reset<0>(a); break;
@@ -384,14 +459,69 @@ __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const Sta
                 case 2: op_reset<2>(a); break;
                 default: op_reset<3>(a); break;
                 }
+            } else if (op == I_GRAD) {
+                if constexpr (PAIR) {
+                    const R *__restrict__ M = pool + 2 * I.mat;
+                    double v = 0;
+#define GRAD_CASE(MK)                                \
+    case MK:                                         \
+        v = op_grad<MK>(a, b, M, I.cmask, L);        \
+        break;
+                    switch (I.smask) {
+                        GRAD_CASE(1)
+                        GRAD_CASE(2)
+                        GRAD_CASE(3)
+                        GRAD_CASE(4)
+                        GRAD_CASE(5)
+                        GRAD_CASE(6)
+                        GRAD_CASE(8)
+                        GRAD_CASE(9)
+                        GRAD_CASE(10)
+                        GRAD_CASE(12)
+                    default:
+                        break;
+                    }
+#undef GRAD_CASE
+                    // blocks of fewer than 64 threads only read active lanes
+#pragma unroll
+                    for (int off = 32; off >= 1; off >>= 1)
+                        if (off < (int)blockDim.x) v += __shfl_xor(v, off, 64);
+                    if (lane == 0) gacc[I.pad * 8 + wave] += v;  // one writer per (slot, wave)
+                }
             }
         }
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             st[L.tg | rgof<r>(L)] = a[r];
+            if constexpr (PAIR) st2[L.tg | rgof<r>(L)] = b[r];
         });
         if (tile + gridDim.x < ntiles) __syncthreads();
     }
+    if constexpr (PAIR) {
+        __syncthreads();
+        const int nw = blockDim.x >> 6;
+        for (int g = tid; g < ngrad; g += blockDim.x) {
+            double s = 0;
+            for (int w = 0; w < nw; w++) s += gacc[g * 8 + w];
+            gpart[(size_t)g * gridDim.x + blockIdx.x] = s;
+        }
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+                                               const Instr *__restrict__ prog, const R *__restrict__ pool,
+                                               uint64_t ntiles) {
+    stage_body<R, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<R> *__restrict__ lam,
+                                                    const StageHeader *__restrict__ hdr,
+                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
+                                                    uint64_t ntiles, double *__restrict__ gpart) {
+    __shared__ double gacc[kMaxGradPerStage * 8];
+    stage_body<R, true>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
 }
 
 // ---- profiling ---------------------------------------------------------------------------------
@@ -452,12 +582,6 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
     const StageHeader *hdrs = reinterpret_cast<const StageHeader *>(blob + dp.hdr_off);
     const Instr *instrs = reinterpret_cast<const Instr *>(blob + dp.instr_off);
     const R *pool = reinterpret_cast<const R *>(blob + dp.pool_off);
-    static bool attr_done = false;
-    if (!attr_done) {
-        GOQU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_stage<R>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_done = true;
-    }
     const bool prof = g_profile.load();
     std::vector<hipEvent_t> ev;
     for (size_t i = 0; i < dp.stages.size(); i++) {
@@ -496,6 +620,83 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
 
 int run_program(State &st, const DeviceProgram &dp, hipStream_t s) {
     return st.prec == GOQU_C128 ? run_program_t<double>(st, dp, s) : run_program_t<float>(st, dp, s);
+}
+
+// sums[slot] = sum over b of part[slot * nb + b], in a fixed order
+__global__ void __launch_bounds__(256) k_sum_rows(const double *__restrict__ part, int64_t nb,
+                                                  double *__restrict__ sums) {
+    __shared__ double sh[256];
+    const double *row = part + (size_t)blockIdx.x * nb;
+    double v = 0;
+    for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) v += row[b];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+        for (int i = 0; i < (int)blockDim.x; i++) t += sh[i];
+        sums[blockIdx.x] = t;
+    }
+}
+
+template <typename R>
+static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, std::vector<double> &slot_vals) {
+    static thread_local DeviceProgram dp;
+    int rc = upload_program(P, psi.prec, s, dp);
+    if (rc) return rc;
+    const unsigned char *blob = static_cast<const unsigned char *>(dp.blob);
+    const StageHeader *hdrs = reinterpret_cast<const StageHeader *>(blob + dp.hdr_off);
+    const Instr *instrs = reinterpret_cast<const Instr *>(blob + dp.instr_off);
+    const R *pool = reinterpret_cast<const R *>(blob + dp.pool_off);
+    const int m = P.stages.empty() ? 0 : P.stages[0].m;
+    if (m > 12) return fail(GOQU_ERR_UNSUPPORTED, "paired stages need tiles of at most 2^12 amplitudes");
+    const uint64_t ntiles = 1ull << (psi.nq - m);
+    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 1ull << 20);
+    const int ng = P.num_grad;
+    // per-stage partials (slot-major, one per workgroup), reduced after every stage into gsum
+    double *gpart = nullptr, *gsum = nullptr;
+    if (ng) {
+        GOQU_HIP(hipMalloc(&gpart, ((size_t)kMaxGradPerStage * grid + ng) * sizeof(double)));
+        gsum = gpart + (size_t)kMaxGradPerStage * grid;
+    }
+    for (size_t i = 0; i < P.stages.size() && !rc; i++) {
+        const StageHeader &H = P.stages[i];
+        if (H.ninstr == 0) continue;
+        const unsigned threads = 1u << (m - kRegBits);
+        const size_t lds = (size_t(1) << m) * sizeof(cx<R>);
+        hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),
+                           static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
+        if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "paired stage launch failed");
+        if (!rc && H.ngrad) {
+            hipLaunchKernelGGL(k_sum_rows, dim3(H.ngrad), dim3(256), 0, s, gpart, (int64_t)grid, gsum + H.grad_base);
+            if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "gradient reduction launch failed");
+        }
+    }
+    slot_vals.assign(ng, 0.0);
+    hipError_t e = hipSuccess;
+    if (!rc && ng) e = hipMemcpyAsync(slot_vals.data(), gsum, ng * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (gpart) (void)hipFree(gpart);
+    if (!rc && e != hipSuccess) rc = fail(GOQU_ERR_DEVICE, std::string("paired stages: ") + hipGetErrorString(e));
+    return rc;
+}
+
+// Plans `eops` (which may hold EOP_GRAD ops) and runs it on psi and lam together; slot_vals
+// receives Re <lam| D |psi> per gradient slot, indexed like the EOps' `slot` (slots absent from
+// the program stay 0).
+int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, std::vector<double> &slot_vals,
+             hipStream_t s) {
+    slot_vals.assign(nslots, 0.0);
+    if (eops.empty()) return GOQU_OK;
+    Program P = plan_program(eops, psi.nq, default_plan_config(psi.prec, psi.nq));
+    std::vector<double> prog_vals;
+    int rc = psi.prec == GOQU_C128 ? run_pair_t<double>(psi, lam, P, s, prog_vals)
+                                   : run_pair_t<float>(psi, lam, P, s, prog_vals);
+    if (rc) return rc;
+    for (size_t j = 0; j < prog_vals.size(); j++) {
+        const int slot = P.grad_slots[j];
+        if (slot >= 0 && slot < nslots) slot_vals[slot] += prog_vals[j];
+    }
+    return GOQU_OK;
 }
 
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s) {

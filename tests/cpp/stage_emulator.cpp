@@ -66,7 +66,10 @@ int dep4(int v, int mask) {
     return r;
 }
 
-int run(const Program &P, cd *st, int nq, std::string &err) {
+// st2 != nullptr: paired mode (every op on both states; I_GRAD adds Re <st2| M |st> of each
+// tile to vals[grad_base + local slot])
+int run(const Program &P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vector<double> *vals = nullptr) {
+    if (st2 && vals) vals->assign(P.num_grad, 0.0);
     const uint64_t N = 1ull << nq;
     for (const StageHeader &H : P.stages) {
         const int m = H.m;
@@ -88,7 +91,7 @@ int run(const Program &P, cd *st, int nq, std::string &err) {
                 base = ((base >> t) << (t + 1)) | (base & ((1ull << t) - 1));
             }
             std::vector<Lay> L(nthr);
-            std::vector<cd> a(size_t(nthr) * 16);
+            std::vector<cd> a(size_t(nthr) * 16), b2(st2 ? size_t(nthr) * 16 : 0);
             for (int t = 0; t < nthr; t++) {
                 if (!set_layout(L[t], H.lay0, H.gq0, H.T, m, t, base)) {
                     err = "layout global qubits disagree with the tile list";
@@ -101,12 +104,14 @@ int run(const Program &P, cd *st, int nq, std::string &err) {
                         return 1;
                     }
                     a[t * 16 + r] = st[g];
+                    if (st2) b2[t * 16 + r] = st2[g];
                 }
             }
             for (int pc = 0; pc < H.ninstr; pc++) {
                 const Instr &I = P.instrs[H.instr_off + pc];
                 const double *pool = P.pool.data() + 2 * I.mat;
                 if (I.op == I_REMAP) {
+                    std::vector<cd> lds2(st2 ? lds.size() : 0);
                     std::fill(seen.begin(), seen.end(), 0);
                     for (int t = 0; t < nthr; t++)
                         for (int r = 0; r < 16; r++) {
@@ -117,21 +122,55 @@ int run(const Program &P, cd *st, int nq, std::string &err) {
                             }
                             seen[l] = 1;
                             lds[l] = a[t * 16 + r];
+                            if (st2) lds2[l] = b2[t * 16 + r];
                         }
                     for (int t = 0; t < nthr; t++) {
                         if (!set_layout(L[t], I.lay, I.gq, H.T, m, t, base)) {
                             err = "remap global qubits disagree with the tile list";
                             return 1;
                         }
-                        for (int r = 0; r < 16; r++) a[t * 16 + r] = lds[L[t].tl | rlo(L[t], r)];
+                        for (int r = 0; r < 16; r++) {
+                            a[t * 16 + r] = lds[L[t].tl | rlo(L[t], r)];
+                            if (st2) b2[t * 16 + r] = lds2[L[t].tl | rlo(L[t], r)];
+                        }
                     }
                     continue;
                 }
-                for (int t = 0; t < nthr; t++) {
-                    cd *A = &a[t * 16];
+                for (int t = 0; t < nthr; t++)
+                  for (int which = 0; which < (st2 ? 2 : 1); which++) {
+                    cd *A = which ? &b2[t * 16] : &a[t * 16];
                     const Lay &LL = L[t];
                     auto ok = [&](int r) { return ((LL.tg | rg(LL, r)) & I.cmask) == I.cmask; };
+                    if (I.op == I_RESET && which) break;  // the kernel resets only the first state
                     switch (I.op) {
+                    case I_GRAD: {
+                        if (!st2 || !vals) {
+                            err = "I_GRAD outside a paired program";
+                            return 1;
+                        }
+                        if (which) break;
+                        const cd *B = &b2[t * 16];
+                        const int MASK = I.smask, K = __builtin_popcount(MASK), D = 1 << K;
+                        if (K < 1 || K > 2) {
+                            err = "I_GRAD on more than two slots";
+                            return 1;
+                        }
+                        if (H.grad_base + I.pad >= P.num_grad || I.pad >= H.ngrad) {
+                            err = "gradient slot out of range";
+                            return 1;
+                        }
+                        for (int o = 0; o < (16 >> K); o++) {
+                            const int bb = dep4(o, 15 & ~MASK);
+                            if (!ok(bb)) continue;
+                            for (int i = 0; i < D; i++) {
+                                cd y = 0;
+                                for (int j = 0; j < D; j++)
+                                    y += cd(pool[2 * (i * D + j)], pool[2 * (i * D + j) + 1]) * A[bb | dep4(j, MASK)];
+                                (*vals)[H.grad_base + I.pad] += std::real(std::conj(B[bb | dep4(i, MASK)]) * y);
+                            }
+                        }
+                        break;
+                    }
                     case I_DENSE: {
                         const int MASK = I.smask, K = __builtin_popcount(MASK), D = 1 << K;
                         for (int o = 0; o < (16 >> K); o++) {
@@ -198,7 +237,10 @@ int run(const Program &P, cd *st, int nq, std::string &err) {
                 }
             }
             for (int t = 0; t < nthr; t++)
-                for (int r = 0; r < 16; r++) st[L[t].tg | rg(L[t], r)] = a[t * 16 + r];
+                for (int r = 0; r < 16; r++) {
+                    st[L[t].tg | rg(L[t], r)] = a[t * 16 + r];
+                    if (st2) st2[L[t].tg | rg(L[t], r)] = b2[t * 16 + r];
+                }
         }
     }
     return 0;
@@ -229,6 +271,51 @@ int emu_apply(double *state, int nq_internal, int nq_user, const goqu_op_t *ops,
     return rc ? 100 : 0;
 }
 const char *emu_error(void) { return g_err.c_str(); }
+
+// The statevector adjoint's reverse sweep (build_sv_reverse + planner) on psi and lam in place;
+// vals[i] = Re <lam_k| D_i |psi_k> of derivative i.  Returns 200 when the derivatives do not
+// fit the fused program.
+int emu_sv_reverse(double *psi, double *lam, int nq_internal, int nq_user, const goqu_op_t *ops, int nops,
+                   const goqu_deriv_t *derivs, int nd, int tile_bits, int low_bits, double *vals,
+                   int64_t *num_stages) {
+    std::vector<EOp> rev;
+    bool fused = false;
+    std::string err;
+    int rc = build_sv_reverse(ops, nops, nq_user, derivs, nd, rev, fused, err);
+    if (rc) {
+        g_err = err;
+        return rc;
+    }
+    if (!fused) {
+        g_err = "not fusable";
+        return 200;
+    }
+    PlanConfig cfg;
+    cfg.tile_bits = tile_bits;
+    cfg.low_bits = low_bits;
+    Program P = plan_program(rev, nq_internal, cfg);
+    if (num_stages) *num_stages = (int64_t)P.stages.size();
+    std::vector<double> pv;
+    rc = run(P, reinterpret_cast<cd *>(psi), nq_internal, err, reinterpret_cast<cd *>(lam), &pv);
+    if (rc) {
+        g_err = err;
+        return 100;
+    }
+    for (int i = 0; i < nd; i++) vals[i] = 0;
+    if ((int)P.grad_slots.size() != P.num_grad) {
+        g_err = "grad_slots size mismatch";
+        return 100;
+    }
+    for (int j = 0; j < P.num_grad; j++) {
+        const int slot = P.grad_slots[j];
+        if (slot < 0 || slot >= nd) {
+            g_err = "gradient slot outside the derivative list";
+            return 100;
+        }
+        vals[slot] += pv[j];
+    }
+    return 0;
+}
 }
 
 extern "C" int emu_plan_stats(const goqu_op_t *ops, int nops, int nq, int tile_bits, int low_bits, int64_t *stages,

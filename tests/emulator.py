@@ -10,7 +10,8 @@ import numpy as np
 from goqu_amd import _native as N
 
 HERE = Path(__file__).resolve().parent / "cpp"
-SRC = [HERE / "stage_emulator.cpp", HERE.parent.parent / "goqu_amd" / "csrc" / "planner.cpp"]
+CSRC = HERE.parent.parent / "goqu_amd" / "csrc"
+SRC = [HERE / "stage_emulator.cpp", CSRC / "planner.cpp", CSRC / "adjoint_plan.cpp"]
 LIB = HERE / "_build" / "libstage_emu.so"
 _lib = None
 
@@ -28,6 +29,10 @@ def lib():
                                 C.c_int, C.c_int, C.POINTER(C.c_int64)]
         L.emu_apply.restype = C.c_int
         L.emu_error.restype = C.c_char_p
+        L.emu_sv_reverse.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_int,
+                                     C.POINTER(N.GoquOp), C.c_int, C.POINTER(N.GoquDeriv), C.c_int, C.c_int,
+                                     C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.emu_sv_reverse.restype = C.c_int
         _lib = L
     return _lib
 
@@ -42,3 +47,18 @@ def apply(state: np.ndarray, nq_user: int, buf: "N.OpBuffer", tile_bits: int = 1
     if rc:
         raise RuntimeError(f"emulator status {rc}: {lib().emu_error().decode()}")
     return ns.value
+
+
+def sv_reverse(psi: np.ndarray, lam: np.ndarray, nq_user: int, buf: "N.OpBuffer", derivs, nd: int,
+               tile_bits: int = 12, low_bits: int = 3):
+    """The statevector adjoint's fused reverse sweep on psi and lam (complex128, padded to the
+    internal width) in place.  Returns (per-derivative Re <lam|D|psi>, stage count)."""
+    nqi = int(np.log2(len(psi)))
+    vals = np.zeros(max(nd, 1))
+    ns = C.c_int64(0)
+    dp = lambda a: a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    rc = lib().emu_sv_reverse(dp(psi), dp(lam), nqi, nq_user, buf.array(), len(buf), derivs, nd, tile_bits,
+                              low_bits, vals.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ns))
+    if rc:
+        raise RuntimeError(f"emulator status {rc}: {lib().emu_error().decode()}")
+    return vals[:nd], ns.value
