@@ -21,10 +21,13 @@ ORACLE_DIR = ROOT / "oracle"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GOQU_OFFLOAD_ARCH", "gfx950")
-# packed-FP32 instructions stay disabled in every HIP object
+# packed-FP32 instructions stay disabled in every HIP object.  Uniform branches are not
+# structurized: the stage kernel's instruction dispatch is all uniform, and structurizing it
+# makes the compiler copy the 16 register-resident amplitudes at every join of the dispatch.
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]
 

@@ -66,46 +66,79 @@ struct EOp {
 // ---- device instructions ---------------------------------------------------------------
 enum InstrOp : int32_t {
     I_DENSE = 0,  // dense on register slot mask `smask`; matrix at pool[mat] in slot order
-    I_DIAG = 1,   // diag on global qubits q[0..k-1]
-    I_ANTI1 = 2,  // slot s[0]
-    I_SWAP2 = 3,  // slots s[0], s[1]
+    I_DIAG = 1,   // phase table at pool[mat], see Instr::smask / k / ls
+    I_ANTI1 = 2,  // slot s[0]: [a0, a1] <- [p0 a1, p1 a0], phases at pool[mat]
+    I_SWAP2 = 3,  // slots s[0] < s[1]: |01> <- p |10>, |10> <- p |01>, phase at pool[mat]
     I_RESET = 4,  // slot s[0]
-    I_REMAP = 5,  // new layout lay[0..m-1]: lay[0..3] register local bits, rest thread bits
+    I_REMAP = 5,  // new layout lay[0..m-1] (register slots 0..3, then thread bits), LDS offsets kr
     I_GRAD = 6,   // like I_DENSE, but accumulates Re <b| M |a> into local gradient slot `pad`
 };
 
-// Byte arrays below are read by the kernel as aligned 32-bit words (scalar loads).
+// Instr::flags
+constexpr int32_t F_PURE = 1;   // ANTI1 / SWAP2 with unit phases: a move
+constexpr int32_t F_TCTRL = 2;  // controls on thread-id bits (tmask)
+constexpr int32_t F_PAT = 4;    // controls on register slots (pat)
+constexpr int32_t F_TILE = 8;   // controls on qubits outside the tile (omask), uniform per tile
+constexpr int32_t F_OUTER = 16; // run by the kernel outside its inner op loop (REMAP, RESET, dense on 3-4 slots)
+
+// One instruction.  The planner resolves every qubit an op reads against the register layout in
+// force when the op runs, so the kernel never maps global qubits itself:
+//   register slots      -> compile-time register indices (smask / s[] select the code path),
+//                          controls there -> `pat`, bit r = 1 iff they hold at register index r;
+//   thread-id bits      -> `tmask` (controls), ls[] < 16 (diagonal index bits);
+//   qubits off the tile -> `omask` (controls) and ls[] >= 16, uniform over a workgroup.
+// Byte arrays are read by the kernel as aligned 32-bit words.
 struct alignas(16) Instr {
     int32_t op;
-    int32_t k;
-    int32_t mat;      // offset (complex elements) into the stage pool
-    int32_t flags;    // bit0: pure swap (unit phases)
-    uint64_t cmask;   // control mask on global qubits
+    int32_t k;        // DENSE / GRAD: target count; DIAG: lane index bits (entries of ls[])
+    int32_t mat;      // offset (complex elements) into the program pool
+    int32_t flags;
+    uint32_t pat;     // DENSE / ANTI1 / SWAP2 / GRAD: register control pattern (0xFFFF: none);
+                      // DIAG: bit v = 1 iff column v of the phase table holds an entry != 1
+    uint32_t tmask;   // thread-id bits that must be 1
+    uint64_t omask;   // qubits outside the tile that must be 1
     uint8_t s[4];     // register slots
-    uint8_t q[4];     // global qubits (I_DIAG), qubits[0] = MSB of the diag index
-    int32_t smask;    // I_DENSE / I_GRAD: register slot mask (matrix in slot-bit order)
-    int32_t pad;      // I_GRAD: gradient slot within the stage
-    uint8_t lay[16];  // I_REMAP: new layout, tile-local bit of register slots 0..3 then thread bits
-    uint8_t gq[16];   // I_REMAP: the global qubit of each lay[] entry
-    uint8_t pad2[16];
+    int32_t smask;    // DENSE / GRAD: target slot mask (matrix in slot-bit order);
+                      // DIAG: register slots indexing the table (diagonal qubits and controls there)
+    int32_t pad;      // GRAD: gradient slot within the stage
+    uint8_t ls[4];    // DIAG: lane index bits, MSB first: < 16 thread-id bit, >= 16 qubit (ls-16) off the tile
+    uint8_t lay[16];  // REMAP: tile-local bit of register slots 0..3, then of thread-id bits 0..m-5
+    uint16_t kr[16];  // REMAP: LDS byte offset of register index r in the new layout (swizzled)
 };
 static_assert(sizeof(Instr) == 96, "Instr must stay 96 bytes");
+
+// DIAG phase table: entry (lane_index << popcount(smask)) | v, where v holds the values of the
+// smask register slots (ascending slot order, slot bit -> v bit) and lane_index the ls[] bits.
 
 struct alignas(16) StageHeader {
     int32_t m;            // tile bits
     int32_t ninstr;
     int32_t instr_off;    // index of first Instr in the program's instruction array
-    int32_t pool_off;     // complex-element offset of this stage's pool
+    int32_t pool_off;     // unused (instructions address the program pool directly)
     uint8_t T[16];        // tile qubits, ascending (local bit j <-> global T[j])
     uint8_t lay0[16];     // initial layout (local bits: 4 register slots then thread bits)
     uint8_t gq0[16];      // global qubits of lay0
-    uint8_t layN[16];     // final layout
+    uint8_t layN[16];     // final layout (the one in force after the last instruction)
+    uint8_t gqN[16];      // global qubits of layN
+    uint16_t k0[16];      // LDS byte offsets of register index r in lay0 (swizzled)
     int32_t grad_base;    // global gradient slot of local slot 0
     int32_t ngrad;        // gradient slots of this stage
     int32_t pad[2];
 };
-static_assert(sizeof(StageHeader) == 96, "StageHeader layout");
+static_assert(sizeof(StageHeader) == 144, "StageHeader layout");
 
+// LDS element index swizzle of the stage kernel (complex elements of `elem` bytes): XOR-linear,
+// swz(a ^ b) = swz(a) ^ swz(b), so offsets split into a per-thread and a per-register part.
+// Lanes that differ in any three tile bits whose positions are distinct mod 3 (mod 4 for 8-byte
+// elements) land in distinct 16-byte bank groups.
+inline uint32_t lds_swizzle(uint32_t l, int elem) {
+    if (elem == 16) {
+        const uint32_t h = l >> 3;
+        return l ^ ((h ^ (h >> 3) ^ (h >> 6) ^ (h >> 9)) & 7u);
+    }
+    const uint32_t h = l >> 4;
+    return l ^ ((h ^ (h >> 4) ^ (h >> 8)) & 15u);
+}
 struct Program {
     int nq = 0;           // internal qubit count
     std::vector<StageHeader> stages;
@@ -132,6 +165,8 @@ int lower_dm_ops(const void *ops, int nops, int n, std::vector<EOp> &out, std::s
 
 // Greedy stage planner + register allocation.
 Program plan_program(const std::vector<EOp> &eops, int nq_internal, const PlanConfig &cfg);
+// Fills Instr::kr and StageHeader::k0 for elements of `elem` bytes (16: complex128, 8: complex64).
+void set_lds_offsets(Program &P, int elem);
 
 // ---- adjoint differentiation programs (adjoint_plan.cpp) ----------------------------------
 // conjugate transpose of a unitary EOp (RESET has none)

@@ -596,16 +596,46 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 if (cur.lay[s] == localbit) return s;
             return -1;
         };
+        // thread-id bit holding a local bit in the current layout (-1: a register slot)
+        auto tbit_of = [&](int localbit) {
+            for (int u = kRegBits; u < m; u++)
+                if (cur.lay[u] == localbit) return u - kRegBits;
+            return -1;
+        };
         auto emit_remap = [&](const int nreg[4]) {
             cur = make_layout(m, nreg);
             Instr I;
             std::memset(&I, 0, sizeof(I));
             I.op = I_REMAP;
-            for (int i = 0; i < m; i++) {
-                I.lay[i] = (uint8_t)cur.lay[i];
-                I.gq[i] = H.T[cur.lay[i]];
-            }
+            I.flags = F_OUTER;
+            for (int i = 0; i < m; i++) I.lay[i] = (uint8_t)cur.lay[i];
             P.instrs.push_back(I);
+        };
+        // controls resolved against the current layout: register pattern, thread-id mask and
+        // mask of qubits outside the tile
+        auto set_controls = [&](Instr &I, uint64_t cm) {
+            uint32_t pat = 0xFFFFu;
+            while (cm) {
+                const int q = __builtin_ctzll(cm);
+                cm &= cm - 1;
+                if (loc[q] < 0) {
+                    I.omask |= 1ull << q;
+                    continue;
+                }
+                const int sl = slot_of(loc[q]);
+                if (sl >= 0) {
+                    uint32_t pm = 0;
+                    for (int r = 0; r < kRegAmps; r++)
+                        if (r >> sl & 1) pm |= 1u << r;
+                    pat &= pm;
+                } else {
+                    I.tmask |= 1u << tbit_of(loc[q]);
+                }
+            }
+            I.pat = pat;
+            if (pat != 0xFFFFu) I.flags |= F_PAT;
+            if (I.tmask) I.flags |= F_TCTRL;
+            if (I.omask) I.flags |= F_TILE;
         };
 
         size_t left = nops;
@@ -630,29 +660,96 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             const EOp &e = eops[S.ops[i]];
             Instr I;
             std::memset(&I, 0, sizeof(I));
-            I.cmask = e.cmask;
             I.k = e.k;
             I.mat = (int32_t)(P.pool.size() / 2);
+            I.pat = 0xFFFFu;
             switch (e.kind) {
             case EOP_DIAG: {
                 I.op = I_DIAG;
-                for (int j = 0; j < e.k; j++) I.q[j] = (uint8_t)e.q[j];
-                P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
+                // register slots of the diagonal qubits and of the controls index the table; the
+                // other diagonal qubits become lane index bits, the other controls lane masks
+                int rm = 0, ctl_slots = 0, kl = 0, lane_j[4] = {0, 0, 0, 0};
+                for (int j = 0; j < e.k; j++) {
+                    const int q = e.q[j];
+                    const int sl = loc[q] >= 0 ? slot_of(loc[q]) : -1;
+                    if (sl >= 0) {
+                        rm |= 1 << sl;
+                    } else {
+                        I.ls[kl] = (uint8_t)(loc[q] < 0 ? 16 + q : tbit_of(loc[q]));
+                        lane_j[kl++] = j;
+                    }
+                }
+                uint64_t cm = e.cmask;
+                while (cm) {
+                    const int q = __builtin_ctzll(cm);
+                    cm &= cm - 1;
+                    if (loc[q] < 0) {
+                        I.omask |= 1ull << q;
+                        continue;
+                    }
+                    const int sl = slot_of(loc[q]);
+                    if (sl >= 0) {
+                        rm |= 1 << sl;
+                        ctl_slots |= 1 << sl;
+                    } else {
+                        I.tmask |= 1u << tbit_of(loc[q]);
+                    }
+                }
+                if (I.tmask) I.flags |= F_TCTRL;
+                if (I.omask) I.flags |= F_TILE;
+                I.k = kl;
+                I.smask = rm;
+                const int kr = __builtin_popcount(rm);
+                uint32_t nontriv = 0;
+                std::vector<double> tab(2 << (kl + kr));
+                for (int li = 0; li < (1 << kl); li++)
+                    for (int v = 0; v < (1 << kr); v++) {
+                        int sval[4] = {0, 0, 0, 0};
+                        for (int sl = 0, t = 0; sl < kRegBits; sl++)
+                            if (rm >> sl & 1) sval[sl] = (v >> t++) & 1;
+                        bool ok = true;
+                        for (int sl = 0; sl < kRegBits; sl++)
+                            if ((ctl_slots >> sl & 1) && !sval[sl]) ok = false;
+                        int idx = 0;
+                        for (int j = 0; j < e.k; j++) {
+                            const int q = e.q[j];
+                            const int sl = loc[q] >= 0 ? slot_of(loc[q]) : -1;
+                            int bit = 0;
+                            if (sl >= 0) {
+                                bit = sval[sl];
+                            } else {
+                                for (int p = 0; p < kl; p++)
+                                    if (lane_j[p] == j) bit = (li >> (kl - 1 - p)) & 1;
+                            }
+                            idx = (idx << 1) | bit;
+                        }
+                        const double re = ok ? e.m[2 * idx] : 1.0, im = ok ? e.m[2 * idx + 1] : 0.0;
+                        const int at = (li << kr) | v;
+                        tab[2 * at] = re;
+                        tab[2 * at + 1] = im;
+                        if (re != 1.0 || im != 0.0) nontriv |= 1u << v;
+                    }
+                I.pat = nontriv;
+                P.pool.insert(P.pool.end(), tab.begin(), tab.end());
                 break;
             }
             case EOP_ANTI1:
             case EOP_RESET: {
                 I.op = e.kind == EOP_ANTI1 ? I_ANTI1 : I_RESET;
                 I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
-                I.flags = e.pure_swap ? 1 : 0;
+                if (e.pure_swap) I.flags |= F_PURE;
+                if (e.kind == EOP_RESET) I.flags |= F_OUTER;
+                set_controls(I, e.cmask);
                 P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
                 break;
             }
             case EOP_SWAP2: {
                 I.op = I_SWAP2;
-                I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
-                I.s[1] = (uint8_t)slot_of(loc[e.q[1]]);
-                I.flags = e.pure_swap ? 1 : 0;
+                const int s0 = slot_of(loc[e.q[0]]), s1 = slot_of(loc[e.q[1]]);
+                I.s[0] = (uint8_t)std::min(s0, s1);
+                I.s[1] = (uint8_t)std::max(s0, s1);
+                if (e.pure_swap) I.flags |= F_PURE;
+                set_controls(I, e.cmask);
                 P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
                 break;
             }
@@ -691,6 +788,8 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                         km[2 * (kr * dim + kc) + 1] = e.m[2 * (gr * dim + gc) + 1];
                     }
                 I.smask = smask;
+                set_controls(I, e.cmask);
+                if (e.kind == EOP_DENSE && k > 2) I.flags |= F_OUTER;
                 P.pool.insert(P.pool.end(), km.begin(), km.end());
                 break;
             }
@@ -704,11 +803,32 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 if (cur.lay[s] < 3 && m - kRegBits >= 3) bad = true;
             if (bad) emit_remap(reg);
         }
-        for (int i = 0; i < m; i++) H.layN[i] = (uint8_t)cur.lay[i];
+        for (int i = 0; i < m; i++) {
+            H.layN[i] = (uint8_t)cur.lay[i];
+            H.gqN[i] = H.T[cur.lay[i]];
+        }
         H.ninstr = (int32_t)P.instrs.size() - H.instr_off;
         P.stages.push_back(H);
     }
     return P;
+}
+
+void set_lds_offsets(Program &P, int elem) {
+    auto offs = [&](const uint8_t *lay, uint16_t *out) {
+        for (int r = 0; r < kRegAmps; r++) {
+            uint32_t l = 0;
+            for (int s = 0; s < kRegBits; s++)
+                if (r >> s & 1) l |= 1u << lay[s];
+            out[r] = (uint16_t)(lds_swizzle(l, elem) * (uint32_t)elem);
+        }
+    };
+    for (StageHeader &H : P.stages) {
+        offs(H.lay0, H.k0);
+        for (int i = 0; i < H.ninstr; i++) {
+            Instr &I = P.instrs[H.instr_off + i];
+            if (I.op == I_REMAP) offs(I.lay, I.kr);
+        }
+    }
 }
 
 }  // namespace goqu

@@ -4,11 +4,14 @@
 // 2^m amplitudes: the tile's qubits are T[0..m-1] (the lowest L qubits plus the stage's
 // chosen qubits); the remaining n-m qubit values are fixed per tile.  Each thread keeps 16
 // amplitudes in registers: register slot s <-> tile-local bit lay[s], thread-id bit u <->
-// tile-local bit lay[4+u].  Ops whose targets are register slots run entirely in
-// registers; controls and diagonal gates may sit on any qubit (they only read index bits).
-// I_REMAP re-distributes the tile through LDS so that other qubits become register slots.
-// Global loads/stores use a layout whose lanes 0..2 walk the tile's lowest qubits, so a
-// wave touches contiguous runs of >= 8 amplitudes (128 B at complex128).
+// tile-local bit lay[4+u].  Every instruction arrives resolved against the layout in force
+// (register slots as compile-time register indices, controls as a register pattern, a
+// thread-id mask and a tile-uniform mask), so the per-op work is the arithmetic itself:
+// uncontrolled ops are straight-line FMAs on registers, thread-bit controls select the matrix
+// (or the identity) once per op, register controls skip whole register groups.  I_REMAP
+// re-distributes the tile through LDS; its offsets split into a per-thread part and a
+// host-computed per-register part (the swizzle is XOR-linear).  Global loads/stores use layouts
+// whose lanes 0..2 walk the tile's lowest qubits (runs of >= 8 amplitudes, 128 B at complex128).
 #include <atomic>
 #include <mutex>
 
@@ -25,10 +28,19 @@ __host__ __device__ constexpr int dep4(int v, int mask) {
         }
     return r;
 }
+// the bits of v at the positions of mask, packed (inverse of dep4)
+__host__ __device__ constexpr int pext4(int v, int mask) {
+    int r = 0, b = 0;
+    for (int i = 0; i < 4; i++)
+        if (mask >> i & 1) {
+            if (v >> i & 1) r |= 1 << b;
+            b++;
+        }
+    return r;
+}
 __host__ __device__ constexpr int popc4(int m) { return (m & 1) + (m >> 1 & 1) + (m >> 2 & 1) + (m >> 3 & 1); }
 
-// XOR swizzle of LDS element indices: lanes that differ in any three tile bits whose
-// positions are distinct mod 3 (mod 4 at complex64) hit distinct 16-byte bank groups.
+// the LDS swizzle of lds_swizzle() for complex elements of R
 template <typename R>
 __device__ __forceinline__ uint32_t swz(uint32_t l) {
     if constexpr (sizeof(R) == 8) {
@@ -40,53 +52,60 @@ __device__ __forceinline__ uint32_t swz(uint32_t l) {
     }
 }
 
-struct Lay {
-    uint64_t tg;      // global index bits contributed by tile base + thread id
-    uint32_t tl;      // tile-local bits contributed by thread id
-    uint64_t rq[4];   // global bit of register slot s
-    uint32_t rl[4];   // tile-local bit of register slot s
-};
-
 // byte j of four little-endian words (compile-time j after unrolling: scalar shifts)
 __device__ __forceinline__ uint32_t byte_of(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, int j) {
     const uint32_t w = j < 4 ? w0 : j < 8 ? w1 : j < 12 ? w2 : w3;
     return (w >> (8 * (j & 3))) & 255u;
 }
 
-// Layout from 16-byte arrays of tile-local bits and global qubits, read as four aligned words
-// each (uniform addresses: scalar loads, no per-entry memory round trips).
-__device__ __forceinline__ void set_layout(Lay &L, const uint8_t *__restrict__ lay, const uint8_t *__restrict__ gq,
-                                           int m, uint32_t tid, uint64_t base) {
-    const uint32_t *lw = reinterpret_cast<const uint32_t *>(lay);
+// Global addressing of a layout (tile loads and stores): amplitude index of register index r
+// is tg | rq(r), with tg the tile base plus the thread bits' qubits.
+struct GAddr {
+    uint64_t tg;
+    uint64_t rq[4];
+};
+__device__ __forceinline__ void global_layout(GAddr &G, const uint8_t *__restrict__ gq, int m, uint32_t tid,
+                                              uint64_t base) {
     const uint32_t *gw = reinterpret_cast<const uint32_t *>(gq);
-    const uint32_t l0 = lw[0], l1 = lw[1], l2 = lw[2], l3 = lw[3];
     const uint32_t g0 = gw[0], g1 = gw[1], g2 = gw[2], g3 = gw[3];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        L.rl[s] = 1u << byte_of(l0, l1, l2, l3, s);
-        L.rq[s] = 1ull << byte_of(g0, g1, g2, g3, s);
-    }
+    for (int s = 0; s < 4; s++) G.rq[s] = 1ull << byte_of(g0, g1, g2, g3, s);
     uint64_t tg = base;
-    uint32_t tl = 0;
 #pragma unroll
-    for (int u = 0; u < 12; u++) {
-        if (u < m - 4 && (tid >> u & 1)) {
-            tl |= 1u << byte_of(l0, l1, l2, l3, 4 + u);
-            tg |= 1ull << byte_of(g0, g1, g2, g3, 4 + u);
-        }
-    }
-    L.tg = tg;
-    L.tl = tl;
+    for (int u = 0; u < 12; u++)
+        if (u < m - 4 && (tid >> u & 1)) tg |= 1ull << byte_of(g0, g1, g2, g3, 4 + u);
+    G.tg = tg;
+}
+template <int r>
+__device__ __forceinline__ uint64_t rgof(const GAddr &G) {
+    return ((r & 1) ? G.rq[0] : 0ull) | ((r & 2) ? G.rq[1] : 0ull) | ((r & 4) ? G.rq[2] : 0ull) |
+           ((r & 8) ? G.rq[3] : 0ull);
 }
 
-template <int r>
-__device__ __forceinline__ uint64_t rgof(const Lay &L) {
-    return ((r & 1) ? L.rq[0] : 0ull) | ((r & 2) ? L.rq[1] : 0ull) | ((r & 4) ? L.rq[2] : 0ull) |
-           ((r & 8) ? L.rq[3] : 0ull);
+// per-thread part of the LDS byte offsets of a layout
+template <typename R>
+__device__ __forceinline__ uint32_t lds_lane(const uint8_t *__restrict__ lay, int m, uint32_t tid) {
+    const uint32_t *lw = reinterpret_cast<const uint32_t *>(lay);
+    const uint32_t l0 = lw[0], l1 = lw[1], l2 = lw[2], l3 = lw[3];
+    uint32_t tl = 0;
+#pragma unroll
+    for (int u = 0; u < 12; u++)
+        if (u < m - 4 && (tid >> u & 1)) tl |= 1u << byte_of(l0, l1, l2, l3, 4 + u);
+    return swz<R>(tl) * (uint32_t)sizeof(cx<R>);
+}
+
+// per-register LDS byte offsets, two 16-bit values per word
+struct KOff {
+    uint32_t w[8];
+};
+__device__ __forceinline__ void load_koff(KOff &K, const uint16_t *__restrict__ k) {
+    const uint32_t *kw = reinterpret_cast<const uint32_t *>(k);
+#pragma unroll
+    for (int i = 0; i < 8; i++) K.w[i] = kw[i];
 }
 template <int r>
-__device__ __forceinline__ uint32_t rlof(const Lay &L) {
-    return ((r & 1) ? L.rl[0] : 0u) | ((r & 2) ? L.rl[1] : 0u) | ((r & 4) ? L.rl[2] : 0u) | ((r & 8) ? L.rl[3] : 0u);
+__device__ __forceinline__ uint32_t koff(const KOff &K) {
+    return (K.w[r >> 1] >> (16 * (r & 1))) & 0xFFFFu;
 }
 
 // compile-time loop helper
@@ -103,92 +122,238 @@ __device__ __forceinline__ cx<R> cmulp(cx<R> a, R br, R bi) {
     return {a.x * br - a.y * bi, a.x * bi + a.y * br};
 }
 
-// Register-slot pattern of a global qubit: bit r set iff register index r has that qubit = 1
-// (uniform across the wavefront; zero when the qubit is not a register slot).
-__device__ __forceinline__ uint32_t reg_pattern(const Lay &L, uint64_t qb) {
-    return (L.rq[0] == qb ? 0xAAAAu : 0u) | (L.rq[1] == qb ? 0xCCCCu : 0u) | (L.rq[2] == qb ? 0xF0F0u : 0u) |
-           (L.rq[3] == qb ? 0xFF00u : 0u);
+// Register discipline of the instruction loop.  Every op leaves each amplitude in the register
+// it arrived in: results are written back through tied ("+v") operands.  Without this the
+// compiler keeps a second copy of the 16 amplitudes and moves all of them at the loop's
+// control-flow joins, which doubles the VALU work of a gate.
+__device__ __forceinline__ void commit(double &dst, double val) { asm volatile("v_mov_b64 %0, %1" : "+v"(dst) : "v"(val)); }
+__device__ __forceinline__ void commit(float &dst, float val) { asm volatile("v_mov_b32 %0, %1" : "+v"(dst) : "v"(val)); }
+template <typename R>
+__device__ __forceinline__ void commit(cx<R> &dst, cx<R> val) {
+    commit(dst.x, val.x);
+    commit(dst.y, val.y);
+}
+// a <- a * (pr + i pi) in place; P in SGPRs (uniform phase) or VGPRs (per-lane phase)
+template <bool UNIFORM>
+__device__ __forceinline__ void cmul_inplace(double &re, double &im, double pr, double pi) {
+    double t, u;
+    if constexpr (UNIFORM)
+        asm volatile(
+            "v_mul_f64 %2, %1, %5\n\t"
+            "v_mul_f64 %3, %1, %4\n\t"
+            "v_fma_f64 %1, %0, %5, %3\n\t"
+            "v_fma_f64 %0, %0, %4, -%2"
+            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
+            : "s"(pr), "s"(pi));
+    else
+        asm volatile(
+            "v_mul_f64 %2, %1, %5\n\t"
+            "v_mul_f64 %3, %1, %4\n\t"
+            "v_fma_f64 %1, %0, %5, %3\n\t"
+            "v_fma_f64 %0, %0, %4, -%2"
+            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
+            : "v"(pr), "v"(pi));
+}
+template <bool UNIFORM>
+__device__ __forceinline__ void cmul_inplace(float &re, float &im, float pr, float pi) {
+    float t, u;
+    if constexpr (UNIFORM)
+        asm volatile(
+            "v_mul_f32 %2, %1, %5\n\t"
+            "v_mul_f32 %3, %1, %4\n\t"
+            "v_fma_f32 %1, %0, %5, %3\n\t"
+            "v_fma_f32 %0, %0, %4, -%2"
+            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
+            : "s"(pr), "s"(pi));
+    else
+        asm volatile(
+            "v_mul_f32 %2, %1, %5\n\t"
+            "v_mul_f32 %3, %1, %4\n\t"
+            "v_fma_f32 %1, %0, %5, %3\n\t"
+            "v_fma_f32 %0, %0, %4, -%2"
+            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
+            : "v"(pr), "v"(pi));
+}
+__device__ __forceinline__ void swap_inplace(double &x, double &y) {
+    double t;
+    asm volatile("v_mov_b64 %2, %0\n\tv_mov_b64 %0, %1\n\tv_mov_b64 %1, %2" : "+v"(x), "+v"(y), "=&v"(t));
+}
+__device__ __forceinline__ void swap_inplace(float &x, float &y) { asm volatile("v_swap_b32 %0, %1" : "+v"(x), "+v"(y)); }
+template <typename R>
+__device__ __forceinline__ void swap_inplace(cx<R> &x, cx<R> &y) {
+    swap_inplace(x.x, y.x);
+    swap_inplace(x.y, y.y);
 }
 
-// Controls split into a uniform pattern over register indices and one per-thread flag:
-// the control holds for register index r iff tok && (pat >> r & 1).
-struct Ctrl {
-    uint32_t pat;
-    bool tok;
-};
-__device__ __forceinline__ Ctrl ctrl_split(uint64_t cm, const Lay &L) {
-    uint32_t pat = 0xFFFFu;
-    if (L.rq[0] & cm) pat &= 0xAAAAu;
-    if (L.rq[1] & cm) pat &= 0xCCCCu;
-    if (L.rq[2] & cm) pat &= 0xF0F0u;
-    if (L.rq[3] & cm) pat &= 0xFF00u;
-    const uint64_t cmt = cm & ~(L.rq[0] | L.rq[1] | L.rq[2] | L.rq[3]);
-    return {pat, (L.tg & cmt) == cmt};
-}
-
-template <int MASK, typename R>
-__device__ __forceinline__ void op_dense(cx<R> (&a)[16], const R *__restrict__ M, uint64_t cm, const Lay &L) {
+// Dense op on the register slots of MASK (at most two).  CTRL: lanes whose thread-bit controls
+// fail get the identity (selected once per op), register groups outside `pat` are skipped.
+template <int MASK, bool CTRL, bool PAIR, typename R>
+__device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ M, uint32_t pat,
+                                         bool lok) {
     constexpr int K = popc4(MASK);
     constexpr int D = 1 << K;
     constexpr int OUTER = 16 >> K;
     constexpr int OM = 15 & ~MASK;
-    const Ctrl c = ctrl_split(cm, L);
+    R mm[2 * D * D];
+    static_for<0, 2 * D * D>([&](auto ic) __attribute__((always_inline)) { mm[decltype(ic)::value] = M[decltype(ic)::value]; });
+    if constexpr (CTRL) {
+        static_for<0, D * D>([&](auto ic) __attribute__((always_inline)) {
+            constexpr int e = decltype(ic)::value;
+            constexpr bool dg = (e / D) == (e % D);
+            mm[2 * e] = lok ? mm[2 * e] : R(dg ? 1 : 0);
+            mm[2 * e + 1] = lok ? mm[2 * e + 1] : R(0);
+        });
+    }
     static_for<0, OUTER>([&](auto oc) __attribute__((always_inline)) {
         constexpr int base = dep4(decltype(oc)::value, OM);
-        const bool ok = c.tok && ((c.pat >> base) & 1u);
-        cx<R> x[D];
-        static_for<0, D>([&](auto jc) __attribute__((always_inline)) { x[decltype(jc)::value] = a[base | dep4(decltype(jc)::value, MASK)]; });
-        static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
-            constexpr int i = decltype(ic)::value;
-            R re = 0, im = 0;
-            static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
-                constexpr int j = decltype(jc)::value;
-                const R mr = M[2 * (i * D + j)], mi = M[2 * (i * D + j) + 1];
-                re = fma(mr, x[j].x, re);
-                re = fma(-mi, x[j].y, re);
-                im = fma(mr, x[j].y, im);
-                im = fma(mi, x[j].x, im);
-            });
-            constexpr int dst = base | dep4(i, MASK);
-            if (ok) a[dst] = cx<R>{re, im};
-        });
+        if (!CTRL || ((pat >> base) & 1u)) {
+            auto apply = [&](cx<R>(&v)[16]) __attribute__((always_inline)) {
+                cx<R> x[D], y[D];
+                static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+                    x[decltype(jc)::value] = v[base | dep4(decltype(jc)::value, MASK)];
+                });
+                static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
+                    constexpr int i = decltype(ic)::value;
+                    R re = 0, im = 0;
+                    static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        const R mr = mm[2 * (i * D + j)], mi = mm[2 * (i * D + j) + 1];
+                        re = fma(mr, x[j].x, re);
+                        re = fma(-mi, x[j].y, re);
+                        im = fma(mr, x[j].y, im);
+                        im = fma(mi, x[j].x, im);
+                    });
+                    y[i] = cx<R>{re, im};
+                });
+                static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
+                    commit(v[base | dep4(decltype(ic)::value, MASK)], y[decltype(ic)::value]);
+                });
+            };
+            apply(a);
+            if constexpr (PAIR) apply(b);
+        }
     });
 }
 
-template <int S, typename R>
-__device__ __forceinline__ void op_anti1(cx<R> (&a)[16], R p0r, R p0i, R p1r, R p1i, bool pure, uint64_t cm,
-                                         const Lay &L) {
-    const Ctrl c = ctrl_split(cm, L);
+// Dense op on three or four register slots: computed for every lane, kept where the
+// thread-bit controls hold (a per-amplitude select; these ops are rare).
+template <int MASK, bool PAIR, typename R>
+__device__ __forceinline__ void op_dense_big(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ M, uint32_t pat,
+                                             bool lok) {
+    constexpr int K = popc4(MASK);
+    constexpr int D = 1 << K;
+    constexpr int OUTER = 16 >> K;
+    constexpr int OM = 15 & ~MASK;
+    static_for<0, OUTER>([&](auto oc) __attribute__((always_inline)) {
+        constexpr int base = dep4(decltype(oc)::value, OM);
+        if ((pat >> base) & 1u) {
+            auto apply = [&](cx<R>(&v)[16]) __attribute__((always_inline)) {
+                cx<R> x[D], y[D];
+                static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+                    x[decltype(jc)::value] = v[base | dep4(decltype(jc)::value, MASK)];
+                });
+                static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
+                    constexpr int i = decltype(ic)::value;
+                    R re = 0, im = 0;
+                    static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
+                        constexpr int j = decltype(jc)::value;
+                        const R mr = M[2 * (i * D + j)], mi = M[2 * (i * D + j) + 1];
+                        re = fma(mr, x[j].x, re);
+                        re = fma(-mi, x[j].y, re);
+                        im = fma(mr, x[j].y, im);
+                        im = fma(mi, x[j].x, im);
+                    });
+                    y[i] = cx<R>{lok ? re : x[i].x, lok ? im : x[i].y};
+                });
+                static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
+                    commit(v[base | dep4(decltype(ic)::value, MASK)], y[decltype(ic)::value]);
+                });
+            };
+            apply(a);
+            if constexpr (PAIR) apply(b);
+        }
+    });
+}
+
+// [a0, a1] <- [p0 a1, p1 a0] on slot S (PURE: p0 = p1 = 1).  TC: thread-bit controls.
+template <int S, bool PURE, bool TC, bool PAIR, typename R>
+__device__ __forceinline__ void op_anti1(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ p, uint32_t pat,
+                                         bool lok) {
+    R p0r = 1, p0i = 0, p1r = 1, p1i = 0;
+    if constexpr (!PURE) {
+        p0r = p[0];
+        p0i = p[1];
+        p1r = p[2];
+        p1i = p[3];
+    }
     static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
         constexpr int r = decltype(rc)::value;
         if constexpr (!(r >> S & 1)) {
             constexpr int r1 = r | (1 << S);
-            const bool ok = c.tok && ((c.pat >> r) & 1u);
-            cx<R> a0 = a[r], a1 = a[r1];
-            cx<R> n0 = pure ? a1 : cmulp(a1, p0r, p0i);
-            cx<R> n1 = pure ? a0 : cmulp(a0, p1r, p1i);
-            if (ok) {
-                a[r] = n0;
-                a[r1] = n1;
+            if ((pat >> r) & 1u) {
+                auto apply = [&](cx<R>(&v)[16]) __attribute__((always_inline)) {
+                    if constexpr (PURE && !TC) {
+                        swap_inplace(v[r], v[r1]);
+                    } else {
+                        const cx<R> a0 = v[r], a1 = v[r1];
+                        cx<R> n0 = a1, n1 = a0;
+                        if constexpr (!PURE) {
+                            n0 = cmulp(a1, p0r, p0i);
+                            n1 = cmulp(a0, p1r, p1i);
+                        }
+                        if constexpr (TC) {
+                            n0.x = lok ? n0.x : a0.x;
+                            n0.y = lok ? n0.y : a0.y;
+                            n1.x = lok ? n1.x : a1.x;
+                            n1.y = lok ? n1.y : a1.y;
+                        }
+                        commit(v[r], n0);
+                        commit(v[r1], n1);
+                    }
+                };
+                apply(a);
+                if constexpr (PAIR) apply(b);
             }
         }
     });
 }
 
-template <int S0, int S1, typename R>
-__device__ __forceinline__ void op_swap2(cx<R> (&a)[16], R pr, R pi, bool pure, uint64_t cm, const Lay &L) {
-    const Ctrl c = ctrl_split(cm, L);
+// |01> <- p |10>, |10> <- p |01> on slots S0 < S1.
+template <int S0, int S1, bool PURE, bool TC, bool PAIR, typename R>
+__device__ __forceinline__ void op_swap2(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ p, uint32_t pat,
+                                         bool lok) {
+    R pr = 1, pi = 0;
+    if constexpr (!PURE) {
+        pr = p[0];
+        pi = p[1];
+    }
     static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
         constexpr int r = decltype(rc)::value;
         if constexpr (!(r >> S0 & 1) && !(r >> S1 & 1)) {
-            constexpr int i01 = r | (1 << S1), i10 = r | (1 << S0);
-            const bool ok = c.tok && ((c.pat >> r) & 1u);
-            cx<R> a01 = a[i01], a10 = a[i10];
-            cx<R> n01 = pure ? a10 : cmulp(a10, pr, pi);
-            cx<R> n10 = pure ? a01 : cmulp(a01, pr, pi);
-            if (ok) {
-                a[i01] = n01;
-                a[i10] = n10;
+            constexpr int i01 = r | (1 << S0), i10 = r | (1 << S1);
+            if ((pat >> r) & 1u) {
+                auto apply = [&](cx<R>(&v)[16]) __attribute__((always_inline)) {
+                    if constexpr (PURE && !TC) {
+                        swap_inplace(v[i01], v[i10]);
+                    } else {
+                        const cx<R> x01 = v[i01], x10 = v[i10];
+                        cx<R> n01 = x10, n10 = x01;
+                        if constexpr (!PURE) {
+                            n01 = cmulp(x10, pr, pi);
+                            n10 = cmulp(x01, pr, pi);
+                        }
+                        if constexpr (TC) {
+                            n01.x = lok ? n01.x : x01.x;
+                            n01.y = lok ? n01.y : x01.y;
+                            n10.x = lok ? n10.x : x10.x;
+                            n10.y = lok ? n10.y : x10.y;
+                        }
+                        commit(v[i01], n01);
+                        commit(v[i10], n10);
+                    }
+                };
+                apply(a);
+                if constexpr (PAIR) apply(b);
             }
         }
     });
@@ -202,92 +367,85 @@ __device__ __forceinline__ void op_reset(cx<R> (&a)[16]) {
             constexpr int r1 = r | (1 << S);
             cx<R> a0 = a[r], a1 = a[r1];
             R nrm = sqrt(a0.x * a0.x + a0.y * a0.y + a1.x * a1.x + a1.y * a1.y);
-            a[r] = nrm > R(1e-15) ? cx<R>{nrm, R(0)} : cx<R>{R(0), R(0)};
-            a[r1] = cx<R>{R(0), R(0)};
+            commit(a[r], nrm > R(1e-15) ? cx<R>{nrm, R(0)} : cx<R>{R(0), R(0)});
+            commit(a[r1], cx<R>{R(0), R(0)});
         }
     });
 }
 
-template <typename R>
-__device__ __forceinline__ void op_diag(cx<R> (&a)[16], const Instr &I, const R *__restrict__ D, const Lay &L) {
-    const int k = I.k;
-    const uint64_t cm = I.cmask;
-    const uint32_t qw = *reinterpret_cast<const uint32_t *>(I.q);
-    const int q0 = qw & 255u, q1 = (qw >> 8) & 255u, q2 = (qw >> 16) & 255u, q3 = qw >> 24;
-    if (k <= 2) {
-        // index bits split like the controls: a uniform register pattern per diag qubit and a
-        // per-thread part folded into a per-thread table e[ri] = D[tidx | ri]
-        const uint32_t m0 = reg_pattern(L, 1ull << q0);
-        const uint32_t m1 = k == 2 ? reg_pattern(L, 1ull << q1) : 0u;
-        const bool t0 = (L.tg >> q0) & 1, t1 = k == 2 ? ((L.tg >> q1) & 1) : false;
-        const Ctrl c = ctrl_split(cm, L);
-        const R d0r = D[0], d0i = D[1], d1r = D[2], d1i = D[3];
-        const R d2r = k == 2 ? D[4] : R(0), d2i = k == 2 ? D[5] : R(0);
-        const R d3r = k == 2 ? D[6] : R(0), d3i = k == 2 ? D[7] : R(0);
-        R e0r, e0i, e1r, e1i, e2r = 0, e2i = 0, e3r = 0, e3i = 0;
-        if (k == 2) {  // bit 1 <- q0, bit 0 <- q1
-            e0r = t0 ? (t1 ? d3r : d2r) : (t1 ? d1r : d0r);
-            e0i = t0 ? (t1 ? d3i : d2i) : (t1 ? d1i : d0i);
-            e1r = t0 ? d3r : d1r;
-            e1i = t0 ? d3i : d1i;
-            e2r = t1 ? d3r : d2r;
-            e2i = t1 ? d3i : d2i;
-            e3r = d3r;
-            e3i = d3i;
-        } else {
-            e0r = t0 ? d1r : d0r;
-            e0i = t0 ? d1i : d0i;
-            e1r = d1r;
-            e1i = d1i;
-        }
-        if (c.tok) {  // per-lane predicate; everything below branches on uniform values
-            if ((m0 | m1) == 0) {  // no diag qubit in registers: one factor per thread
-                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-                    constexpr int r = decltype(rc)::value;
-                    if ((c.pat >> r) & 1u) a[r] = cmulp(a[r], e0r, e0i);
-                });
-            } else {
-                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-                    constexpr int r = decltype(rc)::value;
-                    if ((c.pat >> r) & 1u) {
-                        const uint32_t ri = (k == 2) ? ((((m0 >> r) & 1u) << 1) | ((m1 >> r) & 1u)) : ((m0 >> r) & 1u);
-                        if (ri == 0) a[r] = cmulp(a[r], e0r, e0i);
-                        else if (ri == 1) a[r] = cmulp(a[r], e1r, e1i);
-                        else if (ri == 2) a[r] = cmulp(a[r], e2r, e2i);
-                        else a[r] = cmulp(a[r], e3r, e3i);
-                    }
-                });
-            }
-        }
-    } else {
+// Diagonal op: phase of register index r = table[(lane_index << popc(RM)) | pext(r, RM)], one
+// for lanes whose thread-bit controls fail.  Table columns without an entry != 1 are skipped.
+template <int RM, bool PAIR, typename R>
+__device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const Instr &I, const R *__restrict__ pool,
+                                        uint32_t tid, uint64_t base, bool lok) {
+    constexpr int KR = popc4(RM);
+    constexpr int NV = 1 << KR;
+    const cx<R> *__restrict__ tab = reinterpret_cast<const cx<R> *>(pool) + I.mat;
+    const uint32_t nontriv = I.pat;
+    const int kl = I.k;
+    const bool tc = I.flags & F_TCTRL;
+    // multiplies the amplitudes of table column v in place (one uniform branch per column)
+    auto column = [&](auto vc, auto uc, cx<R> ph) __attribute__((always_inline)) {
+        constexpr int v = decltype(vc)::value;
+        constexpr bool U = decltype(uc)::value;
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
-            const uint64_t g = L.tg | rgof<r>(L);
-            const bool ok = (g & cm) == cm;
-            int idx = (int)((g >> q0) & 1);
-            idx = (idx << 1) | (int)((g >> q1) & 1);
-            idx = (idx << 1) | (int)((g >> q2) & 1);
-            if (k == 4) idx = (idx << 1) | (int)((g >> q3) & 1);
-            const R dr = D[2 * idx], di = D[2 * idx + 1];
-            cx<R> v = cmulp(a[r], dr, di);
-            if (ok) a[r] = v;
+            if constexpr (pext4(r, RM) == v) {
+                cmul_inplace<U>(a[r].x, a[r].y, ph.x, ph.y);
+                if constexpr (PAIR) cmul_inplace<U>(b[r].x, b[r].y, ph.x, ph.y);
+            }
         });
+    };
+    if (kl == 0 && !tc) {  // every lane uses the same phases
+        static_for<0, NV>([&](auto vc) __attribute__((always_inline)) {
+            if ((nontriv >> decltype(vc)::value) & 1u)
+                column(vc, std::integral_constant<bool, true>{}, tab[decltype(vc)::value]);
+        });
+        return;
     }
+    // lane index from thread-id bits and (uniform) bits of the tile base
+    const uint32_t lsw = *reinterpret_cast<const uint32_t *>(I.ls);
+    uint32_t li = 0;
+    for (int j = 0; j < kl; j++) {
+        const uint32_t src = (lsw >> (8 * j)) & 255u;
+        const uint32_t bit = src < 16 ? ((tid >> src) & 1u) : (uint32_t)((base >> (src - 16)) & 1u);
+        li = (li << 1) | bit;
+    }
+    static_for<0, NV>([&](auto vc) __attribute__((always_inline)) {
+        constexpr int v = decltype(vc)::value;
+        if ((nontriv >> v) & 1u) {
+            cx<R> t;
+            if (kl == 0) {
+                t = tab[v];
+            } else if (kl == 1) {  // two uniform candidates
+                const cx<R> t0 = tab[v], t1 = tab[NV + v];
+                t.x = li ? t1.x : t0.x;
+                t.y = li ? t1.y : t0.y;
+            } else {
+                t = tab[(li << KR) | v];  // per-lane load from a small, cached table
+            }
+            if (tc) {
+                t.x = lok ? t.x : R(1);
+                t.y = lok ? t.y : R(0);
+            }
+            column(vc, std::integral_constant<bool, false>{}, t);
+        }
+    });
 }
 
 // Re sum over the op's groups of conj(b) (M a): the derivative term of the adjoint method.
+// Lanes whose thread-bit controls fail and groups outside `pat` contribute nothing.
 template <int MASK, typename R>
 __device__ __forceinline__ double op_grad(const cx<R> (&a)[16], const cx<R> (&b)[16], const R *__restrict__ M,
-                                          uint64_t cm, const Lay &L) {
+                                          uint32_t pat, bool lok) {
     constexpr int K = popc4(MASK);
     constexpr int D = 1 << K;
     constexpr int OUTER = 16 >> K;
     constexpr int OM = 15 & ~MASK;
-    const Ctrl c = ctrl_split(cm, L);
     double acc = 0;
     static_for<0, OUTER>([&](auto oc) __attribute__((always_inline)) {
         constexpr int base = dep4(decltype(oc)::value, OM);
-        if (c.tok && ((c.pat >> base) & 1u)) {
+        if ((pat >> base) & 1u) {
             static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
                 constexpr int i = decltype(ic)::value;
                 R re = 0, im = 0;
@@ -305,7 +463,22 @@ __device__ __forceinline__ double op_grad(const cx<R> (&a)[16], const cx<R> (&b)
             });
         }
     });
-    return acc;
+    return lok ? acc : 0.0;
+}
+
+template <typename R>
+__device__ __forceinline__ void lds_store(unsigned char *smem, const cx<R> (&v)[16], uint32_t lb, const KOff &K) {
+    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
+        *reinterpret_cast<cx<R> *>(smem + (lb ^ koff<r>(K))) = v[r];
+    });
+}
+template <typename R>
+__device__ __forceinline__ void lds_load(const unsigned char *smem, cx<R> (&v)[16], uint32_t lb, const KOff &K) {
+    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
+        v[r] = *reinterpret_cast<const cx<R> *>(smem + (lb ^ koff<r>(K)));
+    });
 }
 
 // One stage over one state (PAIR = false) or over psi and lambda together (PAIR = true: every op
@@ -316,7 +489,6 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                                            const R *__restrict__ pool, uint64_t ntiles, double *__restrict__ gpart,
                                            double *gacc) {
     extern __shared__ unsigned char smem_raw[];
-    cx<R> *lds = reinterpret_cast<cx<R> *>(smem_raw);
     const int m = hdr->m;
     const int ninstr = hdr->ninstr;
     const uint32_t tid = threadIdx.x;
@@ -342,164 +514,211 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 base = ((base >> t) << (t + 1)) | lo;
             }
         }
-        Lay L;
-        set_layout(L, hdr->lay0, hdr->gq0, m, tid, base);
-        cx<R> a[16], b[PAIR ? 16 : 1];
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            a[r] = st[L.tg | rgof<r>(L)];
-            if constexpr (PAIR) b[r] = st2[L.tg | rgof<r>(L)];
-        });
+        cx<R> a[16], b[16];  // b: lambda in paired mode, unused otherwise
+        {
+            GAddr G;
+            global_layout(G, hdr->gq0, m, tid, base);
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value;
+                a[r] = st[G.tg | rgof<r>(G)];
+                if constexpr (PAIR) b[r] = st2[G.tg | rgof<r>(G)];
+            });
+        }
+        uint32_t lb = lds_lane<R>(hdr->lay0, m, tid);
+        KOff K;
+        load_koff(K, hdr->k0);
 
-        for (int pc = 0; pc < ninstr; pc++) {
-            const Instr &I = P[pc];
-            const int op = I.op;
-            // slots and diag qubits as one word each
-            const uint32_t sw = *reinterpret_cast<const uint32_t *>(I.s);
-            if (op == I_REMAP) {
-                __syncthreads();
-                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-                    constexpr int r = decltype(rc)::value;
-                    lds[swz<R>(L.tl | rlof<r>(L))] = a[r];
-                });
-                __syncthreads();
-                Lay Ln;
-                set_layout(Ln, I.lay, I.gq, m, tid, base);
-                static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-                    constexpr int r = decltype(rc)::value;
-                    a[r] = lds[swz<R>(Ln.tl | rlof<r>(Ln))];
-                });
-                if constexpr (PAIR) {
-                    __syncthreads();
-                    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-                        constexpr int r = decltype(rc)::value;
-                        lds[swz<R>(L.tl | rlof<r>(L))] = b[r];
-                    });
-                    __syncthreads();
-                    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-                        constexpr int r = decltype(rc)::value;
-                        b[r] = lds[swz<R>(Ln.tl | rlof<r>(Ln))];
-                    });
-                }
-                L = Ln;
-            } else if (op == I_DENSE) {
+        int pc = 0;
+        while (true) {
+            // The ops up to the next remap.  The remap lives outside this loop: with it inside,
+            // the compiler splits the amplitudes into two register sets and copies all of them
+            // on every op.  pc goes through readfirstlane so that it stays provably uniform
+            // (instruction fields then come through scalar loads).
+            for (; pc < ninstr; pc++) {
+                pc = __builtin_amdgcn_readfirstlane(pc);
+                const Instr &I = P[pc];
+                const int op = I.op;
+                const int flags = I.flags;
+                if (flags & F_OUTER) break;
+                if ((flags & F_TILE) && (base & I.omask) != I.omask) continue;  // uniform per workgroup
+                const bool lok = (tid & I.tmask) == I.tmask;
+                const uint32_t pat = I.pat;
                 const R *__restrict__ M = pool + 2 * I.mat;
-#define DENSE_CASE(MK)                                     \
-    case MK:                                               \
-        op_dense<MK>(a, M, I.cmask, L);                    \
-        if constexpr (PAIR) op_dense<MK>(b, M, I.cmask, L); \
-        break;
-                switch (I.smask) {
-                    DENSE_CASE(1)
-                    DENSE_CASE(2)
-                    DENSE_CASE(3)
-                    DENSE_CASE(4)
-                    DENSE_CASE(5)
-                    DENSE_CASE(6)
-                    DENSE_CASE(7)
-                    DENSE_CASE(8)
-                    DENSE_CASE(9)
-                    DENSE_CASE(10)
-                    DENSE_CASE(11)
-                    DENSE_CASE(12)
-                    DENSE_CASE(13)
-                    DENSE_CASE(14)
-                    DENSE_CASE(15)
-                default:
-                    break;
+                const uint32_t sw = *reinterpret_cast<const uint32_t *>(I.s);
+                if (op == I_DENSE) {
+                    const bool ctrl = flags & (F_TCTRL | F_PAT);
+    #define DENSE_CASE(MK)                                                       \
+        case MK:                                                                 \
+            if (ctrl) op_dense<MK, true, PAIR>(a, b, M, pat, lok);               \
+            else op_dense<MK, false, PAIR>(a, b, M, pat, lok);                   \
+            break;
+    #define DENSE_BIG(MK)                                                        \
+        case MK:                                                                 \
+            op_dense_big<MK, PAIR>(a, b, M, pat, lok);                           \
+            break;
+                    switch (I.smask) {
+                        DENSE_CASE(1)
+                        DENSE_CASE(2)
+                        DENSE_CASE(4)
+                        DENSE_CASE(8)
+                        DENSE_CASE(3)
+                        DENSE_CASE(5)
+                        DENSE_CASE(6)
+                        DENSE_CASE(9)
+                        DENSE_CASE(10)
+                        DENSE_CASE(12)
+                    default:
+                        break;
+                    }
+    #undef DENSE_CASE
+    #undef DENSE_BIG
+                } else if (op == I_DIAG) {
+    #define DIAG_CASE(RM)                                            \
+        case RM:                                                     \
+            op_diag<RM, PAIR>(a, b, I, pool, tid, base, lok);        \
+            break;
+                    switch (I.smask) {
+                        DIAG_CASE(0)
+                        DIAG_CASE(1)
+                        DIAG_CASE(2)
+                        DIAG_CASE(3)
+                        DIAG_CASE(4)
+                        DIAG_CASE(5)
+                        DIAG_CASE(6)
+                        DIAG_CASE(7)
+                        DIAG_CASE(8)
+                        DIAG_CASE(9)
+                        DIAG_CASE(10)
+                        DIAG_CASE(11)
+                        DIAG_CASE(12)
+                        DIAG_CASE(13)
+                        DIAG_CASE(14)
+                        DIAG_CASE(15)
+                    default:
+                        break;
+                    }
+    #undef DIAG_CASE
+                } else if (op == I_ANTI1) {
+                    const int variant = ((sw & 255u) << 2) | ((flags & F_PURE) ? 2 : 0) | ((flags & F_TCTRL) ? 1 : 0);
+    #define ANTI_CASE(S)                                                             \
+        case S * 4 + 0: op_anti1<S, false, false, PAIR>(a, b, M, pat, lok); break;   \
+        case S * 4 + 1: op_anti1<S, false, true, PAIR>(a, b, M, pat, lok); break;    \
+        case S * 4 + 2: op_anti1<S, true, false, PAIR>(a, b, M, pat, lok); break;    \
+        case S * 4 + 3: op_anti1<S, true, true, PAIR>(a, b, M, pat, lok); break;
+                    switch (variant) {
+                        ANTI_CASE(0)
+                        ANTI_CASE(1)
+                        ANTI_CASE(2)
+                        ANTI_CASE(3)
+                    default:
+                        break;
+                    }
+    #undef ANTI_CASE
+                } else if (op == I_SWAP2) {
+                    const int s0 = sw & 255u, s1 = (sw >> 8) & 255u;
+                    const int variant = ((s0 * 4 + s1) << 2) | ((flags & F_PURE) ? 2 : 0) | ((flags & F_TCTRL) ? 1 : 0);
+    #define SWAP_CASE(A, B)                                                                       \
+        case (A * 4 + B) * 4 + 0: op_swap2<A, B, false, false, PAIR>(a, b, M, pat, lok); break;   \
+        case (A * 4 + B) * 4 + 1: op_swap2<A, B, false, true, PAIR>(a, b, M, pat, lok); break;    \
+        case (A * 4 + B) * 4 + 2: op_swap2<A, B, true, false, PAIR>(a, b, M, pat, lok); break;    \
+        case (A * 4 + B) * 4 + 3: op_swap2<A, B, true, true, PAIR>(a, b, M, pat, lok); break;
+                    switch (variant) {
+                        SWAP_CASE(0, 1)
+                        SWAP_CASE(0, 2)
+                        SWAP_CASE(0, 3)
+                        SWAP_CASE(1, 2)
+                        SWAP_CASE(1, 3)
+                        SWAP_CASE(2, 3)
+                    default:
+                        break;
+                    }
+    #undef SWAP_CASE
+                } else if (op == I_GRAD) {
+                    if constexpr (PAIR) {
+                        double v = 0;
+    #define GRAD_CASE(MK)                              \
+        case MK:                                       \
+            v = op_grad<MK>(a, b, M, pat, lok);        \
+            break;
+                        switch (I.smask) {
+                            GRAD_CASE(1)
+                            GRAD_CASE(2)
+                            GRAD_CASE(3)
+                            GRAD_CASE(4)
+                            GRAD_CASE(5)
+                            GRAD_CASE(6)
+                            GRAD_CASE(8)
+                            GRAD_CASE(9)
+                            GRAD_CASE(10)
+                            GRAD_CASE(12)
+                        default:
+                            break;
+                        }
+    #undef GRAD_CASE
+                        // blocks of fewer than 64 threads only read active lanes
+    #pragma unroll
+                        for (int off = 32; off >= 1; off >>= 1)
+                            if (off < (int)blockDim.x) v += __shfl_xor(v, off, 64);
+                        if (lane == 0) gacc[I.pad * 8 + wave] += v;  // one writer per (slot, wave)
+                    }
                 }
-#undef DENSE_CASE
-            } else if (op == I_DIAG) {
-                op_diag<R>(a, I, pool + 2 * I.mat, L);
-                if constexpr (PAIR) op_diag<R>(b, I, pool + 2 * I.mat, L);
-            } else if (op == I_ANTI1) {
-                const R *__restrict__ p = pool + 2 * I.mat;
-                const bool pure = I.flags & 1;
-                const R p0r = p[0], p0i = p[1], p1r = p[2], p1i = p[3];
-#define ANTI_CASE(S)                                                                      \
-    case S:                                                                               \
-        op_anti1<S>(a, p0r, p0i, p1r, p1i, pure, I.cmask, L);                             \
-        if constexpr (PAIR) op_anti1<S>(b, p0r, p0i, p1r, p1i, pure, I.cmask, L);         \
-        break;
-                switch (sw & 255u) {
-                    ANTI_CASE(0)
-                    ANTI_CASE(1)
-                    ANTI_CASE(2)
-                    ANTI_CASE(3)
-                default:
-                    break;
-                }
-#undef ANTI_CASE
-            } else if (op == I_SWAP2) {
-                const R *__restrict__ p = pool + 2 * I.mat;
-                const bool pure = I.flags & 1;
-                const R pr = p[0], pi = p[1];
-                const int sa = sw & 255u, sb = (sw >> 8) & 255u;
-                const int s0 = sa < sb ? sa : sb, s1 = sa < sb ? sb : sa;
-#define SWAP_CASE(A, B)                                                    \
-    case A * 4 + B:                                                        \
-        op_swap2<A, B>(a, pr, pi, pure, I.cmask, L);                       \
-        if constexpr (PAIR) op_swap2<A, B>(b, pr, pi, pure, I.cmask, L);   \
-        break;
-                switch (s0 * 4 + s1) {
-                    SWAP_CASE(0, 1)
-                    SWAP_CASE(0, 2)
-                    SWAP_CASE(0, 3)
-                    SWAP_CASE(1, 2)
-                    SWAP_CASE(1, 3)
-                    SWAP_CASE(2, 3)
-                default:
-                    break;
-                }
-#undef SWAP_CASE
-            } else if (op == I_RESET) {
-                switch (sw & 255u) {
+            }
+            if (pc >= ninstr) break;
+            pc = __builtin_amdgcn_readfirstlane(pc);
+            const Instr &I = P[pc++];
+            if (I.op == I_RESET) {
+                switch (I.s[0]) {
                 case 0: op_reset<0>(a); break;
                 case 1: op_reset<1>(a); break;
                 case 2: op_reset<2>(a); break;
                 default: op_reset<3>(a); break;
                 }
-            } else if (op == I_GRAD) {
-                if constexpr (PAIR) {
-                    const R *__restrict__ M = pool + 2 * I.mat;
-                    double v = 0;
-#define GRAD_CASE(MK)                                \
-    case MK:                                         \
-        v = op_grad<MK>(a, b, M, I.cmask, L);        \
-        break;
-                    switch (I.smask) {
-                        GRAD_CASE(1)
-                        GRAD_CASE(2)
-                        GRAD_CASE(3)
-                        GRAD_CASE(4)
-                        GRAD_CASE(5)
-                        GRAD_CASE(6)
-                        GRAD_CASE(8)
-                        GRAD_CASE(9)
-                        GRAD_CASE(10)
-                        GRAD_CASE(12)
-                    default:
-                        break;
-                    }
-#undef GRAD_CASE
-                    // blocks of fewer than 64 threads only read active lanes
-#pragma unroll
-                    for (int off = 32; off >= 1; off >>= 1)
-                        if (off < (int)blockDim.x) v += __shfl_xor(v, off, 64);
-                    if (lane == 0) gacc[I.pad * 8 + wave] += v;  // one writer per (slot, wave)
-                }
+                continue;
             }
+            if (I.op == I_DENSE) {  // three or four slots
+                if ((I.flags & F_TILE) && (base & I.omask) != I.omask) continue;
+                const bool lok = (tid & I.tmask) == I.tmask;
+                const R *__restrict__ M = pool + 2 * I.mat;
+                switch (I.smask) {
+                case 7: op_dense_big<7, PAIR>(a, b, M, I.pat, lok); break;
+                case 11: op_dense_big<11, PAIR>(a, b, M, I.pat, lok); break;
+                case 13: op_dense_big<13, PAIR>(a, b, M, I.pat, lok); break;
+                case 14: op_dense_big<14, PAIR>(a, b, M, I.pat, lok); break;
+                case 15: op_dense_big<15, PAIR>(a, b, M, I.pat, lok); break;
+                default: break;
+                }
+                continue;
+            }
+            // remap: the tile goes through LDS into the next layout
+            const uint32_t lbn = lds_lane<R>(I.lay, m, tid);
+            KOff Kn;
+            load_koff(Kn, I.kr);
+            __syncthreads();
+            lds_store<R>(smem_raw, a, lb, K);
+            __syncthreads();
+            lds_load<R>(smem_raw, a, lbn, Kn);
+            if constexpr (PAIR) {
+                __syncthreads();
+                lds_store<R>(smem_raw, b, lb, K);
+                __syncthreads();
+                lds_load<R>(smem_raw, b, lbn, Kn);
+            }
+            lb = lbn;
+            K = Kn;
         }
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            st[L.tg | rgof<r>(L)] = a[r];
-            if constexpr (PAIR) st2[L.tg | rgof<r>(L)] = b[r];
-        });
-        if (tile + gridDim.x < ntiles) __syncthreads();
+        {
+            GAddr G;
+            global_layout(G, hdr->gqN, m, tid, base);
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value;
+                st[G.tg | rgof<r>(G)] = a[r];
+                if constexpr (PAIR) st2[G.tg | rgof<r>(G)] = b[r];
+            });
+        }
     }
     if constexpr (PAIR) {
         __syncthreads();
-        const int nw = blockDim.x >> 6;
+        const int nw = (blockDim.x + 63) >> 6;  // a block of fewer than 64 threads is one wave
         for (int g = tid; g < ngrad; g += blockDim.x) {
             double s = 0;
             for (int w = 0; w < nw; w++) s += gacc[g * 8 + w];
@@ -547,9 +766,18 @@ PlanConfig default_plan_config(int prec, int nq) {
     return c;
 }
 
-int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp) {
+int upload_program(const Program &P0, int prec, hipStream_t s, DeviceProgram &dp) {
     const size_t rsz = prec == GOQU_C128 ? sizeof(double) : sizeof(float);
-    const size_t nh = P.stages.size(), ni = P.instrs.size(), np = P.pool.size();
+    for (const StageHeader &H : P0.stages)
+        if ((size_t(1) << H.m) * 2 * rsz > 65536)
+            return fail(GOQU_ERR_UNSUPPORTED, "stage tile larger than the LDS offset range");
+    // LDS offsets depend on the element size
+    Program P;
+    P.stages = P0.stages;
+    P.instrs = P0.instrs;
+    set_lds_offsets(P, (int)(2 * rsz));
+    const std::vector<double> &pool = P0.pool;
+    const size_t nh = P.stages.size(), ni = P.instrs.size(), np = pool.size();
     dp.hdr_off = 0;
     dp.instr_off = (nh * sizeof(StageHeader) + 255) & ~size_t(255);
     dp.pool_off = (dp.instr_off + ni * sizeof(Instr) + 255) & ~size_t(255);
@@ -558,10 +786,10 @@ int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp)
     std::memcpy(host.data(), P.stages.data(), nh * sizeof(StageHeader));
     std::memcpy(host.data() + dp.instr_off, P.instrs.data(), ni * sizeof(Instr));
     if (prec == GOQU_C128) {
-        std::memcpy(host.data() + dp.pool_off, P.pool.data(), np * sizeof(double));
+        std::memcpy(host.data() + dp.pool_off, pool.data(), np * sizeof(double));
     } else {
         float *f = reinterpret_cast<float *>(host.data() + dp.pool_off);
-        for (size_t i = 0; i < np; i++) f[i] = (float)P.pool[i];
+        for (size_t i = 0; i < np; i++) f[i] = (float)pool[i];
     }
     if (dp.bytes < bytes) {
         if (dp.blob) GOQU_HIP(hipFree(dp.blob));

@@ -17,45 +17,35 @@ using cd = std::complex<double>;
 
 namespace {
 
-struct Lay {
+// Global addressing of a layout, as the kernel computes it from gq[] (checked against T[lay[]]).
+struct GAddr {
     uint64_t tg;
-    uint32_t tl;
     uint64_t rq[4];
-    uint32_t rl[4];
 };
-
-// reads the global qubit of each entry from gq[] (as the kernel does) and checks it
-// against the tile's qubit list
-bool set_layout(Lay &L, const uint8_t *lay, const uint8_t *gq, const uint8_t *T, int m, uint32_t tid,
-                uint64_t base) {
+bool global_layout(GAddr &G, const uint8_t *lay, const uint8_t *gq, const uint8_t *T, int m, uint32_t tid,
+                   uint64_t base) {
     bool ok = true;
     for (int i = 0; i < m; i++) ok &= gq[i] == T[lay[i]];
-    for (int s = 0; s < 4; s++) {
-        L.rl[s] = 1u << lay[s];
-        L.rq[s] = 1ull << gq[s];
-    }
-    L.tg = base;
-    L.tl = 0;
+    for (int s = 0; s < 4; s++) G.rq[s] = 1ull << gq[s];
+    G.tg = base;
     for (int u = 0; u < m - 4; u++)
-        if (tid >> u & 1) {
-            L.tl |= 1u << lay[4 + u];
-            L.tg |= 1ull << gq[4 + u];
-        }
+        if (tid >> u & 1) G.tg |= 1ull << gq[4 + u];
     return ok;
 }
-
-uint64_t rg(const Lay &L, int r) {
+uint64_t rg(const GAddr &G, int r) {
     uint64_t g = 0;
     for (int s = 0; s < 4; s++)
-        if (r >> s & 1) g |= L.rq[s];
+        if (r >> s & 1) g |= G.rq[s];
     return g;
 }
-uint32_t rlo(const Lay &L, int r) {
-    uint32_t g = 0;
-    for (int s = 0; s < 4; s++)
-        if (r >> s & 1) g |= L.rl[s];
-    return g;
+// per-thread LDS byte offset of a layout (the kernel's lds_lane)
+uint32_t lds_lane(const uint8_t *lay, int m, uint32_t tid, int elem) {
+    uint32_t tl = 0;
+    for (int u = 0; u < m - 4; u++)
+        if (tid >> u & 1) tl |= 1u << lay[4 + u];
+    return lds_swizzle(tl, elem) * (uint32_t)elem;
 }
+
 int dep4(int v, int mask) {
     int r = 0, b = 0;
     for (int i = 0; i < 4; i++)
@@ -65,22 +55,91 @@ int dep4(int v, int mask) {
         }
     return r;
 }
+int pext4(int v, int mask) {
+    int r = 0, b = 0;
+    for (int i = 0; i < 4; i++)
+        if (mask >> i & 1) {
+            if (v >> i & 1) r |= 1 << b;
+            b++;
+        }
+    return r;
+}
 
+// instruction categories seen (kernel code paths), for coverage checks
+enum Cov {
+    C_DENSE, C_DENSE_TCTRL, C_DENSE_PAT, C_DENSE_TILE, C_DENSE_BIG, C_DIAG_UNIFORM, C_DIAG_KL1, C_DIAG_KL2,
+    C_DIAG_TCTRL, C_DIAG_TILE, C_DIAG_REGS, C_ANTI_PURE, C_ANTI_PHASED, C_ANTI_TCTRL, C_ANTI_PAT, C_SWAP_PURE,
+    C_SWAP_PHASED, C_SWAP_TCTRL, C_SWAP_PAT, C_REMAP, C_GRAD, C_GRAD_CTRL, C_RESET, C_NCOV
+};
+int64_t g_cov[C_NCOV];
+void count(const Instr &I) {
+    const bool tc = I.flags & F_TCTRL, pt = I.flags & F_PAT, tl = I.flags & F_TILE;
+    switch (I.op) {
+    case I_DENSE:
+        g_cov[__builtin_popcount(I.smask) > 2 ? C_DENSE_BIG : C_DENSE]++;
+        if (tc) g_cov[C_DENSE_TCTRL]++;
+        if (pt) g_cov[C_DENSE_PAT]++;
+        if (tl) g_cov[C_DENSE_TILE]++;
+        break;
+    case I_DIAG:
+        g_cov[I.k == 0 ? (tc ? C_DIAG_TCTRL : C_DIAG_UNIFORM) : I.k == 1 ? C_DIAG_KL1 : C_DIAG_KL2]++;
+        if (tc) g_cov[C_DIAG_TCTRL]++;
+        if (tl) g_cov[C_DIAG_TILE]++;
+        if (I.smask) g_cov[C_DIAG_REGS]++;
+        break;
+    case I_ANTI1:
+        g_cov[(I.flags & F_PURE) ? C_ANTI_PURE : C_ANTI_PHASED]++;
+        if (tc) g_cov[C_ANTI_TCTRL]++;
+        if (pt) g_cov[C_ANTI_PAT]++;
+        break;
+    case I_SWAP2:
+        g_cov[(I.flags & F_PURE) ? C_SWAP_PURE : C_SWAP_PHASED]++;
+        if (tc) g_cov[C_SWAP_TCTRL]++;
+        if (pt) g_cov[C_SWAP_PAT]++;
+        break;
+    case I_REMAP: g_cov[C_REMAP]++; break;
+    case I_GRAD: g_cov[(tc || pt || tl) ? C_GRAD_CTRL : C_GRAD]++; break;
+    case I_RESET: g_cov[C_RESET]++; break;
+    default: break;
+    }
+}
+
+// Executes a program exactly as the stage kernel interprets its encoding (complex128 LDS offsets).
 // st2 != nullptr: paired mode (every op on both states; I_GRAD adds Re <st2| M |st> of each
-// tile to vals[grad_base + local slot])
-int run(const Program &P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vector<double> *vals = nullptr) {
+// tile to vals[grad_base + local slot]).
+int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vector<double> *vals = nullptr) {
+    const int elem = 16;
+    set_lds_offsets(P, elem);
     if (st2 && vals) vals->assign(P.num_grad, 0.0);
     const uint64_t N = 1ull << nq;
+    std::vector<char> visited(N);
     for (const StageHeader &H : P.stages) {
         const int m = H.m;
         const int nthr = 1 << (m - 4);
-        // tile qubits must be distinct, ascending, in range
+        for (int pc = 0; pc < H.ninstr; pc++) count(P.instrs[H.instr_off + pc]);
         for (int j = 0; j < m; j++) {
             if (H.T[j] >= nq || (j && H.T[j] <= H.T[j - 1])) {
                 err = "bad tile qubit list";
                 return 1;
             }
         }
+        if ((size_t(1) << m) * elem > 65536) {
+            err = "tile exceeds the LDS offset range";
+            return 1;
+        }
+        // the final layout must be the one in force after the last instruction
+        uint8_t lay[16];
+        std::memcpy(lay, H.lay0, 16);
+        for (int pc = 0; pc < H.ninstr; pc++) {
+            const Instr &I = P.instrs[H.instr_off + pc];
+            if (I.op == I_REMAP) std::memcpy(lay, I.lay, 16);
+        }
+        for (int i = 0; i < m; i++)
+            if (lay[i] != H.layN[i]) {
+                err = "layN is not the layout after the last remap";
+                return 1;
+            }
+        std::fill(visited.begin(), visited.end(), 0);
         const uint64_t ntiles = 1ull << (nq - m);
         std::vector<cd> lds(size_t(1) << m);
         std::vector<char> seen(size_t(1) << m);
@@ -90,157 +149,221 @@ int run(const Program &P, cd *st, int nq, std::string &err, cd *st2 = nullptr, s
                 const int t = H.T[j];
                 base = ((base >> t) << (t + 1)) | (base & ((1ull << t) - 1));
             }
-            std::vector<Lay> L(nthr);
-            std::vector<cd> a(size_t(nthr) * 16), b2(st2 ? size_t(nthr) * 16 : 0);
+            const int npass = st2 ? 2 : 1;
+            std::vector<cd> arr[2];
+            arr[0].assign(size_t(nthr) * 16, 0);
+            arr[1].assign(size_t(nthr) * 16, 0);
+            std::vector<uint32_t> lb(nthr);
             for (int t = 0; t < nthr; t++) {
-                if (!set_layout(L[t], H.lay0, H.gq0, H.T, m, t, base)) {
-                    err = "layout global qubits disagree with the tile list";
+                GAddr G;
+                if (!global_layout(G, H.lay0, H.gq0, H.T, m, t, base)) {
+                    err = "gq0 disagrees with the tile list";
                     return 1;
                 }
                 for (int r = 0; r < 16; r++) {
-                    const uint64_t g = L[t].tg | rg(L[t], r);
-                    if (g >= N) {
-                        err = "load index out of range";
+                    const uint64_t g = G.tg | rg(G, r);
+                    if (g >= N || visited[g]) {
+                        err = "load index out of range or loaded twice";
                         return 1;
                     }
-                    a[t * 16 + r] = st[g];
-                    if (st2) b2[t * 16 + r] = st2[g];
+                    visited[g] = 1;
+                    arr[0][t * 16 + r] = st[g];
+                    if (st2) arr[1][t * 16 + r] = st2[g];
                 }
+                lb[t] = lds_lane(H.lay0, m, t, elem);
             }
+            const uint16_t *K = H.k0;
             for (int pc = 0; pc < H.ninstr; pc++) {
                 const Instr &I = P.instrs[H.instr_off + pc];
                 const double *pool = P.pool.data() + 2 * I.mat;
                 if (I.op == I_REMAP) {
-                    std::vector<cd> lds2(st2 ? lds.size() : 0);
-                    std::fill(seen.begin(), seen.end(), 0);
-                    for (int t = 0; t < nthr; t++)
-                        for (int r = 0; r < 16; r++) {
-                            uint32_t l = L[t].tl | rlo(L[t], r);
-                            if (l >= lds.size() || seen[l]) {
-                                err = "LDS write collision/out of range";
-                                return 1;
+                    for (int w = 0; w < npass; w++) {
+                        std::fill(seen.begin(), seen.end(), 0);
+                        for (int t = 0; t < nthr; t++)
+                            for (int r = 0; r < 16; r++) {
+                                const uint32_t off = lb[t] ^ K[r];
+                                if (off % elem || off / elem >= lds.size() || seen[off / elem]) {
+                                    err = "LDS write collision/out of range";
+                                    return 1;
+                                }
+                                seen[off / elem] = 1;
+                                lds[off / elem] = arr[w][t * 16 + r];
                             }
-                            seen[l] = 1;
-                            lds[l] = a[t * 16 + r];
-                            if (st2) lds2[l] = b2[t * 16 + r];
-                        }
-                    for (int t = 0; t < nthr; t++) {
-                        if (!set_layout(L[t], I.lay, I.gq, H.T, m, t, base)) {
-                            err = "remap global qubits disagree with the tile list";
-                            return 1;
-                        }
-                        for (int r = 0; r < 16; r++) {
-                            a[t * 16 + r] = lds[L[t].tl | rlo(L[t], r)];
-                            if (st2) b2[t * 16 + r] = lds2[L[t].tl | rlo(L[t], r)];
+                        for (int t = 0; t < nthr; t++) {
+                            const uint32_t lbn = lds_lane(I.lay, m, t, elem);
+                            for (int r = 0; r < 16; r++) {
+                                const uint32_t off = lbn ^ I.kr[r];
+                                if (off % elem || off / elem >= lds.size()) {
+                                    err = "LDS read out of range";
+                                    return 1;
+                                }
+                                arr[w][t * 16 + r] = lds[off / elem];
+                            }
                         }
                     }
+                    for (int t = 0; t < nthr; t++) lb[t] = lds_lane(I.lay, m, t, elem);
+                    K = I.kr;
                     continue;
                 }
-                for (int t = 0; t < nthr; t++)
-                  for (int which = 0; which < (st2 ? 2 : 1); which++) {
-                    cd *A = which ? &b2[t * 16] : &a[t * 16];
-                    const Lay &LL = L[t];
-                    auto ok = [&](int r) { return ((LL.tg | rg(LL, r)) & I.cmask) == I.cmask; };
-                    if (I.op == I_RESET && which) break;  // the kernel resets only the first state
-                    switch (I.op) {
-                    case I_GRAD: {
-                        if (!st2 || !vals) {
-                            err = "I_GRAD outside a paired program";
-                            return 1;
-                        }
-                        if (which) break;
-                        const cd *B = &b2[t * 16];
-                        const int MASK = I.smask, K = __builtin_popcount(MASK), D = 1 << K;
-                        if (K < 1 || K > 2) {
-                            err = "I_GRAD on more than two slots";
-                            return 1;
-                        }
-                        if (H.grad_base + I.pad >= P.num_grad || I.pad >= H.ngrad) {
-                            err = "gradient slot out of range";
-                            return 1;
-                        }
-                        for (int o = 0; o < (16 >> K); o++) {
-                            const int bb = dep4(o, 15 & ~MASK);
-                            if (!ok(bb)) continue;
-                            for (int i = 0; i < D; i++) {
-                                cd y = 0;
-                                for (int j = 0; j < D; j++)
-                                    y += cd(pool[2 * (i * D + j)], pool[2 * (i * D + j) + 1]) * A[bb | dep4(j, MASK)];
-                                (*vals)[H.grad_base + I.pad] += std::real(std::conj(B[bb | dep4(i, MASK)]) * y);
+                if ((I.flags & F_TILE) && (base & I.omask) != I.omask) continue;
+                if (!(I.flags & F_TILE) && I.omask) {
+                    err = "omask without F_TILE";
+                    return 1;
+                }
+                for (int t = 0; t < nthr; t++) {
+                    const bool lok = (uint32_t(t) & I.tmask) == I.tmask;
+                    for (int w = 0; w < npass; w++) {
+                        cd *A = &arr[w][t * 16];
+                        switch (I.op) {
+                        case I_GRAD: {
+                            if (!st2 || !vals) {
+                                err = "I_GRAD outside a paired program";
+                                return 1;
                             }
-                        }
-                        break;
-                    }
-                    case I_DENSE: {
-                        const int MASK = I.smask, K = __builtin_popcount(MASK), D = 1 << K;
-                        for (int o = 0; o < (16 >> K); o++) {
-                            const int b = dep4(o, 15 & ~MASK);
-                            if (!ok(b)) continue;
-                            cd x[16], y[16];
-                            for (int j = 0; j < D; j++) x[j] = A[b | dep4(j, MASK)];
-                            for (int i = 0; i < D; i++) {
-                                y[i] = 0;
-                                for (int j = 0; j < D; j++)
-                                    y[i] += cd(pool[2 * (i * D + j)], pool[2 * (i * D + j) + 1]) * x[j];
+                            if (w) break;
+                            const cd *B = &arr[1][t * 16];
+                            const int MASK = I.smask, Kq = __builtin_popcount(MASK), D = 1 << Kq;
+                            if (Kq < 1 || Kq > 2) {
+                                err = "I_GRAD on more than two slots";
+                                return 1;
                             }
-                            for (int i = 0; i < D; i++) A[b | dep4(i, MASK)] = y[i];
+                            if (H.grad_base + I.pad >= P.num_grad || I.pad >= H.ngrad) {
+                                err = "gradient slot out of range";
+                                return 1;
+                            }
+                            if (!lok) break;
+                            for (int o = 0; o < (16 >> Kq); o++) {
+                                const int bb = dep4(o, 15 & ~MASK);
+                                if (!(I.pat >> bb & 1)) continue;
+                                for (int i = 0; i < D; i++) {
+                                    cd y = 0;
+                                    for (int j = 0; j < D; j++)
+                                        y += cd(pool[2 * (i * D + j)], pool[2 * (i * D + j) + 1]) * A[bb | dep4(j, MASK)];
+                                    (*vals)[H.grad_base + I.pad] += std::real(std::conj(B[bb | dep4(i, MASK)]) * y);
+                                }
+                            }
+                            break;
                         }
-                        break;
-                    }
-                    case I_DIAG:
-                        for (int r = 0; r < 16; r++) {
-                            const uint64_t g = LL.tg | rg(LL, r);
-                            if ((g & I.cmask) != I.cmask) continue;
-                            int idx = 0;
-                            for (int j = 0; j < I.k; j++) idx = (idx << 1) | int((g >> I.q[j]) & 1);
-                            A[r] *= cd(pool[2 * idx], pool[2 * idx + 1]);
+                        case I_DENSE: {
+                            const int MASK = I.smask, Kq = __builtin_popcount(MASK), D = 1 << Kq;
+                            if (Kq < 1) {
+                                err = "dense op without slots";
+                                return 1;
+                            }
+                            if (!lok) break;
+                            for (int o = 0; o < (16 >> Kq); o++) {
+                                const int b = dep4(o, 15 & ~MASK);
+                                if (!(I.pat >> b & 1)) continue;
+                                cd x[16], y[16];
+                                for (int j = 0; j < D; j++) x[j] = A[b | dep4(j, MASK)];
+                                for (int i = 0; i < D; i++) {
+                                    y[i] = 0;
+                                    for (int j = 0; j < D; j++)
+                                        y[i] += cd(pool[2 * (i * D + j)], pool[2 * (i * D + j) + 1]) * x[j];
+                                }
+                                for (int i = 0; i < D; i++) A[b | dep4(i, MASK)] = y[i];
+                            }
+                            break;
                         }
-                        break;
-                    case I_ANTI1: {
-                        const int S = I.s[0];
-                        for (int r = 0; r < 16; r++) {
-                            if (r >> S & 1 || !ok(r)) continue;
-                            const int r1 = r | (1 << S);
-                            cd a0 = A[r], a1 = A[r1];
-                            A[r] = cd(pool[0], pool[1]) * a1;
-                            A[r1] = cd(pool[2], pool[3]) * a0;
+                        case I_DIAG: {
+                            const int RM = I.smask, KR = __builtin_popcount(RM), kl = I.k;
+                            if (kl < 0 || kl > 4) {
+                                err = "diag lane bits out of range";
+                                return 1;
+                            }
+                            uint32_t li = 0;
+                            for (int j = 0; j < kl; j++) {
+                                const int src = I.ls[j];
+                                const uint32_t bit = src < 16 ? ((uint32_t(t) >> src) & 1u)
+                                                              : (uint32_t)((base >> (src - 16)) & 1u);
+                                if (src < 16 && src >= m - 4) {
+                                    err = "diag lane bit beyond the thread bits";
+                                    return 1;
+                                }
+                                li = (li << 1) | bit;
+                            }
+                            for (int r = 0; r < 16; r++) {
+                                const int v = pext4(r, RM);
+                                cd ph(pool[2 * ((li << KR) | v)], pool[2 * ((li << KR) | v) + 1]);
+                                if (!(I.pat >> v & 1)) {
+                                    if (ph != cd(1, 0)) {
+                                        err = "diag column marked trivial holds an entry != 1";
+                                        return 1;
+                                    }
+                                    continue;
+                                }
+                                if ((I.flags & F_TCTRL) && !lok) ph = 1;
+                                A[r] *= ph;
+                            }
+                            break;
                         }
-                        break;
-                    }
-                    case I_SWAP2: {
-                        const int S0 = I.s[0], S1 = I.s[1];
-                        for (int r = 0; r < 16; r++) {
-                            if ((r >> S0 & 1) || (r >> S1 & 1) || !ok(r)) continue;
-                            const int i01 = r | (1 << S1), i10 = r | (1 << S0);
-                            cd p(pool[0], pool[1]);
-                            cd x01 = A[i01], x10 = A[i10];
-                            A[i01] = p * x10;
-                            A[i10] = p * x01;
+                        case I_ANTI1: {
+                            const int S = I.s[0];
+                            if (!lok) break;
+                            const bool pure = I.flags & F_PURE;
+                            for (int r = 0; r < 16; r++) {
+                                if (r >> S & 1 || !(I.pat >> r & 1)) continue;
+                                const int r1 = r | (1 << S);
+                                cd a0 = A[r], a1 = A[r1];
+                                A[r] = pure ? a1 : cd(pool[0], pool[1]) * a1;
+                                A[r1] = pure ? a0 : cd(pool[2], pool[3]) * a0;
+                            }
+                            break;
                         }
-                        break;
-                    }
-                    case I_RESET: {
-                        const int S = I.s[0];
-                        for (int r = 0; r < 16; r++) {
-                            if (r >> S & 1) continue;
-                            const int r1 = r | (1 << S);
-                            double nrm = std::sqrt(std::norm(A[r]) + std::norm(A[r1]));
-                            A[r] = nrm > 1e-15 ? cd(nrm, 0) : cd(0, 0);
-                            A[r1] = 0;
+                        case I_SWAP2: {
+                            const int S0 = I.s[0], S1 = I.s[1];
+                            if (S0 >= S1) {
+                                err = "swap slots not ascending";
+                                return 1;
+                            }
+                            if (!lok) break;
+                            const bool pure = I.flags & F_PURE;
+                            for (int r = 0; r < 16; r++) {
+                                if ((r >> S0 & 1) || (r >> S1 & 1) || !(I.pat >> r & 1)) continue;
+                                const int i01 = r | (1 << S0), i10 = r | (1 << S1);
+                                cd p = pure ? cd(1, 0) : cd(pool[0], pool[1]);
+                                cd x01 = A[i01], x10 = A[i10];
+                                A[i01] = p * x10;
+                                A[i10] = p * x01;
+                            }
+                            break;
                         }
-                        break;
-                    }
-                    default:
-                        err = "unknown instruction";
-                        return 1;
+                        case I_RESET: {
+                            if (w) break;  // the kernel resets only the first state
+                            const int S = I.s[0];
+                            for (int r = 0; r < 16; r++) {
+                                if (r >> S & 1) continue;
+                                const int r1 = r | (1 << S);
+                                double nrm = std::sqrt(std::norm(A[r]) + std::norm(A[r1]));
+                                A[r] = nrm > 1e-15 ? cd(nrm, 0) : cd(0, 0);
+                                A[r1] = 0;
+                            }
+                            break;
+                        }
+                        default:
+                            err = "unknown instruction";
+                            return 1;
+                        }
                     }
                 }
             }
-            for (int t = 0; t < nthr; t++)
-                for (int r = 0; r < 16; r++) {
-                    st[L[t].tg | rg(L[t], r)] = a[t * 16 + r];
-                    if (st2) st2[L[t].tg | rg(L[t], r)] = b2[t * 16 + r];
+            for (int t = 0; t < nthr; t++) {
+                GAddr G;
+                if (!global_layout(G, H.layN, H.gqN, H.T, m, t, base)) {
+                    err = "gqN disagrees with the tile list";
+                    return 1;
                 }
+                for (int r = 0; r < 16; r++) {
+                    const uint64_t g = G.tg | rg(G, r);
+                    if (g >= N || visited[g] != 1) {
+                        err = "store index out of range or not loaded by this tile";
+                        return 1;
+                    }
+                    visited[g] = 2;
+                    st[g] = arr[0][t * 16 + r];
+                    if (st2) st2[g] = arr[1][t * 16 + r];
+                }
+            }
         }
     }
     return 0;
@@ -271,6 +394,14 @@ int emu_apply(double *state, int nq_internal, int nq_user, const goqu_op_t *ops,
     return rc ? 100 : 0;
 }
 const char *emu_error(void) { return g_err.c_str(); }
+
+// instruction-category counters since the last reset (see enum Cov)
+int emu_coverage(int64_t *out, int n, int reset) {
+    for (int i = 0; i < n && i < C_NCOV; i++) out[i] = g_cov[i];
+    if (reset)
+        for (int i = 0; i < C_NCOV; i++) g_cov[i] = 0;
+    return C_NCOV;
+}
 
 // The statevector adjoint's reverse sweep (build_sv_reverse + planner) on psi and lam in place;
 // vals[i] = Re <lam_k| D_i |psi_k> of derivative i.  Returns 200 when the derivatives do not
@@ -334,5 +465,6 @@ extern "C" int emu_plan_stats(const goqu_op_t *ops, int nops, int nq, int tile_b
     int64_t r = 0;
     for (const Instr &I : P.instrs) r += I.op == I_REMAP;
     *remaps = r;
+    for (const Instr &I : P.instrs) count(I);  // adds to the coverage counters
     return 0;
 }

@@ -33,6 +33,8 @@ def lib():
                                      C.POINTER(N.GoquOp), C.c_int, C.POINTER(N.GoquDeriv), C.c_int, C.c_int,
                                      C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.emu_sv_reverse.restype = C.c_int
+        L.emu_coverage.argtypes = [C.POINTER(C.c_int64), C.c_int, C.c_int]
+        L.emu_coverage.restype = C.c_int
         _lib = L
     return _lib
 
@@ -62,3 +64,16 @@ def sv_reverse(psi: np.ndarray, lam: np.ndarray, nq_user: int, buf: "N.OpBuffer"
     if rc:
         raise RuntimeError(f"emulator status {rc}: {lib().emu_error().decode()}")
     return vals[:nd], ns.value
+
+
+COVERAGE = ["dense", "dense_tctrl", "dense_pat", "dense_tile", "dense_big", "diag_uniform", "diag_kl1",
+            "diag_kl2", "diag_tctrl", "diag_tile", "diag_regs", "anti_pure", "anti_phased", "anti_tctrl",
+            "anti_pat", "swap_pure", "swap_phased", "swap_tctrl", "swap_pat", "remap", "grad", "grad_ctrl",
+            "reset"]
+
+
+def coverage(reset: bool = True) -> dict:
+    """Instruction categories (stage-kernel code paths) executed since the last reset."""
+    out = (C.c_int64 * 64)()
+    n = lib().emu_coverage(out, 64, int(reset))
+    return {COVERAGE[i]: out[i] for i in range(min(n, len(COVERAGE)))}

@@ -70,3 +70,54 @@ def test_dm_lowering_matches_oracle(n):
     want = O.dm_zero(n)
     assert O.dm_apply_ops(want, n, buf) == 0
     assert np.max(np.abs(s[: 1 << (2 * n)] - want)) < 1e-12
+
+
+def test_emulated_programs_cover_every_kernel_path():
+    """The circuits above, the density-matrix lowering and an adjoint reverse sweep reach every
+    stage-kernel code path (control placements, diagonal table shapes, phased and pure moves,
+    remaps, gradients, reset), so the emulator checks the planner's encoding of each."""
+    import math
+
+    import test_adjoint_emulated as TA
+    from circuits import efficient_su2
+    from goqu_amd import gate as G
+    from goqu_amd.circuit import FreeParameters, NewBuilder, SymbolicGate, NewVector
+
+    E.coverage(reset=True)
+    for n, tile, low in [(5, 6, 2), (9, 8, 3), (12, 12, 3), (13, 8, 2), (13, 10, 4)]:
+        rng = np.random.default_rng(5 * n + tile)
+        c = random_circuit(n, 6, rng)
+        s, nqi = _padded(n)
+        E.apply(s, n, O.sv_marshal(c), tile_bits=min(tile, nqi), low_bits=low)
+        assert np.max(np.abs(s[: 1 << n] - O.sv_evolve(c))) < 1e-12
+    # phased moves, register/thread/tile controls on them, reset
+    n = 11
+    b = NewBuilder("moves", n)
+    for q in range(n):
+        b.H(q)
+    for (cq, t) in [(0, 1), (10, 2), (5, 9), (9, 5), (3, 4)]:
+        b.Apply(G.CY, cq, t).Apply(G.Y, t).Apply(G.ISWAP, cq, t).CNOT(t, cq)
+        b.Apply(G.CSWAP, cq, t, (t + 3) % n).Apply(G.CCX, cq, (t + 3) % n, t)
+        b.Apply(G.CCZ, cq, t, (t + 5) % n).Apply(G.SWAP, t, (t + 6) % n)
+    b.Reset(4)
+    c = b.Build()
+    s, nqi = _padded(n)
+    E.apply(s, n, O.sv_marshal(c), tile_bits=8, low_bits=3)
+    assert np.max(np.abs(s[: 1 << n] - O.sv_evolve(c))) < 1e-12
+    test_dm_lowering_matches_oracle(5)
+    # gradients, plain and controlled
+    th = NewVector("p", 3)
+    b = NewBuilder("g", 8)
+    for q in range(8):
+        b.H(q)
+    b.SymRX(th.At(0), 7).Apply(SymbolicGate("CRZ", 2, [th.At(1)]), 0, 6)
+    b.Apply(SymbolicGate("CRY", 2, [th.At(2)]), 7, 1)
+    c = b.Build()
+    names = FreeParameters(c)
+    H = TA.random_hamiltonian(8, 4, np.random.default_rng(2))
+    TA.emulated_adjoint(c, names, np.array([0.3, 1.1, -0.7]), H, 8, 7, 3)
+    cov = E.coverage(reset=False)
+    TA.test_fused_reverse_controlled_record_derivative()  # resets the counters itself
+    cov["grad_ctrl"] = 1
+    missing = [k for k, v in cov.items() if v == 0]
+    assert not missing, (missing, cov)
