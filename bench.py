@@ -9,7 +9,8 @@ Printed JSON line:
   value     = HBM sweep throughput of the whole step: stage sweeps x 32 B x 2^30 / step time
   roofline  = the stage kernel (k_stage): algorithmic bytes per launch (read + write of the
               whole state, 32 B x 2^30) / its mean launch duration from HIP events recorded on
-              its stream, against the 8 TB/s HBM peak
+              its stream, against the 8 TB/s HBM peak; traffic = HBM bytes per launch from the
+              committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary under profiles/
   cpu_baseline = the oracle (a C restatement of goqu's CPU statevector with its worker policy)
               applying the first gates of the same circuit at 30 qubits on the host cores;
               bytes per gate as for the GPU metric (32 B x 2^n, a quarter for CZ/CP, half for
@@ -31,6 +32,18 @@ sys.path.insert(0, str(ROOT / "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_stage_statevector_30q_c128.md"
+
+
+def measured_traffic(n: int, layers: int):
+    """HBM bytes per k_stage launch from the committed FETCH_SIZE / WRITE_SIZE summary (calibrated
+    against sweeps of known size), for the workload that summary covers; None otherwise."""
+    if (n, layers) != (30, 40) or not TRAFFIC_PROFILE.exists():
+        return None
+    for line in TRAFFIC_PROFILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
 
 
 def gate_bytes(op, n: int) -> float:
@@ -198,7 +211,7 @@ def main() -> int:
                    "qubits": n, "layers": args.layers, "gates": ngates, "stage_sweeps": sweeps_per_step,
                    "gate_equiv_GBps": round(gate_equiv, 1), "norm2": norm},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(n, args.layers),
                      "kernel": "k_stage<double>", "avg_launch_ms": round(avg_launch_ms, 4)},
         "cpu_baseline": None,
     }
