@@ -133,6 +133,119 @@ def adjoint_section(qubits: int, reps: int, terms: int, ps_params: int, seed: in
             "speedup": round(t_ps / t_adj, 1), "max_abs_diff_vs_param_shift": float(np.max(np.abs(g[:k] - ps)))}
 
 
+def sampling_section(n: int = 20, layers: int = 20, shots: int = 1024) -> dict:
+    """20-qubit layered circuit (RX/RY/RZ per qubit + CNOT brickwork, 1,580 gates), MeasureAll,
+    1,024 shots: Evolve + device CDF + sampling, absolute time (the 16 MiB state is cache-resident,
+    so no HBM roofline applies)."""
+    from circuits import config1_circuit
+    from goqu_amd import statevector
+
+    c = config1_circuit(n, layers)
+    sim = statevector.New(n)
+    rng = np.random.default_rng(11)
+    sim.Run(c, shots, rng=rng)  # warm-up
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        counts = sim.Run(c, shots, rng=rng)
+    dt = (time.perf_counter() - t0) / reps
+    sim.Close()
+    return {"workload": f"{n}q {layers}-layer RX/RY/RZ + CNOT brickwork, MeasureAll, {shots} shots, c128",
+            "ms_per_run": round(dt * 1e3, 3), "distinct_outcomes": len(counts)}
+
+
+def qaoa_batch_section(num: int = 4096, seed: int = 5) -> dict:
+    """4,096 MaxCut instances, n ~ U{10..22}, G(n, 0.3), p ~ U{1..4}, gamma/beta ~ U[0, pi):
+    energy and all 2p gradient entries of every instance in one captured launch sequence.
+    Device bytes count what the tile kernels move for instances above 12 qubits (192 B x 2^n per
+    layer: two forward sweeps of psi, two reverse sweeps of psi and lambda); smaller instances
+    stay in LDS."""
+    import math
+
+    from goqu_amd import qaoa as Q
+
+    rng = np.random.default_rng(seed)
+    insts = []
+    for _ in range(num):
+        n = int(rng.integers(10, 23))
+        while True:
+            edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < 0.3]
+            if edges:
+                break
+        p = int(rng.integers(1, 5))
+        insts.append(Q.MaxCutInstance(n, edges, p, rng.uniform(0, math.pi, 2 * p)))
+    Q.BatchMaxCut(insts[:64])  # warm-up (module load, graph machinery)
+    t0 = time.perf_counter()
+    energies, grads = Q.BatchMaxCut(insts)
+    dt = time.perf_counter() - t0
+    big = [it for it in insts if it.num_qubits > 12]
+    dev_bytes = sum(192.0 * (1 << it.num_qubits) * it.layers for it in big)
+    return {"workload": f"{num} MaxCut instances, n~U{{10..22}}, G(n,0.3), p~U{{1..4}}, energy + 2p gradients, c128",
+            "s": round(dt, 4), "instances_per_s": round(num / dt, 1),
+            "large_instance_GBps": round(dev_bytes / dt / 1e9, 1),
+            "mean_energy": float(np.mean(energies))}
+
+
+def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
+    """14-qubit density matrix (4 GiB): per layer U3 on every qubit and 7 CNOTs on random disjoint
+    pairs, Depolarizing1Q(1e-3) / Depolarizing2Q(1e-2) default errors and an AmplitudeDamping(1e-3)
+    Delay per qubit at each layer end.  Forward: Evolve + <sum Z_i>, value = stage sweeps x
+    32 B x 4^n / time.  Gradient: <sum Z_i> and all 3 x 14 x layers U3 angles by the
+    checkpointed density-matrix adjoint."""
+    from goqu_amd import densitymatrix
+    from goqu_amd import gradient as GR
+    from goqu_amd import noise as NZ
+    from goqu_amd import _native as N
+    from goqu_amd.circuit import Bind, FreeParameters, NewBuilder, NewVector
+    from goqu_amd.pauli import NewPauliString, NewPauliSum, Z
+
+    rng = np.random.default_rng(seed)
+    th = NewVector("t", 3 * n * layers)
+    b = NewBuilder("dm", n)
+    k = 0
+    for _ in range(layers):
+        for q in range(n):
+            b.SymU3(th.At(k), th.At(k + 1), th.At(k + 2), q)
+            k += 3
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            b.CNOT(int(perm[i]), int(perm[i + 1]))
+        for q in range(n):
+            b.Delay(q, 1.0)
+    c = b.Build()
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(1e-3))
+    nm.AddDefaultError(2, NZ.Depolarizing2Q(1e-2))
+    nm.AddGateError("delay", NZ.AmplitudeDamping(1e-3))
+    H = NewPauliSum([NewPauliString(1.0, {q: Z}, n) for q in range(n)])
+    names = FreeParameters(c)
+    x = rng.uniform(0, 2 * np.pi, len(names))
+    bound = Bind(c, dict(zip(names, x)))
+    dm = densitymatrix.New(n).WithNoise(nm)
+    dm.Evolve(bound)  # warm-up
+    N.profile_read()
+    N.profile(True)
+    t0 = time.perf_counter()
+    dm.Evolve(bound)
+    e = dm.ExpectPauliSum(H)
+    t_fwd = time.perf_counter() - t0
+    kms, launches, _ = N.profile_read()
+    N.profile(False)
+    dm.Close()
+    sweep = 32.0 * (1 << (2 * n))
+    out = {"workload": f"{n}q density matrix, {layers} layers U3 + 7 CNOT + depolarizing/amplitude-damping noise, c128",
+           "forward_ms": round(t_fwd * 1e3, 2), "stage_sweeps": launches,
+           "forward_GBps": round(launches * sweep / t_fwd / 1e9, 1),
+           "k_stage_avg_ms": round(kms / max(launches, 1), 3), "energy": e}
+    g = GR.AdjointDensity(c, H, names, noise=nm, checkpoint_every=16)
+    t0 = time.perf_counter()
+    e2, grad = g.value_and_grad(x)
+    out["gradient_s"] = round(time.perf_counter() - t0, 3)
+    out["gradient_params"] = len(names)
+    out["gradient_energy_matches_forward"] = bool(abs(e2 - e) < 1e-9)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +258,8 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-qubits", type=int, default=24)
     ap.add_argument("--ps-params", type=int, default=16)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the 20q sampling, batched QAOA and density-matrix sections")
     ap.add_argument("--calib", action="store_true",
                     help="also run one read-only sweep (norm) and one write-only sweep (reset) of the state, "
                          "whose HBM bytes are known, to scale PMC byte counters")
@@ -217,6 +332,10 @@ def main() -> int:
     }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_section(args.adj_qubits, 7, 200, args.ps_params)
+    if not args.no_extra:
+        out["sampling"] = sampling_section()
+        out["qaoa_batch"] = qaoa_batch_section()
+        out["density_matrix"] = dm_section()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(circ, n, args.cpu_budget)
     print(json.dumps(out), flush=True)

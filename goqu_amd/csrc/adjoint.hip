@@ -213,7 +213,16 @@ int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_der
         int rc = lower_dm_ops(&ops[k], 1, n, low[k], err);
         if (rc) return fail(rc, "op " + std::to_string(k) + ": " + err);
     }
-    // forward with checkpoints
+    // forward with checkpoints; widen the spacing until they fit in free device memory (h, the
+    // work copy and two more states stay free)
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(GOQU_ERR_DEVICE, "hipMemGetInfo failed");
+        const size_t sbytes = rho.size() * rho.elem;
+        const size_t avail = fr > 4 * sbytes ? fr - 4 * sbytes : 0;
+        const int64_t max_ck = std::max<int64_t>(1, (int64_t)(avail / sbytes));
+        every = (int)std::max<int64_t>(every, (nops + max_ck - 1) / max_ck);
+    }
     const int nck = (nops + every - 1) / every;
     std::vector<State> ck(std::max(nck, 0));
     State h, work;

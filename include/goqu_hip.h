@@ -160,7 +160,9 @@ int goqu_sv_adjoint(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops,
 
 /* Noisy adjoint on the density matrix: the forward pass stores rho every `checkpoint_every`
  * ops and the reverse pass back-propagates the observable through adjoint channels,
- * recomputing rho from the nearest checkpoint.  grads[p] = d Tr(H rho)/d theta_p. */
+ * recomputing rho from the nearest checkpoint.  grads[p] = d Tr(H rho)/d theta_p.
+ * checkpoint_every is the densest spacing: the library widens it so that the checkpoints fit
+ * in free device memory. */
 int goqu_dm_adjoint(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops,
                     const goqu_deriv_t *derivs, int32_t num_derivs, int32_t num_params,
                     int32_t num_terms, const uint64_t *xmask, const uint64_t *zmask,
