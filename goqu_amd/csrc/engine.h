@@ -103,7 +103,7 @@ struct alignas(16) Instr {
     int32_t pad;      // GRAD: gradient slot within the stage
     uint8_t ls[4];    // DIAG: lane index bits, MSB first: < 16 thread-id bit, >= 16 qubit (ls-16) off the tile
     uint8_t lay[16];  // REMAP: tile-local bit of register slots 0..3, then of thread-id bits 0..m-5
-    uint16_t kr[16];  // REMAP: LDS byte offset of register index r in the new layout (swizzled)
+    uint16_t kr[16];  // REMAP: LDS byte offset (8-byte elements) of register index r in the new layout
 };
 static_assert(sizeof(Instr) == 96, "Instr must stay 96 bytes");
 
@@ -120,22 +120,22 @@ struct alignas(16) StageHeader {
     uint8_t gq0[16];      // global qubits of lay0
     uint8_t layN[16];     // final layout (the one in force after the last instruction)
     uint8_t gqN[16];      // global qubits of layN
-    uint16_t k0[16];      // LDS byte offsets of register index r in lay0 (swizzled)
+    uint16_t k0[16];      // LDS byte offsets (8-byte elements) of register index r in lay0
     int32_t grad_base;    // global gradient slot of local slot 0
     int32_t ngrad;        // gradient slots of this stage
-    int32_t pad[2];
+    int32_t flags;        // SF_BIG: holds a dense op on 3-4 register slots
+    int32_t pad;
 };
 static_assert(sizeof(StageHeader) == 144, "StageHeader layout");
+// StageHeader::flags
+constexpr int32_t SF_BIG = 1;
 
-// LDS element index swizzle of the stage kernel (complex elements of `elem` bytes): XOR-linear,
-// swz(a ^ b) = swz(a) ^ swz(b), so offsets split into a per-thread and a per-register part.
-// Lanes that differ in any three tile bits whose positions are distinct mod 3 (mod 4 for 8-byte
-// elements) land in distinct 16-byte bank groups.
-inline uint32_t lds_swizzle(uint32_t l, int elem) {
-    if (elem == 16) {
-        const uint32_t h = l >> 3;
-        return l ^ ((h ^ (h >> 3) ^ (h >> 6) ^ (h >> 9)) & 7u);
-    }
+// LDS element index swizzle of the stage kernel's remaps, for 8-byte elements (a complex64
+// amplitude, or one component of a complex128 amplitude: those move real parts and imaginary
+// parts in two passes).  XOR-linear, swz(a ^ b) = swz(a) ^ swz(b), so offsets split into a
+// per-thread and a per-register part.  Lanes that differ in any three tile bits whose positions
+// are distinct mod 4 land in distinct 8-byte bank pairs.
+inline uint32_t lds_swizzle(uint32_t l) {
     const uint32_t h = l >> 4;
     return l ^ ((h ^ (h >> 4) ^ (h >> 8)) & 15u);
 }
@@ -165,8 +165,8 @@ int lower_dm_ops(const void *ops, int nops, int n, std::vector<EOp> &out, std::s
 
 // Greedy stage planner + register allocation.
 Program plan_program(const std::vector<EOp> &eops, int nq_internal, const PlanConfig &cfg);
-// Fills Instr::kr and StageHeader::k0 for elements of `elem` bytes (16: complex128, 8: complex64).
-void set_lds_offsets(Program &P, int elem);
+// Fills Instr::kr and StageHeader::k0: LDS byte offsets of 8-byte elements.
+void set_lds_offsets(Program &P);
 
 // ---- adjoint differentiation programs (adjoint_plan.cpp) ----------------------------------
 // conjugate transpose of a unitary EOp (RESET has none)

@@ -827,7 +827,10 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                     }
                 I.smask = smask;
                 set_controls(I, e.cmask);
-                if (e.kind == EOP_DENSE && k > 2) I.flags |= F_OUTER;
+                if (e.kind == EOP_DENSE && k > 2) {
+                    I.flags |= F_OUTER;
+                    H.flags |= SF_BIG;
+                }
                 P.pool.insert(P.pool.end(), km.begin(), km.end());
                 break;
             }
@@ -851,13 +854,13 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
     return P;
 }
 
-void set_lds_offsets(Program &P, int elem) {
+void set_lds_offsets(Program &P) {
     auto offs = [&](const uint8_t *lay, uint16_t *out) {
         for (int r = 0; r < kRegAmps; r++) {
             uint32_t l = 0;
             for (int s = 0; s < kRegBits; s++)
                 if (r >> s & 1) l |= 1u << lay[s];
-            out[r] = (uint16_t)(lds_swizzle(l, elem) * (uint32_t)elem);
+            out[r] = (uint16_t)(lds_swizzle(l) * 8u);
         }
     };
     for (StageHeader &H : P.stages) {

@@ -13,6 +13,7 @@
 // host-computed per-register part (the swizzle is XOR-linear).  Global loads/stores use layouts
 // whose lanes 0..2 walk the tile's lowest qubits (runs of >= 8 amplitudes, 128 B at complex128).
 #include <atomic>
+#include <cstdlib>
 #include <mutex>
 
 #include "runtime.h"
@@ -40,16 +41,12 @@ __host__ __device__ constexpr int pext4(int v, int mask) {
 }
 __host__ __device__ constexpr int popc4(int m) { return (m & 1) + (m >> 1 & 1) + (m >> 2 & 1) + (m >> 3 & 1); }
 
-// the LDS swizzle of lds_swizzle() for complex elements of R
-template <typename R>
-__device__ __forceinline__ uint32_t swz(uint32_t l) {
-    if constexpr (sizeof(R) == 8) {
-        uint32_t h = l >> 3;
-        return l ^ ((h ^ (h >> 3) ^ (h >> 6) ^ (h >> 9)) & 7u);
-    } else {
-        uint32_t h = l >> 4;
-        return l ^ ((h ^ (h >> 4) ^ (h >> 8)) & 15u);
-    }
+// the LDS swizzle of lds_swizzle() for 8-byte elements.  Remaps move 8-byte LDS elements: a
+// complex64 amplitude, or one component of a complex128 amplitude (real parts in one pass,
+// imaginary parts in a second), so a 2^12 tile needs 32 KiB of LDS at either precision.
+__device__ __forceinline__ uint32_t swz8(uint32_t l) {
+    const uint32_t h = l >> 4;
+    return l ^ ((h ^ (h >> 4) ^ (h >> 8)) & 15u);
 }
 
 // byte j of four little-endian words (compile-time j after unrolling: scalar shifts)
@@ -91,7 +88,7 @@ __device__ __forceinline__ uint32_t lds_lane(const uint8_t *__restrict__ lay, in
 #pragma unroll
     for (int u = 0; u < 12; u++)
         if (u < m - 4 && (tid >> u & 1)) tl |= 1u << byte_of(l0, l1, l2, l3, 4 + u);
-    return swz<R>(tl) * (uint32_t)sizeof(cx<R>);
+    return swz8(tl) * 8u;
 }
 
 // per-register LDS byte offsets, two 16-bit values per word
@@ -185,8 +182,11 @@ __device__ __forceinline__ void swap_inplace(cx<R> &x, cx<R> &y) {
     swap_inplace(x.y, y.y);
 }
 
-// Dense op on the register slots of MASK (at most two).  CTRL: lanes whose thread-bit controls
-// fail get the identity (selected once per op), register groups outside `pat` are skipped.
+// Dense op on the register slots of MASK (at most two).  CTRL: register groups outside `pat` are
+// skipped; lanes whose thread-bit controls fail keep their amplitudes.  For one slot those lanes
+// get the identity matrix (selected once per op); for two slots the outputs are selected, which
+// keeps the 16-entry matrix in scalar registers (a per-lane copy would cost 64 VGPRs and lower
+// the occupancy of the whole kernel).
 template <int MASK, bool CTRL, bool PAIR, typename R>
 __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ M, uint32_t pat,
                                          bool lok) {
@@ -194,9 +194,11 @@ __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R
     constexpr int D = 1 << K;
     constexpr int OUTER = 16 >> K;
     constexpr int OM = 15 & ~MASK;
+    constexpr bool SEL_MATRIX = CTRL && K == 1;
+    constexpr bool SEL_OUTPUT = CTRL && K == 2;
     R mm[2 * D * D];
     static_for<0, 2 * D * D>([&](auto ic) __attribute__((always_inline)) { mm[decltype(ic)::value] = M[decltype(ic)::value]; });
-    if constexpr (CTRL) {
+    if constexpr (SEL_MATRIX) {
         static_for<0, D * D>([&](auto ic) __attribute__((always_inline)) {
             constexpr int e = decltype(ic)::value;
             constexpr bool dg = (e / D) == (e % D);
@@ -223,7 +225,8 @@ __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R
                         im = fma(mr, x[j].y, im);
                         im = fma(mi, x[j].x, im);
                     });
-                    y[i] = cx<R>{re, im};
+                    if constexpr (SEL_OUTPUT) y[i] = cx<R>{lok ? re : x[i].x, lok ? im : x[i].y};
+                    else y[i] = cx<R>{re, im};
                 });
                 static_for<0, D>([&](auto ic) __attribute__((always_inline)) {
                     commit(v[base | dep4(decltype(ic)::value, MASK)], y[decltype(ic)::value]);
@@ -466,24 +469,50 @@ __device__ __forceinline__ double op_grad(const cx<R> (&a)[16], const cx<R> (&b)
     return lok ? acc : 0.0;
 }
 
+// One remap through LDS: the 16 amplitudes leave by the old offsets (lb ^ K[r]) and arrive by
+// the new ones.  complex64: one pass of 8-byte amplitudes.  complex128: the real parts, then the
+// imaginary parts (each pass writes only registers the other pass does not read).
 template <typename R>
-__device__ __forceinline__ void lds_store(unsigned char *smem, const cx<R> (&v)[16], uint32_t lb, const KOff &K) {
-    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-        constexpr int r = decltype(rc)::value;
-        *reinterpret_cast<cx<R> *>(smem + (lb ^ koff<r>(K))) = v[r];
-    });
-}
-template <typename R>
-__device__ __forceinline__ void lds_load(const unsigned char *smem, cx<R> (&v)[16], uint32_t lb, const KOff &K) {
-    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-        constexpr int r = decltype(rc)::value;
-        v[r] = *reinterpret_cast<const cx<R> *>(smem + (lb ^ koff<r>(K)));
-    });
+__device__ __forceinline__ void lds_remap(unsigned char *smem, cx<R> (&v)[16], uint32_t lb, const KOff &K,
+                                          uint32_t lbn, const KOff &Kn) {
+    if constexpr (sizeof(R) == 4) {
+        __syncthreads();
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            *reinterpret_cast<cx<R> *>(smem + (lb ^ koff<r>(K))) = v[r];
+        });
+        __syncthreads();
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            v[r] = *reinterpret_cast<const cx<R> *>(smem + (lbn ^ koff<r>(Kn)));
+        });
+    } else {
+        __syncthreads();
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            *reinterpret_cast<R *>(smem + (lb ^ koff<r>(K))) = v[r].x;
+        });
+        __syncthreads();
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            v[r].x = *reinterpret_cast<const R *>(smem + (lbn ^ koff<r>(Kn)));
+        });
+        __syncthreads();
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            *reinterpret_cast<R *>(smem + (lb ^ koff<r>(K))) = v[r].y;
+        });
+        __syncthreads();
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            v[r].y = *reinterpret_cast<const R *>(smem + (lbn ^ koff<r>(Kn)));
+        });
+    }
 }
 
 // One stage over one state (PAIR = false) or over psi and lambda together (PAIR = true: every op
 // acts on both, and I_GRAD accumulates Re <lambda| M |psi> per gradient slot).
-template <typename R, bool PAIR>
+template <typename R, bool PAIR, bool BIG>
 __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__restrict__ st2,
                                            const StageHeader *__restrict__ hdr, const Instr *__restrict__ prog,
                                            const R *__restrict__ pool, uint64_t ntiles, double *__restrict__ gpart,
@@ -675,7 +704,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 }
                 continue;
             }
-            if (I.op == I_DENSE) {  // three or four slots
+            if (I.op == I_DENSE) {  // three or four slots (stages flagged SF_BIG only)
+                if constexpr (!BIG) continue;
                 if ((I.flags & F_TILE) && (base & I.omask) != I.omask) continue;
                 const bool lok = (tid & I.tmask) == I.tmask;
                 const R *__restrict__ M = pool + 2 * I.mat;
@@ -693,16 +723,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             const uint32_t lbn = lds_lane<R>(I.lay, m, tid);
             KOff Kn;
             load_koff(Kn, I.kr);
-            __syncthreads();
-            lds_store<R>(smem_raw, a, lb, K);
-            __syncthreads();
-            lds_load<R>(smem_raw, a, lbn, Kn);
-            if constexpr (PAIR) {
-                __syncthreads();
-                lds_store<R>(smem_raw, b, lb, K);
-                __syncthreads();
-                lds_load<R>(smem_raw, b, lbn, Kn);
-            }
+            lds_remap<R>(smem_raw, a, lb, K, lbn, Kn);
+            if constexpr (PAIR) lds_remap<R>(smem_raw, b, lb, K, lbn, Kn);
             lb = lbn;
             K = Kn;
         }
@@ -727,11 +749,19 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
     }
 }
 
+// Two instantiations: k_stage runs stages without 3-4-slot dense ops and so needs few enough
+// VGPRs for four waves per SIMD; k_stage_big runs the stages that hold them.
 template <typename R>
-__global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+__global__ void __launch_bounds__(256) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                uint64_t ntiles) {
-    stage_body<R, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+    stage_body<R, false, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+}
+template <typename R>
+__global__ void __launch_bounds__(256) k_stage_big(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+                                                   const Instr *__restrict__ prog, const R *__restrict__ pool,
+                                                   uint64_t ntiles) {
+    stage_body<R, false, true>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
 }
 
 template <typename R>
@@ -740,7 +770,7 @@ __global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<
                                                     const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                     uint64_t ntiles, double *__restrict__ gpart) {
     __shared__ double gacc[kMaxGradPerStage * 8];
-    stage_body<R, true>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
+    stage_body<R, true, true>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
 }
 
 // ---- profiling ---------------------------------------------------------------------------------
@@ -757,10 +787,16 @@ static void profile_add(double ms, int64_t launches, double bytes) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Tile size: 2^12 amplitudes (64 KiB of LDS at complex128, two workgroups per CU).
+// GOQU_TILE_BITS (8..12) overrides it, for tuning.
 PlanConfig default_plan_config(int prec, int nq) {
     PlanConfig c;
     c.tile_bits = 12;
     c.low_bits = 3;
+    if (const char *e = std::getenv("GOQU_TILE_BITS")) {
+        const int t = std::atoi(e);
+        if (t >= 8 && t <= 12) c.tile_bits = t;
+    }
     (void)prec;
     (void)nq;
     return c;
@@ -769,13 +805,12 @@ PlanConfig default_plan_config(int prec, int nq) {
 int upload_program(const Program &P0, int prec, hipStream_t s, DeviceProgram &dp) {
     const size_t rsz = prec == GOQU_C128 ? sizeof(double) : sizeof(float);
     for (const StageHeader &H : P0.stages)
-        if ((size_t(1) << H.m) * 2 * rsz > 65536)
+        if ((size_t(1) << H.m) * 8 > 65536)
             return fail(GOQU_ERR_UNSUPPORTED, "stage tile larger than the LDS offset range");
-    // LDS offsets depend on the element size
     Program P;
     P.stages = P0.stages;
     P.instrs = P0.instrs;
-    set_lds_offsets(P, (int)(2 * rsz));
+    set_lds_offsets(P);
     const std::vector<double> &pool = P0.pool;
     const size_t nh = P.stages.size(), ni = P.instrs.size(), np = pool.size();
     dp.hdr_off = 0;
@@ -817,7 +852,7 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
         if (dp.stages[i].ninstr == 0) continue;
         const uint64_t ntiles = 1ull << (st.nq - m);
         const unsigned threads = 1u << (m - kRegBits);
-        const size_t lds = (size_t(1) << m) * sizeof(cx<R>);
+        const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
         const uint64_t cap = 1ull << 20;
         const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
         if (prof) {
@@ -826,8 +861,12 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
             GOQU_HIP(hipEventCreate(&ev[ev.size() - 1]));
             GOQU_HIP(hipEventRecord(ev[ev.size() - 2], s));
         }
-        hipLaunchKernelGGL(k_stage<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(st.amps),
-                           hdrs + i, instrs, pool, ntiles);
+        if (dp.stages[i].flags & SF_BIG)
+            hipLaunchKernelGGL(k_stage_big<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(st.amps),
+                               hdrs + i, instrs, pool, ntiles);
+        else
+            hipLaunchKernelGGL(k_stage<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(st.amps),
+                               hdrs + i, instrs, pool, ntiles);
         GOQU_HIP(hipGetLastError());
         if (prof) GOQU_HIP(hipEventRecord(ev.back(), s));
     }
@@ -890,7 +929,7 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
         const StageHeader &H = P.stages[i];
         if (H.ninstr == 0) continue;
         const unsigned threads = 1u << (m - kRegBits);
-        const size_t lds = (size_t(1) << m) * sizeof(cx<R>);
+        const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
         hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),
                            static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
         if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "paired stage launch failed");

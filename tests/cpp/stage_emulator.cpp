@@ -39,11 +39,11 @@ uint64_t rg(const GAddr &G, int r) {
     return g;
 }
 // per-thread LDS byte offset of a layout (the kernel's lds_lane)
-uint32_t lds_lane(const uint8_t *lay, int m, uint32_t tid, int elem) {
+uint32_t lds_lane(const uint8_t *lay, int m, uint32_t tid) {
     uint32_t tl = 0;
     for (int u = 0; u < m - 4; u++)
         if (tid >> u & 1) tl |= 1u << lay[4 + u];
-    return lds_swizzle(tl, elem) * (uint32_t)elem;
+    return lds_swizzle(tl) * 8u;
 }
 
 int dep4(int v, int mask) {
@@ -104,12 +104,13 @@ void count(const Instr &I) {
     }
 }
 
-// Executes a program exactly as the stage kernel interprets its encoding (complex128 LDS offsets).
+// Executes a program exactly as the stage kernel interprets its encoding (complex128: remaps move
+// 8-byte LDS elements, the real parts in one pass and the imaginary parts in a second).
 // st2 != nullptr: paired mode (every op on both states; I_GRAD adds Re <st2| M |st> of each
 // tile to vals[grad_base + local slot]).
 int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vector<double> *vals = nullptr) {
-    const int elem = 16;
-    set_lds_offsets(P, elem);
+    const int elem = 8;
+    set_lds_offsets(P);
     if (st2 && vals) vals->assign(P.num_grad, 0.0);
     const uint64_t N = 1ull << nq;
     std::vector<char> visited(N);
@@ -141,7 +142,7 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
             }
         std::fill(visited.begin(), visited.end(), 0);
         const uint64_t ntiles = 1ull << (nq - m);
-        std::vector<cd> lds(size_t(1) << m);
+        std::vector<double> lds(size_t(1) << m);  // 8-byte elements
         std::vector<char> seen(size_t(1) << m);
         for (uint64_t tile = 0; tile < ntiles; tile++) {
             uint64_t base = tile;
@@ -170,38 +171,41 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                     arr[0][t * 16 + r] = st[g];
                     if (st2) arr[1][t * 16 + r] = st2[g];
                 }
-                lb[t] = lds_lane(H.lay0, m, t, elem);
+                lb[t] = lds_lane(H.lay0, m, t);
             }
             const uint16_t *K = H.k0;
             for (int pc = 0; pc < H.ninstr; pc++) {
                 const Instr &I = P.instrs[H.instr_off + pc];
                 const double *pool = P.pool.data() + 2 * I.mat;
                 if (I.op == I_REMAP) {
-                    for (int w = 0; w < npass; w++) {
-                        std::fill(seen.begin(), seen.end(), 0);
-                        for (int t = 0; t < nthr; t++)
-                            for (int r = 0; r < 16; r++) {
-                                const uint32_t off = lb[t] ^ K[r];
-                                if (off % elem || off / elem >= lds.size() || seen[off / elem]) {
-                                    err = "LDS write collision/out of range";
-                                    return 1;
+                    for (int w = 0; w < npass; w++)
+                        for (int comp = 0; comp < 2; comp++) {
+                            std::fill(seen.begin(), seen.end(), 0);
+                            for (int t = 0; t < nthr; t++)
+                                for (int r = 0; r < 16; r++) {
+                                    const uint32_t off = lb[t] ^ K[r];
+                                    if (off % elem || off / elem >= lds.size() || seen[off / elem]) {
+                                        err = "LDS write collision/out of range";
+                                        return 1;
+                                    }
+                                    seen[off / elem] = 1;
+                                    const cd v = arr[w][t * 16 + r];
+                                    lds[off / elem] = comp ? v.imag() : v.real();
                                 }
-                                seen[off / elem] = 1;
-                                lds[off / elem] = arr[w][t * 16 + r];
-                            }
-                        for (int t = 0; t < nthr; t++) {
-                            const uint32_t lbn = lds_lane(I.lay, m, t, elem);
-                            for (int r = 0; r < 16; r++) {
-                                const uint32_t off = lbn ^ I.kr[r];
-                                if (off % elem || off / elem >= lds.size()) {
-                                    err = "LDS read out of range";
-                                    return 1;
+                            for (int t = 0; t < nthr; t++) {
+                                const uint32_t lbn = lds_lane(I.lay, m, t);
+                                for (int r = 0; r < 16; r++) {
+                                    const uint32_t off = lbn ^ I.kr[r];
+                                    if (off % elem || off / elem >= lds.size()) {
+                                        err = "LDS read out of range";
+                                        return 1;
+                                    }
+                                    cd &v = arr[w][t * 16 + r];
+                                    v = comp ? cd(v.real(), lds[off / elem]) : cd(lds[off / elem], v.imag());
                                 }
-                                arr[w][t * 16 + r] = lds[off / elem];
                             }
                         }
-                    }
-                    for (int t = 0; t < nthr; t++) lb[t] = lds_lane(I.lay, m, t, elem);
+                    for (int t = 0; t < nthr; t++) lb[t] = lds_lane(I.lay, m, t);
                     K = I.kr;
                     continue;
                 }
@@ -248,6 +252,11 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                             const int MASK = I.smask, Kq = __builtin_popcount(MASK), D = 1 << Kq;
                             if (Kq < 1) {
                                 err = "dense op without slots";
+                                return 1;
+                            }
+                            // the lean stage kernel does not run 3-4-slot dense ops
+                            if (Kq > 2 && (!(H.flags & SF_BIG) || !(I.flags & F_OUTER))) {
+                                err = "3-4-slot dense op in a stage not flagged SF_BIG / F_OUTER";
                                 return 1;
                             }
                             if (!lok) break;
