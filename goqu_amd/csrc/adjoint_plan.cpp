@@ -21,7 +21,8 @@ using cd = std::complex<double>;
 int invert_eop(const EOp &e, EOp &inv, std::string &err) {
     inv = e;
     switch (e.kind) {
-    case EOP_DENSE: {
+    case EOP_DENSE:
+    case EOP_PARITY: {  // conjugate transpose of the full matrix (keeps the parity blocks)
         const int d = 1 << e.k;
         for (int r = 0; r < d; r++)
             for (int c = 0; c < d; c++) {

@@ -30,6 +30,9 @@ enum EOpKind : int32_t {
     EOP_SWAP2 = 3,  // 2 targets: |01> <- p * |10>, |10> <- p * |01>  (SWAP, iSWAP, CSWAP)
     EOP_RESET = 4,  // 1 target: deterministic reset (a0 <- |(a0,a1)|, a1 <- 0)
     EOP_GRAD = 5,   // paired states only: Re <lambda| D |psi> on k targets into gradient slot `slot`
+    EOP_PARITY = 6, // density-matrix channel superoperator that preserves each qubit's row^column
+                    // parity: k = 2 (q = row, col) or 4 (q = row a, row b, col a, col b); m holds
+                    // the full 2^k x 2^k superoperator (q[0] = MSB), zero across parity classes
 };
 
 // Gradient slots a single stage may hold (per-wave partials live in LDS).
@@ -72,6 +75,7 @@ enum InstrOp : int32_t {
     I_RESET = 4,  // slot s[0]
     I_REMAP = 5,  // new layout lay[0..m-1] (register slots 0..3, then thread bits), LDS offsets kr
     I_GRAD = 6,   // like I_DENSE, but accumulates Re <b| M |a> into local gradient slot `pad`
+    I_PARITY = 7, // parity-block superoperator on 2 or 4 register slots (canonical form below)
 };
 
 // Instr::flags
@@ -80,6 +84,14 @@ constexpr int32_t F_TCTRL = 2;  // controls on thread-id bits (tmask)
 constexpr int32_t F_PAT = 4;    // controls on register slots (pat)
 constexpr int32_t F_TILE = 8;   // controls on qubits outside the tile (omask), uniform per tile
 constexpr int32_t F_OUTER = 16; // run by the kernel outside its inner op loop (REMAP, RESET, dense on 3-4 slots)
+constexpr int32_t F_REAL = 32;  // I_PARITY: every block entry is real (only real parts are read)
+
+// I_PARITY canonical form (the kernel's view).  k = 2: slots lo < hi (smask); class
+// p = bit lo ^ bit hi; element i of class p has bit lo = i, bit hi = i ^ p; the pool holds
+// 2 blocks of 2x2 (class-major, row-major).  k = 4: all four slots, paired as
+// s[0]-s[1] and s[2]-s[3] (s[0] = 0); class c = 2 pa + pb with pa = bit s0 ^ bit s1,
+// pb = bit s2 ^ bit s3; element i = 2 ia + ib has bit s0 = ia, bit s1 = ia ^ pa, bit s2 = ib,
+// bit s3 = ib ^ pb; the pool holds 4 blocks of 4x4.
 
 // One instruction.  The planner resolves every qubit an op reads against the register layout in
 // force when the op runs, so the kernel never maps global qubits itself:

@@ -268,6 +268,27 @@ static std::vector<double> conj_of(const double *m, int n) {
     return r;
 }
 
+// A channel superoperator on (rows, cols) = q[0..k/2-1], q[k/2..k-1] (k = 2 or 4, q[0] = MSB of
+// the index).  Channels whose superoperator keeps every qubit's row^column parity (Pauli and
+// depolarizing channels, amplitude and phase damping, reset) become EOP_PARITY: block-diagonal
+// over the parity classes, at 1/2 (one qubit) or 1/4 (two qubits) of the dense cost.
+static bool parity_preserving(const double *m, int k) {
+    const int D = 1 << k, h = k / 2;
+    auto parity = [&](int x) { return (x >> h) ^ (x & ((1 << h) - 1)); };  // rows ^ cols, per qubit
+    for (int o = 0; o < D; o++)
+        for (int i = 0; i < D; i++)
+            if (parity(o) != parity(i) && (m[2 * (o * D + i)] != 0.0 || m[2 * (o * D + i) + 1] != 0.0)) return false;
+    return true;
+}
+
+static EOp make_superop(const int *q, int k, const double *m) {
+    const int D = 1 << k;
+    if (is_diagonal(m, D)) return make_diag(q, k, 0, m, D);
+    EOp e = make_dense(q, k, 0, m, D);
+    if (parity_preserving(m, k)) e.kind = EOP_PARITY;
+    return e;
+}
+
 int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std::string &err) {
     const goqu_op_t *ops = static_cast<const goqu_op_t *>(ops_v);
     out.clear();
@@ -329,7 +350,7 @@ int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std:
         if (op.kind == GOQU_OP_RESET) {
             const double S[32] = {1, 0, 0, 0, 0, 0, 1, 0};  // rho00 <- rho00 + rho11
             const int q2[2] = {rq[0], cq[0]};
-            out.push_back(make_dense(q2, 2, 0, S, 4));
+            out.push_back(make_superop(q2, 2, S));
             continue;
         }
         const bool channel = op.kind == GOQU_OP_CHANNEL && op.num_kraus > 1;
@@ -371,7 +392,7 @@ int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std:
             q4[2] = cq[0];
             q4[3] = cq[1];
         }
-        lower_matrix(q4, 2 * k, 0, Sm.data(), out);
+        out.push_back(make_superop(q4, 2 * k, Sm.data()));
     }
     return GOQU_OK;
 }
@@ -789,6 +810,62 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 if (e.pure_swap) I.flags |= F_PURE;
                 set_controls(I, e.cmask);
                 P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
+                break;
+            }
+            case EOP_PARITY: {
+                // blocks in the kernel's canonical form (see I_PARITY in engine.h): for each class
+                // and element, the slot bits, then the row/column bits they carry, then the
+                // superoperator index (q[0] = MSB)
+                I.op = I_PARITY;
+                int slot[4];
+                for (int j = 0; j < e.k; j++) slot[j] = slot_of(loc[e.q[j]]);
+                const int D = 1 << e.k;
+                auto entry = [&](int row_idx, int col_idx) { return cd(e.m[2 * (row_idx * D + col_idx)], e.m[2 * (row_idx * D + col_idx) + 1]); };
+                // superoperator index of the element whose register-slot bits are `bits`
+                auto sup_index = [&](int bits) {
+                    int idx = 0;
+                    for (int j = 0; j < e.k; j++) idx = (idx << 1) | ((bits >> slot[j]) & 1);
+                    return idx;
+                };
+                std::vector<cd> blocks;
+                if (e.k == 2) {
+                    const int lo = std::min(slot[0], slot[1]), hi = std::max(slot[0], slot[1]);
+                    I.smask = (1 << lo) | (1 << hi);
+                    for (int p = 0; p < 2; p++)
+                        for (int i = 0; i < 2; i++)
+                            for (int j = 0; j < 2; j++) {
+                                const int bi = (i << lo) | ((i ^ p) << hi), bj = (j << lo) | ((j ^ p) << hi);
+                                blocks.push_back(entry(sup_index(bi), sup_index(bj)));
+                            }
+                } else {
+                    // pairs (row a, col a) and (row b, col b); s0 = 0 and its partner, then the other two
+                    const int pa_slots[2] = {slot[0], slot[2]}, pb_slots[2] = {slot[1], slot[3]};
+                    const bool a_has0 = pa_slots[0] == 0 || pa_slots[1] == 0;
+                    const int *P0 = a_has0 ? pa_slots : pb_slots, *P1 = a_has0 ? pb_slots : pa_slots;
+                    const int s0 = 0, s1 = P0[0] == 0 ? P0[1] : P0[0];
+                    const int s2 = std::min(P1[0], P1[1]), s3 = std::max(P1[0], P1[1]);
+                    I.s[0] = (uint8_t)s0;
+                    I.s[1] = (uint8_t)s1;
+                    I.s[2] = (uint8_t)s2;
+                    I.s[3] = (uint8_t)s3;
+                    I.smask = 15;
+                    for (int c = 0; c < 4; c++) {
+                        const int pa = c >> 1, pb = c & 1;
+                        auto bits_of = [&](int i) {
+                            const int ia = i >> 1, ib = i & 1;
+                            return (ia << s0) | ((ia ^ pa) << s1) | (ib << s2) | ((ib ^ pb) << s3);
+                        };
+                        for (int i = 0; i < 4; i++)
+                            for (int j = 0; j < 4; j++) blocks.push_back(entry(sup_index(bits_of(i)), sup_index(bits_of(j))));
+                    }
+                }
+                bool real = true;
+                for (const cd &x : blocks) {
+                    real = real && x.imag() == 0.0;
+                    P.pool.push_back(x.real());
+                    P.pool.push_back(x.imag());
+                }
+                if (real) I.flags |= F_REAL;
                 break;
             }
             case EOP_GRAD:

@@ -436,6 +436,65 @@ __device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const In
     });
 }
 
+// Parity-block channel superoperators (I_PARITY, canonical form in engine.h).  y = B x within each
+// parity class; REAL reads only the real parts of B (2 FMAs per entry instead of 4).
+template <int N, bool REAL, typename R>
+__device__ __forceinline__ void block_apply(const R *__restrict__ B, const cx<R> (&x)[N], cx<R> (&y)[N]) {
+    static_for<0, N>([&](auto ic) __attribute__((always_inline)) {
+        constexpr int i = decltype(ic)::value;
+        R re = 0, im = 0;
+        static_for<0, N>([&](auto jc) __attribute__((always_inline)) {
+            constexpr int j = decltype(jc)::value;
+            const R br = B[2 * (i * N + j)];
+            re = fma(br, x[j].x, re);
+            im = fma(br, x[j].y, im);
+            if constexpr (!REAL) {
+                const R bi = B[2 * (i * N + j) + 1];
+                re = fma(-bi, x[j].y, re);
+                im = fma(bi, x[j].x, im);
+            }
+        });
+        y[i] = cx<R>{re, im};
+    });
+}
+
+// one-qubit channel: slots LO < HI, classes p = bit LO ^ bit HI, 2x2 blocks
+template <int LO, int HI, bool REAL, typename R>
+__device__ __forceinline__ void op_parity2(cx<R> (&a)[16], const R *__restrict__ M) {
+    constexpr int OM = 15 & ~((1 << LO) | (1 << HI));
+    static_for<0, 4>([&](auto gc) __attribute__((always_inline)) {
+        constexpr int base = dep4(decltype(gc)::value, OM);
+        static_for<0, 2>([&](auto pc) __attribute__((always_inline)) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int e0 = base | (p << HI), e1 = base | (1 << LO) | ((1 ^ p) << HI);
+            cx<R> x[2] = {a[e0], a[e1]}, y[2];
+            block_apply<2, REAL>(M + 8 * p, x, y);
+            commit(a[e0], y[0]);
+            commit(a[e1], y[1]);
+        });
+    });
+}
+
+// two-qubit channel: slot pairs (0, S1) and (S2, S3), classes c = 2 pa + pb, 4x4 blocks
+template <int S1, int S2, int S3, bool REAL, typename R>
+__device__ __forceinline__ void op_parity4(cx<R> (&a)[16], const R *__restrict__ M) {
+    static_for<0, 4>([&](auto cc) __attribute__((always_inline)) {
+        constexpr int c = decltype(cc)::value;
+        constexpr int pa = c >> 1, pb = c & 1;
+        auto bits = [](int i, int pa_, int pb_) constexpr {
+            const int ia = i >> 1, ib = i & 1;
+            return ia | ((ia ^ pa_) << S1) | (ib << S2) | ((ib ^ pb_) << S3);
+        };
+        constexpr int e0 = bits(0, pa, pb), e1 = bits(1, pa, pb), e2 = bits(2, pa, pb), e3 = bits(3, pa, pb);
+        cx<R> x[4] = {a[e0], a[e1], a[e2], a[e3]}, y[4];
+        block_apply<4, REAL>(M + 32 * c, x, y);
+        commit(a[e0], y[0]);
+        commit(a[e1], y[1]);
+        commit(a[e2], y[2]);
+        commit(a[e3], y[3]);
+    });
+}
+
 // Re sum over the op's groups of conj(b) (M a): the derivative term of the adjoint method.
 // Lanes whose thread-bit controls fail and groups outside `pat` contribute nothing.
 template <int MASK, typename R>
@@ -662,6 +721,44 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                         break;
                     }
     #undef SWAP_CASE
+                } else if (op == I_PARITY) {
+                    if constexpr (!PAIR) {  // density-matrix programs only
+                        const bool real = flags & F_REAL;
+                        if (I.k == 2) {
+#define PAR2_CASE(LO, HI)                                                   \
+    case (1 << LO) | (1 << HI):                                             \
+        if (real) op_parity2<LO, HI, true>(a, M);                           \
+        else op_parity2<LO, HI, false>(a, M);                               \
+        break;
+                            switch (I.smask) {
+                                PAR2_CASE(0, 1)
+                                PAR2_CASE(0, 2)
+                                PAR2_CASE(0, 3)
+                                PAR2_CASE(1, 2)
+                                PAR2_CASE(1, 3)
+                                PAR2_CASE(2, 3)
+                            default:
+                                break;
+                            }
+#undef PAR2_CASE
+                        } else {
+                            // pairing by the partner of slot 0 (the other two slots ascending)
+                            switch ((sw >> 8) & 255u) {
+                            case 1:
+                                if (real) op_parity4<1, 2, 3, true>(a, M);
+                                else op_parity4<1, 2, 3, false>(a, M);
+                                break;
+                            case 2:
+                                if (real) op_parity4<2, 1, 3, true>(a, M);
+                                else op_parity4<2, 1, 3, false>(a, M);
+                                break;
+                            default:
+                                if (real) op_parity4<3, 1, 2, true>(a, M);
+                                else op_parity4<3, 1, 2, false>(a, M);
+                                break;
+                            }
+                        }
+                    }
                 } else if (op == I_GRAD) {
                     if constexpr (PAIR) {
                         double v = 0;

@@ -69,7 +69,8 @@ int pext4(int v, int mask) {
 enum Cov {
     C_DENSE, C_DENSE_TCTRL, C_DENSE_PAT, C_DENSE_TILE, C_DENSE_BIG, C_DIAG_UNIFORM, C_DIAG_KL1, C_DIAG_KL2,
     C_DIAG_TCTRL, C_DIAG_TILE, C_DIAG_REGS, C_ANTI_PURE, C_ANTI_PHASED, C_ANTI_TCTRL, C_ANTI_PAT, C_SWAP_PURE,
-    C_SWAP_PHASED, C_SWAP_TCTRL, C_SWAP_PAT, C_REMAP, C_GRAD, C_GRAD_CTRL, C_RESET, C_NCOV
+    C_SWAP_PHASED, C_SWAP_TCTRL, C_SWAP_PAT, C_REMAP, C_GRAD, C_GRAD_CTRL, C_RESET, C_PARITY2, C_PARITY4,
+    C_PARITY_REAL, C_NCOV
 };
 int64_t g_cov[C_NCOV];
 void count(const Instr &I) {
@@ -100,6 +101,10 @@ void count(const Instr &I) {
     case I_REMAP: g_cov[C_REMAP]++; break;
     case I_GRAD: g_cov[(tc || pt || tl) ? C_GRAD_CTRL : C_GRAD]++; break;
     case I_RESET: g_cov[C_RESET]++; break;
+    case I_PARITY:
+        g_cov[I.k == 2 ? C_PARITY2 : C_PARITY4]++;
+        if (I.flags & F_REAL) g_cov[C_PARITY_REAL]++;
+        break;
     default: break;
     }
 }
@@ -271,6 +276,56 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                                         y[i] += cd(pool[2 * (i * D + j)], pool[2 * (i * D + j) + 1]) * x[j];
                                 }
                                 for (int i = 0; i < D; i++) A[b | dep4(i, MASK)] = y[i];
+                            }
+                            break;
+                        }
+                        case I_PARITY: {
+                            if (st2) {
+                                err = "I_PARITY in a paired program";
+                                return 1;
+                            }
+                            const bool real = I.flags & F_REAL;
+                            auto B = [&](int off) { return real ? cd(pool[2 * off], 0) : cd(pool[2 * off], pool[2 * off + 1]); };
+                            if (I.k == 2) {
+                                const int lo = __builtin_ctz(I.smask), hi = 31 - __builtin_clz(I.smask);
+                                if (__builtin_popcount(I.smask) != 2) {
+                                    err = "I_PARITY k=2 needs two slots";
+                                    return 1;
+                                }
+                                const int om = 15 & ~I.smask;
+                                for (int g = 0; g < 4; g++) {
+                                    const int base = dep4(g, om);
+                                    for (int p = 0; p < 2; p++) {
+                                        const int e0 = base | (p << hi), e1 = base | (1 << lo) | ((1 ^ p) << hi);
+                                        const cd x0 = A[e0], x1 = A[e1];
+                                        A[e0] = B(4 * p + 0) * x0 + B(4 * p + 1) * x1;
+                                        A[e1] = B(4 * p + 2) * x0 + B(4 * p + 3) * x1;
+                                    }
+                                }
+                            } else if (I.k == 4) {
+                                const int s0 = I.s[0], s1 = I.s[1], s2 = I.s[2], s3 = I.s[3];
+                                if (s0 != 0 || s2 >= s3 || ((1 << s0) | (1 << s1) | (1 << s2) | (1 << s3)) != 15) {
+                                    err = "I_PARITY k=4 slots not canonical";
+                                    return 1;
+                                }
+                                for (int c = 0; c < 4; c++) {
+                                    const int pa = c >> 1, pb = c & 1;
+                                    int e[4];
+                                    cd x[4];
+                                    for (int i = 0; i < 4; i++) {
+                                        const int ia = i >> 1, ib = i & 1;
+                                        e[i] = (ia << s0) | ((ia ^ pa) << s1) | (ib << s2) | ((ib ^ pb) << s3);
+                                        x[i] = A[e[i]];
+                                    }
+                                    for (int i = 0; i < 4; i++) {
+                                        cd y = 0;
+                                        for (int j = 0; j < 4; j++) y += B(16 * c + 4 * i + j) * x[j];
+                                        A[e[i]] = y;
+                                    }
+                                }
+                            } else {
+                                err = "I_PARITY with k other than 2 or 4";
+                                return 1;
                             }
                             break;
                         }

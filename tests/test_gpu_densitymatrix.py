@@ -130,3 +130,36 @@ def test_dm_14q_trace_and_hermitian():
     for r in range(64):
         blk[r] = full[r * dim: r * dim + 64]
     assert np.max(np.abs(blk - blk.conj().T)) < 1e-13
+
+
+@pytest.mark.parametrize("n", [2, 4, 7])
+def test_dm_complex_parity_channels_and_dense_fallback(n):
+    """Channels with complex parity blocks (one and two qubits) and a parity-mixing channel
+    (dense superoperator), on the GPU against the oracle's Kraus loop."""
+    p = 0.3
+    k1 = np.sqrt(p) * np.array([[0, 1], [1j, 0]])
+    c1 = NZ.Channel("cplx1", 1, [np.sqrt(1 - p) * np.eye(2), k1])
+    k2 = np.sqrt(p) * np.kron(np.array([[0, 1], [1j, 0]]), np.array([[np.exp(0.4j), 0], [0, 1]]))
+    c2 = NZ.Channel("cplx2", 2, [np.sqrt(1 - p) * np.eye(4), k2])
+    th = 0.7
+    r = np.array([[np.cos(th), -1j * np.sin(th)], [-1j * np.sin(th), np.cos(th)]])
+    c3 = NZ.Channel("mix", 1, [np.sqrt(1 - p) * np.eye(2), np.sqrt(p) * r])
+    nm = NZ.New()
+    nm.AddDefaultError(1, c1)
+    nm.AddDefaultError(2, c2)
+    nm.AddGateError("H", c3)
+    rng = np.random.default_rng(90 + n)
+    b = NewBuilder("dmc", n)
+    for q in range(n):
+        b.U3(*rng.uniform(0, 2 * np.pi, 3), q)
+    b.CNOT(0, 1).H(n - 1).CZ(n - 2, n - 1)
+    if n > 3:
+        b.CNOT(3, 0).H(2)
+    c = b.Build()
+    dm = D.New(n).WithNoise(nm)
+    dm.Evolve(c)
+    got = dm.DensityMatrix().reshape(-1)
+    dm.Close()
+    want = O.dm_zero(n)
+    assert O.dm_apply_ops(want, n, D.marshal(c, nm)) == 0
+    assert np.max(np.abs(got - want)) <= TOL

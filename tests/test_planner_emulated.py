@@ -121,3 +121,56 @@ def test_emulated_programs_cover_every_kernel_path():
     cov["grad_ctrl"] = 1
     missing = [k for k, v in cov.items() if v == 0]
     assert not missing, (missing, cov)
+
+
+def _custom_channels():
+    """Kraus channels whose superoperators keep every qubit's row^column parity but have complex
+    blocks (anti-diagonal Kraus operators with complex entries), and one that does not keep it
+    (falls back to a dense superoperator)."""
+    from goqu_amd import noise as NZ
+
+    p = 0.3
+    k0 = np.sqrt(1 - p) * np.eye(2)
+    k1 = np.sqrt(p) * np.array([[0, 1], [1j, 0]]) / np.sqrt(1.0)  # K1^dag K1 = p I
+    c1 = NZ.Channel("cplx1", 1, [k0, k1])
+    k2 = np.sqrt(p) * np.kron(np.array([[0, 1], [1j, 0]]), np.array([[np.exp(0.4j), 0], [0, 1]]))
+    c2 = NZ.Channel("cplx2", 2, [np.sqrt(1 - p) * np.eye(4), k2])
+    # mixes parities: K = cos I + i sin X style rotation inside a mixture
+    th = 0.7
+    r = np.array([[np.cos(th), -1j * np.sin(th)], [-1j * np.sin(th), np.cos(th)]])
+    c3 = NZ.Channel("mix", 1, [np.sqrt(1 - p) * np.eye(2), np.sqrt(p) * r])
+    return c1, c2, c3
+
+
+@pytest.mark.parametrize("n", [2, 4])
+def test_dm_parity_blocks_complex_and_fallback(n):
+    """Parity-block superoperators with complex blocks (1q and 2q) and a parity-mixing channel
+    (dense fallback), planner + emulator against the oracle's Kraus loop."""
+    from goqu_amd import densitymatrix as D
+    from goqu_amd import gate as G
+    from goqu_amd import noise as NZ
+    from goqu_amd.circuit import NewBuilder
+
+    c1, c2, c3 = _custom_channels()
+    rng = np.random.default_rng(80 + n)
+    b = NewBuilder("dmc", n)
+    for q in range(n):
+        b.U3(*rng.uniform(0, 2 * np.pi, 3), q)
+    b.CNOT(0, 1).H(n - 1).CZ(n - 2, n - 1)
+    c = b.Build()
+    nm = NZ.New()
+    nm.AddDefaultError(1, c1)
+    nm.AddDefaultError(2, c2)
+    nm.AddGateError("H", c3)
+    buf = D.marshal(c, nm)
+    nqi = max(2 * n, 6)
+    s = np.zeros(1 << nqi, dtype=np.complex128)
+    s[0] = 1
+    E.coverage(reset=True)
+    E.apply(s, n, buf, tile_bits=min(12, nqi), low_bits=3, dm=True)
+    cov = E.coverage(reset=True)
+    want = O.dm_zero(n)
+    assert O.dm_apply_ops(want, n, buf) == 0
+    assert np.max(np.abs(s[: 1 << (2 * n)] - want)) < 1e-12
+    assert cov["parity2"] > 0 and cov["parity4"] > 0
+    assert cov["parity2"] + cov["parity4"] > cov["parity_real"]  # complex blocks ran
