@@ -84,7 +84,7 @@ constexpr int32_t F_TCTRL = 2;  // controls on thread-id bits (tmask)
 constexpr int32_t F_PAT = 4;    // controls on register slots (pat)
 constexpr int32_t F_TILE = 8;   // controls on qubits outside the tile (omask), uniform per tile
 constexpr int32_t F_OUTER = 16; // run by the kernel outside its inner op loop (REMAP, RESET, dense on 3-4 slots)
-constexpr int32_t F_REAL = 32;  // I_PARITY: every block entry is real (only real parts are read)
+constexpr int32_t F_REAL = 32;  // I_PARITY, I_DENSE: every matrix entry is real (only real parts are read)
 
 // I_PARITY canonical form (the kernel's view).  k = 2: slots lo < hi (smask); class
 // p = bit lo ^ bit hi; element i of class p has bit lo = i, bit hi = i ^ p; the pool holds

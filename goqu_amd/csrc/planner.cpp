@@ -457,6 +457,55 @@ static std::vector<EOp> fuse_single_qubit(const std::vector<EOp> &in) {
     return out;
 }
 
+// A unitary single-qubit matrix U = e^{ia} diag(1, e^{i phi}) R diag(1, e^{i lam}) with the real
+// rotation R = [[r, -s], [s, r]], r = |u00|, s = |u10|.  An uncontrolled one-qubit dense op
+// becomes DIAG, real DENSE, DIAG: the rotation needs half the FMAs of a complex matrix, and the
+// emission folds the diagonals into pending per-qubit phases applied in batches.
+static std::vector<EOp> split_rotations(const std::vector<EOp> &in) {
+    std::vector<EOp> out;
+    out.reserve(in.size() * 3);
+    for (const EOp &e : in) {
+        if (!(e.kind == EOP_DENSE && e.k == 1 && e.cmask == 0)) {
+            out.push_back(e);
+            continue;
+        }
+        const cd u00(e.m[0], e.m[1]), u01(e.m[2], e.m[3]), u10(e.m[4], e.m[5]), u11(e.m[6], e.m[7]);
+        const bool real = e.m[1] == 0.0 && e.m[3] == 0.0 && e.m[5] == 0.0 && e.m[7] == 0.0;
+        const double r = std::abs(u00), s = std::abs(u10);
+        // only unitary matrices with both a diagonal and an off-diagonal part
+        const bool unitary = std::abs(std::norm(u00) + std::norm(u10) - 1.0) < 1e-12 &&
+                             std::abs(std::norm(u01) + std::norm(u11) - 1.0) < 1e-12 &&
+                             std::abs(std::conj(u00) * u01 + std::conj(u10) * u11) < 1e-12;
+        if (real || !unitary || r < 1e-150 || s < 1e-150) {
+            out.push_back(e);
+            continue;
+        }
+        const cd ea = u00 / r, eaphi = u10 / s, ealam = -u01 / s;
+        EOp b;  // applied first: e^{ia} diag(1, e^{i lam})
+        b.kind = EOP_DIAG;
+        b.k = 1;
+        b.q[0] = e.q[0];
+        push(b.m, ea);
+        push(b.m, ealam);
+        EOp rot;
+        rot.kind = EOP_DENSE;
+        rot.k = 1;
+        rot.q[0] = e.q[0];
+        rot.m = {r, 0.0, -s, 0.0, s, 0.0, r, 0.0};
+        EOp a;  // applied last: diag(1, e^{i phi})
+        a.kind = EOP_DIAG;
+        a.k = 1;
+        a.q[0] = e.q[0];
+        push(a.m, cd(1, 0));
+        push(a.m, eaphi / ea);
+        out.push_back(std::move(b));
+        out.push_back(std::move(rot));
+        out.push_back(std::move(a));
+        (void)u11;
+    }
+    return out;
+}
+
 // ---- stage planning -----------------------------------------------------------------------
 namespace {
 
@@ -492,7 +541,7 @@ static Layout make_layout(int m, const int reg[4]) {
 Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &cfg) {
     Program P;
     P.nq = nq;
-    std::vector<EOp> eops = fuse_single_qubit(eops_in);
+    std::vector<EOp> eops = split_rotations(fuse_single_qubit(eops_in));
     P.num_eops = (int64_t)eops.size();
     const int m = std::min(cfg.tile_bits, nq);
     const int L = std::min(cfg.low_bits, m - kRegBits);
@@ -538,6 +587,10 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
     }
 
     // 2. per stage: local numbering, register allocation, instruction emission
+    std::vector<cd> ph(nq > 0 ? nq : 1, cd(1, 0));  // pending phase diag(1, ph[q]) per qubit
+    uint64_t pending = 0;
+    cd gphase(1, 0);
+    size_t stage_index = 0;
     for (const StageOps &S : stages) {
         StageHeader H;
         std::memset(&H, 0, sizeof(H));
@@ -573,14 +626,30 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         // still waiting; ready ops whose targets are register slots run first, and a remap is
         // emitted only when no ready op can run.
         std::vector<char> fin(nops, 0);
-        auto commute = [&](size_t a, size_t b) {
-            const EOp &x = eops[S.ops[a]], &y = eops[S.ops[b]];
-            const uint64_t shared = x.touch() & y.touch();
-            return (shared & (x.nondiag() | y.nondiag())) == 0;
+        // Readiness per qubit: op i may run when, on every qubit it acts on non-diagonally, no
+        // earlier unfinished op touches that qubit, and on every qubit it acts on diagonally, no
+        // earlier unfinished op acts non-diagonally (ops commute when all shared qubits are
+        // diagonal for both).  Per-qubit op lists with lazily advanced heads make a check cost
+        // O(qubits of the op).
+        std::vector<std::vector<uint32_t>> touch_list(64), nd_list(64);
+        std::vector<size_t> touch_head(64, 0), nd_head(64, 0);
+        for (size_t i = 0; i < nops; i++) {
+            const EOp &x = eops[S.ops[i]];
+            for (uint64_t t = x.touch(); t; t &= t - 1) touch_list[__builtin_ctzll(t)].push_back((uint32_t)i);
+            for (uint64_t t = x.nondiag(); t; t &= t - 1) nd_list[__builtin_ctzll(t)].push_back((uint32_t)i);
+        }
+        auto first_open = [&](const std::vector<uint32_t> &L, size_t &h) -> size_t {
+            while (h < L.size() && fin[L[h]]) h++;
+            return h < L.size() ? L[h] : nops;
         };
         auto ready = [&](size_t i) {
-            for (size_t j = 0; j < i; j++)
-                if (!fin[j] && !commute(i, j)) return false;
+            const EOp &x = eops[S.ops[i]];
+            const uint64_t nd = x.nondiag();
+            for (uint64_t t = x.touch(); t; t &= t - 1) {
+                const int q = __builtin_ctzll(t);
+                const size_t f = (nd >> q & 1) ? first_open(touch_list[q], touch_head[q]) : first_open(nd_list[q], nd_head[q]);
+                if (f < i) return false;
+            }
             return true;
         };
         // new register set: targets of the first ready op that needs registers, then of the
@@ -659,13 +728,281 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             if (I.omask) I.flags |= F_TILE;
         };
 
+        // one elementary op -> one instruction, resolved against the current layout
+        auto emit_eop = [&](const EOp &e) {
+                Instr I;
+                std::memset(&I, 0, sizeof(I));
+                I.k = e.k;
+                I.mat = (int32_t)(P.pool.size() / 2);
+                I.pat = 0xFFFFu;
+                switch (e.kind) {
+                case EOP_DIAG: {
+                    I.op = I_DIAG;
+                    // register slots of the diagonal qubits and of the controls index the table; the
+                    // other diagonal qubits become lane index bits, the other controls lane masks
+                    int rm = 0, ctl_slots = 0, kl = 0, lane_j[4] = {0, 0, 0, 0};
+                    for (int j = 0; j < e.k; j++) {
+                        const int q = e.q[j];
+                        const int sl = loc[q] >= 0 ? slot_of(loc[q]) : -1;
+                        if (sl >= 0) {
+                            rm |= 1 << sl;
+                        } else {
+                            I.ls[kl] = (uint8_t)(loc[q] < 0 ? 16 + q : tbit_of(loc[q]));
+                            lane_j[kl++] = j;
+                        }
+                    }
+                    uint64_t cm = e.cmask;
+                    while (cm) {
+                        const int q = __builtin_ctzll(cm);
+                        cm &= cm - 1;
+                        if (loc[q] < 0) {
+                            I.omask |= 1ull << q;
+                            continue;
+                        }
+                        const int sl = slot_of(loc[q]);
+                        if (sl >= 0) {
+                            rm |= 1 << sl;
+                            ctl_slots |= 1 << sl;
+                        } else {
+                            I.tmask |= 1u << tbit_of(loc[q]);
+                        }
+                    }
+                    if (I.tmask) I.flags |= F_TCTRL;
+                    if (I.omask) I.flags |= F_TILE;
+                    I.k = kl;
+                    I.smask = rm;
+                    const int kr = __builtin_popcount(rm);
+                    uint32_t nontriv = 0;
+                    std::vector<double> tab(2 << (kl + kr));
+                    for (int li = 0; li < (1 << kl); li++)
+                        for (int v = 0; v < (1 << kr); v++) {
+                            int sval[4] = {0, 0, 0, 0};
+                            for (int sl = 0, t = 0; sl < kRegBits; sl++)
+                                if (rm >> sl & 1) sval[sl] = (v >> t++) & 1;
+                            bool ok = true;
+                            for (int sl = 0; sl < kRegBits; sl++)
+                                if ((ctl_slots >> sl & 1) && !sval[sl]) ok = false;
+                            int idx = 0;
+                            for (int j = 0; j < e.k; j++) {
+                                const int q = e.q[j];
+                                const int sl = loc[q] >= 0 ? slot_of(loc[q]) : -1;
+                                int bit = 0;
+                                if (sl >= 0) {
+                                    bit = sval[sl];
+                                } else {
+                                    for (int p = 0; p < kl; p++)
+                                        if (lane_j[p] == j) bit = (li >> (kl - 1 - p)) & 1;
+                                }
+                                idx = (idx << 1) | bit;
+                            }
+                            const double re = ok ? e.m[2 * idx] : 1.0, im = ok ? e.m[2 * idx + 1] : 0.0;
+                            const int at = (li << kr) | v;
+                            tab[2 * at] = re;
+                            tab[2 * at + 1] = im;
+                            if (re != 1.0 || im != 0.0) nontriv |= 1u << v;
+                        }
+                    I.pat = nontriv;
+                    P.pool.insert(P.pool.end(), tab.begin(), tab.end());
+                    break;
+                }
+                case EOP_ANTI1:
+                case EOP_RESET: {
+                    I.op = e.kind == EOP_ANTI1 ? I_ANTI1 : I_RESET;
+                    I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
+                    if (e.pure_swap) I.flags |= F_PURE;
+                    if (e.kind == EOP_RESET) I.flags |= F_OUTER;
+                    set_controls(I, e.cmask);
+                    P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
+                    break;
+                }
+                case EOP_SWAP2: {
+                    I.op = I_SWAP2;
+                    const int s0 = slot_of(loc[e.q[0]]), s1 = slot_of(loc[e.q[1]]);
+                    I.s[0] = (uint8_t)std::min(s0, s1);
+                    I.s[1] = (uint8_t)std::max(s0, s1);
+                    if (e.pure_swap) I.flags |= F_PURE;
+                    set_controls(I, e.cmask);
+                    P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
+                    break;
+                }
+                case EOP_PARITY: {
+                    // blocks in the kernel's canonical form (see I_PARITY in engine.h): for each class
+                    // and element, the slot bits, then the row/column bits they carry, then the
+                    // superoperator index (q[0] = MSB)
+                    I.op = I_PARITY;
+                    int slot[4];
+                    for (int j = 0; j < e.k; j++) slot[j] = slot_of(loc[e.q[j]]);
+                    const int D = 1 << e.k;
+                    auto entry = [&](int row_idx, int col_idx) { return cd(e.m[2 * (row_idx * D + col_idx)], e.m[2 * (row_idx * D + col_idx) + 1]); };
+                    // superoperator index of the element whose register-slot bits are `bits`
+                    auto sup_index = [&](int bits) {
+                        int idx = 0;
+                        for (int j = 0; j < e.k; j++) idx = (idx << 1) | ((bits >> slot[j]) & 1);
+                        return idx;
+                    };
+                    std::vector<cd> blocks;
+                    if (e.k == 2) {
+                        const int lo = std::min(slot[0], slot[1]), hi = std::max(slot[0], slot[1]);
+                        I.smask = (1 << lo) | (1 << hi);
+                        for (int p = 0; p < 2; p++)
+                            for (int i = 0; i < 2; i++)
+                                for (int j = 0; j < 2; j++) {
+                                    const int bi = (i << lo) | ((i ^ p) << hi), bj = (j << lo) | ((j ^ p) << hi);
+                                    blocks.push_back(entry(sup_index(bi), sup_index(bj)));
+                                }
+                    } else {
+                        // pairs (row a, col a) and (row b, col b); s0 = 0 and its partner, then the other two
+                        const int pa_slots[2] = {slot[0], slot[2]}, pb_slots[2] = {slot[1], slot[3]};
+                        const bool a_has0 = pa_slots[0] == 0 || pa_slots[1] == 0;
+                        const int *P0 = a_has0 ? pa_slots : pb_slots, *P1 = a_has0 ? pb_slots : pa_slots;
+                        const int s0 = 0, s1 = P0[0] == 0 ? P0[1] : P0[0];
+                        const int s2 = std::min(P1[0], P1[1]), s3 = std::max(P1[0], P1[1]);
+                        I.s[0] = (uint8_t)s0;
+                        I.s[1] = (uint8_t)s1;
+                        I.s[2] = (uint8_t)s2;
+                        I.s[3] = (uint8_t)s3;
+                        I.smask = 15;
+                        for (int c = 0; c < 4; c++) {
+                            const int pa = c >> 1, pb = c & 1;
+                            auto bits_of = [&](int i) {
+                                const int ia = i >> 1, ib = i & 1;
+                                return (ia << s0) | ((ia ^ pa) << s1) | (ib << s2) | ((ib ^ pb) << s3);
+                            };
+                            for (int i = 0; i < 4; i++)
+                                for (int j = 0; j < 4; j++) blocks.push_back(entry(sup_index(bits_of(i)), sup_index(bits_of(j))));
+                        }
+                    }
+                    bool real = true;
+                    for (const cd &x : blocks) {
+                        real = real && x.imag() == 0.0;
+                        P.pool.push_back(x.real());
+                        P.pool.push_back(x.imag());
+                    }
+                    if (real) I.flags |= F_REAL;
+                    break;
+                }
+                case EOP_GRAD:
+                case EOP_DENSE: {
+                    I.op = e.kind == EOP_GRAD ? I_GRAD : I_DENSE;
+                    if (e.kind == EOP_GRAD) {
+                        I.pad = H.ngrad++;
+                        P.grad_slots.push_back(e.slot);
+                        P.num_grad++;
+                    }
+                    const int k = e.k, dim = 1 << k;
+                    int slot[4], smask = 0;
+                    for (int j = 0; j < k; j++) {
+                        slot[j] = slot_of(loc[e.q[j]]);
+                        smask |= 1 << slot[j];
+                    }
+                    // rank of each gate qubit among the sorted slots: kernel index bit rank[j]
+                    int rank[4];
+                    for (int j = 0; j < k; j++) {
+                        rank[j] = 0;
+                        for (int t = 0; t < k; t++)
+                            if (slot[t] < slot[j]) rank[j]++;
+                    }
+                    auto perm = [&](int g) {
+                        int r = 0;
+                        for (int j = 0; j < k; j++)
+                            if (g >> (k - 1 - j) & 1) r |= 1 << rank[j];
+                        return r;
+                    };
+                    std::vector<double> km(2 * dim * dim);
+                    for (int gr = 0; gr < dim; gr++)
+                        for (int gc = 0; gc < dim; gc++) {
+                            const int kr = perm(gr), kc = perm(gc);
+                            km[2 * (kr * dim + kc)] = e.m[2 * (gr * dim + gc)];
+                            km[2 * (kr * dim + kc) + 1] = e.m[2 * (gr * dim + gc) + 1];
+                        }
+                    I.smask = smask;
+                    set_controls(I, e.cmask);
+                    bool real = true;
+                    for (size_t t = 1; t < e.m.size(); t += 2) real = real && e.m[t] == 0.0;
+                    if (real) I.flags |= F_REAL;
+                    if (e.kind == EOP_DENSE && k > 2) {
+                        I.flags |= F_OUTER;
+                        H.flags |= SF_BIG;
+                    }
+                    P.pool.insert(P.pool.end(), km.begin(), km.end());
+                    break;
+                }
+
+                }
+                P.instrs.push_back(I);
+            };
+        // Pending phases: an uncontrolled one-qubit diagonal op is not emitted; its phase joins
+        // ph[q] (diag(1, ph[q]), the rest goes to the global phase).  Phases are applied only
+        // before an op that acts non-diagonally on their qubit, together with the pending phases
+        // of the qubits in registers at that moment (one diagonal op over up to four qubits).
+        auto emit_phase_group = [&](uint64_t group, cd scale) {
+            EOp d;
+            d.kind = EOP_DIAG;
+            d.k = 0;
+            for (uint64_t t = group; t; t &= t - 1) d.q[d.k++] = __builtin_ctzll(t);
+            for (int idx = 0; idx < (1 << d.k); idx++) {
+                cd v = scale;
+                for (int j = 0; j < d.k; j++)
+                    if (idx >> (d.k - 1 - j) & 1) v *= ph[d.q[j]];
+                d.m.push_back(v.real());
+                d.m.push_back(v.imag());
+            }
+            for (int j = 0; j < d.k; j++) ph[d.q[j]] = cd(1, 0);
+            pending &= ~group;
+            emit_eop(d);
+        };
+        auto next_group = [&](uint64_t must) {
+            uint64_t group = 0;
+            int cnt = 0;
+            for (uint64_t t = must; t && cnt < 4; t &= t - 1, cnt++) group |= t & (~t + 1);
+            for (int sl = 0; sl < kRegBits && cnt < 4; sl++) {
+                const int qq = H.T[cur.lay[sl]];
+                if ((pending >> qq & 1) && !(group >> qq & 1)) {
+                    group |= 1ull << qq;
+                    cnt++;
+                }
+            }
+            return group;
+        };
+        auto flush = [&](uint64_t must) {
+            must &= pending;
+            while (must) {
+                const uint64_t g = next_group(must);
+                emit_phase_group(g, cd(1, 0));
+                must &= ~g;
+            }
+        };
+        auto absorbable = [&](const EOp &e) {
+            if (!(e.kind == EOP_DIAG && e.k == 1 && e.cmask == 0)) return false;
+            const cd d0(e.m[0], e.m[1]), d1(e.m[2], e.m[3]);
+            return std::abs(std::abs(d0) - 1.0) < 1e-13 && std::abs(std::abs(d1) - 1.0) < 1e-13;
+        };
+        auto absorb = [&](const EOp &e) {
+            const cd d0(e.m[0], e.m[1]), d1(e.m[2], e.m[3]);
+            gphase *= d0;
+            ph[e.q[0]] *= d1 / d0;
+            pending |= 1ull << e.q[0];
+        };
         size_t left = nops;
+        size_t lo_open = 0;  // every op before it is finished
         while (left) {
+            while (lo_open < nops && fin[lo_open]) lo_open++;
+            // phases first (free, and they make the ops behind them ready together); the scan
+            // looks a bounded window ahead
+            bool any = false;
+            for (size_t j = lo_open; j < nops && j < lo_open + 256; j++)
+                if (!fin[j] && absorbable(eops[S.ops[j]]) && ready(j)) {
+                    absorb(eops[S.ops[j]]);
+                    fin[j] = 1;
+                    left--;
+                    any = true;
+                }
+            if (any) continue;
             // pick the first ready op that runs on the current registers
             uint32_t regset = 0;
             for (int s = 0; s < kRegBits; s++) regset |= 1u << cur.lay[s];
             size_t i = nops;
-            for (size_t j = 0; j < nops; j++)
+            for (size_t j = lo_open; j < nops; j++)
                 if (!fin[j] && (need[j] & ~regset) == 0 && ready(j)) {
                     i = j;
                     break;
@@ -679,6 +1016,10 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             fin[i] = 1;
             left--;
             const EOp &e = eops[S.ops[i]];
+            if (absorbable(e)) {
+                absorb(e);
+                continue;
+            }
             // Two uncontrolled one-qubit dense ops on different register slots run as one
             // two-slot op with the Kronecker product: the same 16 FMAs per amplitude, half the
             // register write-backs and one dispatch.
@@ -697,6 +1038,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                     fin[j] = 1;
                     left--;
                     const EOp &f = eops[S.ops[j]];
+                    flush(e.nondiag() | f.nondiag());
                     const int sa = slot_of(loc[e.q[0]]), sb = slot_of(loc[f.q[0]]);
                     const EOp &lo = sa < sb ? e : f, &hi = sa < sb ? f : e;
                     Instr I;
@@ -706,6 +1048,9 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                     I.mat = (int32_t)(P.pool.size() / 2);
                     I.pat = 0xFFFFu;
                     I.smask = (1 << sa) | (1 << sb);
+                    bool real = true;
+                    for (int t = 1; t < 8; t += 2) real = real && lo.m[t] == 0.0 && hi.m[t] == 0.0;
+                    if (real) I.flags |= F_REAL;
                     for (int r = 0; r < 4; r++)
                         for (int c = 0; c < 4; c++) {  // bit 0 <-> lower slot, bit 1 <-> higher slot
                             const cd x = cd(lo.m[2 * ((r & 1) * 2 + (c & 1))], lo.m[2 * ((r & 1) * 2 + (c & 1)) + 1]) *
@@ -717,202 +1062,33 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                     continue;
                 }
             }
-            Instr I;
-            std::memset(&I, 0, sizeof(I));
-            I.k = e.k;
-            I.mat = (int32_t)(P.pool.size() / 2);
-            I.pat = 0xFFFFu;
-            switch (e.kind) {
-            case EOP_DIAG: {
-                I.op = I_DIAG;
-                // register slots of the diagonal qubits and of the controls index the table; the
-                // other diagonal qubits become lane index bits, the other controls lane masks
-                int rm = 0, ctl_slots = 0, kl = 0, lane_j[4] = {0, 0, 0, 0};
-                for (int j = 0; j < e.k; j++) {
-                    const int q = e.q[j];
-                    const int sl = loc[q] >= 0 ? slot_of(loc[q]) : -1;
-                    if (sl >= 0) {
-                        rm |= 1 << sl;
-                    } else {
-                        I.ls[kl] = (uint8_t)(loc[q] < 0 ? 16 + q : tbit_of(loc[q]));
-                        lane_j[kl++] = j;
-                    }
-                }
-                uint64_t cm = e.cmask;
-                while (cm) {
-                    const int q = __builtin_ctzll(cm);
-                    cm &= cm - 1;
-                    if (loc[q] < 0) {
-                        I.omask |= 1ull << q;
-                        continue;
-                    }
-                    const int sl = slot_of(loc[q]);
-                    if (sl >= 0) {
-                        rm |= 1 << sl;
-                        ctl_slots |= 1 << sl;
-                    } else {
-                        I.tmask |= 1u << tbit_of(loc[q]);
-                    }
-                }
-                if (I.tmask) I.flags |= F_TCTRL;
-                if (I.omask) I.flags |= F_TILE;
-                I.k = kl;
-                I.smask = rm;
-                const int kr = __builtin_popcount(rm);
-                uint32_t nontriv = 0;
-                std::vector<double> tab(2 << (kl + kr));
-                for (int li = 0; li < (1 << kl); li++)
-                    for (int v = 0; v < (1 << kr); v++) {
-                        int sval[4] = {0, 0, 0, 0};
-                        for (int sl = 0, t = 0; sl < kRegBits; sl++)
-                            if (rm >> sl & 1) sval[sl] = (v >> t++) & 1;
-                        bool ok = true;
-                        for (int sl = 0; sl < kRegBits; sl++)
-                            if ((ctl_slots >> sl & 1) && !sval[sl]) ok = false;
-                        int idx = 0;
-                        for (int j = 0; j < e.k; j++) {
-                            const int q = e.q[j];
-                            const int sl = loc[q] >= 0 ? slot_of(loc[q]) : -1;
-                            int bit = 0;
-                            if (sl >= 0) {
-                                bit = sval[sl];
-                            } else {
-                                for (int p = 0; p < kl; p++)
-                                    if (lane_j[p] == j) bit = (li >> (kl - 1 - p)) & 1;
-                            }
-                            idx = (idx << 1) | bit;
-                        }
-                        const double re = ok ? e.m[2 * idx] : 1.0, im = ok ? e.m[2 * idx + 1] : 0.0;
-                        const int at = (li << kr) | v;
-                        tab[2 * at] = re;
-                        tab[2 * at + 1] = im;
-                        if (re != 1.0 || im != 0.0) nontriv |= 1u << v;
-                    }
-                I.pat = nontriv;
-                P.pool.insert(P.pool.end(), tab.begin(), tab.end());
-                break;
+            // pending phases pass uncontrolled moves: a swap exchanges the two qubits' phases,
+            // [a0, a1] <- [p0 a1, p1 a0] turns diag(1, f) into f diag(1, conj f)
+            if (e.kind == EOP_SWAP2 && e.cmask == 0) {
+                const int qa = e.q[0], qb = e.q[1];
+                std::swap(ph[qa], ph[qb]);
+                const uint64_t ba = pending >> qa & 1, bb = pending >> qb & 1;
+                pending = (pending & ~((1ull << qa) | (1ull << qb))) | (bb << qa) | (ba << qb);
+            } else if (e.kind == EOP_ANTI1 && e.cmask == 0 && (pending >> e.q[0] & 1)) {
+                gphase *= ph[e.q[0]];
+                ph[e.q[0]] = std::conj(ph[e.q[0]]);
+            } else {
+                flush(e.nondiag());
             }
-            case EOP_ANTI1:
-            case EOP_RESET: {
-                I.op = e.kind == EOP_ANTI1 ? I_ANTI1 : I_RESET;
-                I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
-                if (e.pure_swap) I.flags |= F_PURE;
-                if (e.kind == EOP_RESET) I.flags |= F_OUTER;
-                set_controls(I, e.cmask);
-                P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
-                break;
+            emit_eop(e);
+        }
+        if (++stage_index == stages.size()) {  // the last stage applies every pending phase
+            while (pending) {
+                const uint64_t g = next_group(pending);
+                const bool final_group = (pending & ~g) == 0;
+                emit_phase_group(g, final_group ? gphase : cd(1, 0));
+                if (final_group) gphase = cd(1, 0);
             }
-            case EOP_SWAP2: {
-                I.op = I_SWAP2;
-                const int s0 = slot_of(loc[e.q[0]]), s1 = slot_of(loc[e.q[1]]);
-                I.s[0] = (uint8_t)std::min(s0, s1);
-                I.s[1] = (uint8_t)std::max(s0, s1);
-                if (e.pure_swap) I.flags |= F_PURE;
-                set_controls(I, e.cmask);
-                P.pool.insert(P.pool.end(), e.m.begin(), e.m.end());
-                break;
+            if (gphase != cd(1, 0)) {
+                ph[H.T[cur.lay[0]]] = cd(1, 0);
+                emit_phase_group(1ull << H.T[cur.lay[0]], gphase);
+                gphase = cd(1, 0);
             }
-            case EOP_PARITY: {
-                // blocks in the kernel's canonical form (see I_PARITY in engine.h): for each class
-                // and element, the slot bits, then the row/column bits they carry, then the
-                // superoperator index (q[0] = MSB)
-                I.op = I_PARITY;
-                int slot[4];
-                for (int j = 0; j < e.k; j++) slot[j] = slot_of(loc[e.q[j]]);
-                const int D = 1 << e.k;
-                auto entry = [&](int row_idx, int col_idx) { return cd(e.m[2 * (row_idx * D + col_idx)], e.m[2 * (row_idx * D + col_idx) + 1]); };
-                // superoperator index of the element whose register-slot bits are `bits`
-                auto sup_index = [&](int bits) {
-                    int idx = 0;
-                    for (int j = 0; j < e.k; j++) idx = (idx << 1) | ((bits >> slot[j]) & 1);
-                    return idx;
-                };
-                std::vector<cd> blocks;
-                if (e.k == 2) {
-                    const int lo = std::min(slot[0], slot[1]), hi = std::max(slot[0], slot[1]);
-                    I.smask = (1 << lo) | (1 << hi);
-                    for (int p = 0; p < 2; p++)
-                        for (int i = 0; i < 2; i++)
-                            for (int j = 0; j < 2; j++) {
-                                const int bi = (i << lo) | ((i ^ p) << hi), bj = (j << lo) | ((j ^ p) << hi);
-                                blocks.push_back(entry(sup_index(bi), sup_index(bj)));
-                            }
-                } else {
-                    // pairs (row a, col a) and (row b, col b); s0 = 0 and its partner, then the other two
-                    const int pa_slots[2] = {slot[0], slot[2]}, pb_slots[2] = {slot[1], slot[3]};
-                    const bool a_has0 = pa_slots[0] == 0 || pa_slots[1] == 0;
-                    const int *P0 = a_has0 ? pa_slots : pb_slots, *P1 = a_has0 ? pb_slots : pa_slots;
-                    const int s0 = 0, s1 = P0[0] == 0 ? P0[1] : P0[0];
-                    const int s2 = std::min(P1[0], P1[1]), s3 = std::max(P1[0], P1[1]);
-                    I.s[0] = (uint8_t)s0;
-                    I.s[1] = (uint8_t)s1;
-                    I.s[2] = (uint8_t)s2;
-                    I.s[3] = (uint8_t)s3;
-                    I.smask = 15;
-                    for (int c = 0; c < 4; c++) {
-                        const int pa = c >> 1, pb = c & 1;
-                        auto bits_of = [&](int i) {
-                            const int ia = i >> 1, ib = i & 1;
-                            return (ia << s0) | ((ia ^ pa) << s1) | (ib << s2) | ((ib ^ pb) << s3);
-                        };
-                        for (int i = 0; i < 4; i++)
-                            for (int j = 0; j < 4; j++) blocks.push_back(entry(sup_index(bits_of(i)), sup_index(bits_of(j))));
-                    }
-                }
-                bool real = true;
-                for (const cd &x : blocks) {
-                    real = real && x.imag() == 0.0;
-                    P.pool.push_back(x.real());
-                    P.pool.push_back(x.imag());
-                }
-                if (real) I.flags |= F_REAL;
-                break;
-            }
-            case EOP_GRAD:
-            case EOP_DENSE: {
-                I.op = e.kind == EOP_GRAD ? I_GRAD : I_DENSE;
-                if (e.kind == EOP_GRAD) {
-                    I.pad = H.ngrad++;
-                    P.grad_slots.push_back(e.slot);
-                    P.num_grad++;
-                }
-                const int k = e.k, dim = 1 << k;
-                int slot[4], smask = 0;
-                for (int j = 0; j < k; j++) {
-                    slot[j] = slot_of(loc[e.q[j]]);
-                    smask |= 1 << slot[j];
-                }
-                // rank of each gate qubit among the sorted slots: kernel index bit rank[j]
-                int rank[4];
-                for (int j = 0; j < k; j++) {
-                    rank[j] = 0;
-                    for (int t = 0; t < k; t++)
-                        if (slot[t] < slot[j]) rank[j]++;
-                }
-                auto perm = [&](int g) {
-                    int r = 0;
-                    for (int j = 0; j < k; j++)
-                        if (g >> (k - 1 - j) & 1) r |= 1 << rank[j];
-                    return r;
-                };
-                std::vector<double> km(2 * dim * dim);
-                for (int gr = 0; gr < dim; gr++)
-                    for (int gc = 0; gc < dim; gc++) {
-                        const int kr = perm(gr), kc = perm(gc);
-                        km[2 * (kr * dim + kc)] = e.m[2 * (gr * dim + gc)];
-                        km[2 * (kr * dim + kc) + 1] = e.m[2 * (gr * dim + gc) + 1];
-                    }
-                I.smask = smask;
-                set_controls(I, e.cmask);
-                if (e.kind == EOP_DENSE && k > 2) {
-                    I.flags |= F_OUTER;
-                    H.flags |= SF_BIG;
-                }
-                P.pool.insert(P.pool.end(), km.begin(), km.end());
-                break;
-            }
-            }
-            P.instrs.push_back(I);
         }
         // leave the tile with lanes on the lowest local bits for coalesced stores
         {

@@ -182,12 +182,13 @@ __device__ __forceinline__ void swap_inplace(cx<R> &x, cx<R> &y) {
     swap_inplace(x.y, y.y);
 }
 
-// Dense op on the register slots of MASK (at most two).  CTRL: register groups outside `pat` are
+// Dense op on the register slots of MASK (at most two).  REAL: the matrix is real (rotations
+// split off single-qubit gates), 2 FMAs per entry instead of 4.  CTRL: register groups outside `pat` are
 // skipped; lanes whose thread-bit controls fail keep their amplitudes.  For one slot those lanes
 // get the identity matrix (selected once per op); for two slots the outputs are selected, which
 // keeps the 16-entry matrix in scalar registers (a per-lane copy would cost 64 VGPRs and lower
 // the occupancy of the whole kernel).
-template <int MASK, bool CTRL, bool PAIR, typename R>
+template <int MASK, bool CTRL, bool PAIR, typename R, bool REAL = false>
 __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ M, uint32_t pat,
                                          bool lok) {
     constexpr int K = popc4(MASK);
@@ -219,11 +220,14 @@ __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R
                     R re = 0, im = 0;
                     static_for<0, D>([&](auto jc) __attribute__((always_inline)) {
                         constexpr int j = decltype(jc)::value;
-                        const R mr = mm[2 * (i * D + j)], mi = mm[2 * (i * D + j) + 1];
+                        const R mr = mm[2 * (i * D + j)];
                         re = fma(mr, x[j].x, re);
-                        re = fma(-mi, x[j].y, re);
                         im = fma(mr, x[j].y, im);
-                        im = fma(mi, x[j].x, im);
+                        if constexpr (!REAL) {
+                            const R mi = mm[2 * (i * D + j) + 1];
+                            re = fma(-mi, x[j].y, re);
+                            im = fma(mi, x[j].x, im);
+                        }
                     });
                     if constexpr (SEL_OUTPUT) y[i] = cx<R>{lok ? re : x[i].x, lok ? im : x[i].y};
                     else y[i] = cx<R>{re, im};
@@ -635,15 +639,13 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 const uint32_t sw = *reinterpret_cast<const uint32_t *>(I.s);
                 if (op == I_DENSE) {
                     const bool ctrl = flags & (F_TCTRL | F_PAT);
-    #define DENSE_CASE(MK)                                                       \
-        case MK:                                                                 \
-            if (ctrl) op_dense<MK, true, PAIR>(a, b, M, pat, lok);               \
-            else op_dense<MK, false, PAIR>(a, b, M, pat, lok);                   \
-            break;
-    #define DENSE_BIG(MK)                                                        \
-        case MK:                                                                 \
-            op_dense_big<MK, PAIR>(a, b, M, pat, lok);                           \
-            break;
+                    const bool real = flags & F_REAL;
+#define DENSE_CASE(MK)                                                        \
+    case MK:                                                                  \
+        if (ctrl) op_dense<MK, true, PAIR>(a, b, M, pat, lok);                \
+        else if (real) op_dense<MK, false, PAIR, R, true>(a, b, M, pat, lok); \
+        else op_dense<MK, false, PAIR>(a, b, M, pat, lok);                    \
+        break;
                     switch (I.smask) {
                         DENSE_CASE(1)
                         DENSE_CASE(2)
@@ -658,10 +660,9 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     default:
                         break;
                     }
-    #undef DENSE_CASE
-    #undef DENSE_BIG
+#undef DENSE_CASE
                 } else if (op == I_DIAG) {
-    #define DIAG_CASE(RM)                                            \
+#define DIAG_CASE(RM)                                            \
         case RM:                                                     \
             op_diag<RM, PAIR>(a, b, I, pool, tid, base, lok);        \
             break;
@@ -685,10 +686,10 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     default:
                         break;
                     }
-    #undef DIAG_CASE
+#undef DIAG_CASE
                 } else if (op == I_ANTI1) {
                     const int variant = ((sw & 255u) << 2) | ((flags & F_PURE) ? 2 : 0) | ((flags & F_TCTRL) ? 1 : 0);
-    #define ANTI_CASE(S)                                                             \
+#define ANTI_CASE(S)                                                             \
         case S * 4 + 0: op_anti1<S, false, false, PAIR>(a, b, M, pat, lok); break;   \
         case S * 4 + 1: op_anti1<S, false, true, PAIR>(a, b, M, pat, lok); break;    \
         case S * 4 + 2: op_anti1<S, true, false, PAIR>(a, b, M, pat, lok); break;    \
@@ -701,11 +702,11 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     default:
                         break;
                     }
-    #undef ANTI_CASE
+#undef ANTI_CASE
                 } else if (op == I_SWAP2) {
                     const int s0 = sw & 255u, s1 = (sw >> 8) & 255u;
                     const int variant = ((s0 * 4 + s1) << 2) | ((flags & F_PURE) ? 2 : 0) | ((flags & F_TCTRL) ? 1 : 0);
-    #define SWAP_CASE(A, B)                                                                       \
+#define SWAP_CASE(A, B)                                                                       \
         case (A * 4 + B) * 4 + 0: op_swap2<A, B, false, false, PAIR>(a, b, M, pat, lok); break;   \
         case (A * 4 + B) * 4 + 1: op_swap2<A, B, false, true, PAIR>(a, b, M, pat, lok); break;    \
         case (A * 4 + B) * 4 + 2: op_swap2<A, B, true, false, PAIR>(a, b, M, pat, lok); break;    \
@@ -720,7 +721,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     default:
                         break;
                     }
-    #undef SWAP_CASE
+#undef SWAP_CASE
                 } else if (op == I_PARITY) {
                     if constexpr (!PAIR) {  // density-matrix programs only
                         const bool real = flags & F_REAL;
@@ -762,7 +763,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 } else if (op == I_GRAD) {
                     if constexpr (PAIR) {
                         double v = 0;
-    #define GRAD_CASE(MK)                              \
+#define GRAD_CASE(MK)                              \
         case MK:                                       \
             v = op_grad<MK>(a, b, M, pat, lok);        \
             break;
@@ -780,7 +781,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                         default:
                             break;
                         }
-    #undef GRAD_CASE
+#undef GRAD_CASE
                         // blocks of fewer than 64 threads only read active lanes
     #pragma unroll
                         for (int off = 32; off >= 1; off >>= 1)

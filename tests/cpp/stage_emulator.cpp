@@ -270,10 +270,12 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                                 if (!(I.pat >> b & 1)) continue;
                                 cd x[16], y[16];
                                 for (int j = 0; j < D; j++) x[j] = A[b | dep4(j, MASK)];
+                                // F_REAL (uncontrolled, at most two slots): the kernel reads real parts only
+                                const bool real = (I.flags & F_REAL) && Kq <= 2 && !(I.flags & (F_TCTRL | F_PAT));
                                 for (int i = 0; i < D; i++) {
                                     y[i] = 0;
                                     for (int j = 0; j < D; j++)
-                                        y[i] += cd(pool[2 * (i * D + j)], pool[2 * (i * D + j) + 1]) * x[j];
+                                        y[i] += cd(pool[2 * (i * D + j)], real ? 0.0 : pool[2 * (i * D + j) + 1]) * x[j];
                                 }
                                 for (int i = 0; i < D; i++) A[b | dep4(i, MASK)] = y[i];
                             }
