@@ -76,6 +76,8 @@ enum InstrOp : int32_t {
     I_REMAP = 5,  // new layout lay[0..m-1] (register slots 0..3, then thread bits), LDS offsets kr
     I_GRAD = 6,   // like I_DENSE, but accumulates Re <b| M |a> into local gradient slot `pad`
     I_PARITY = 7, // parity-block superoperator on 2 or 4 register slots (canonical form below)
+    I_SHEAR = 8,  // slot s[0]: k in-place shears, a0 += c0 a1, a1 += c1 a0, a0 += c2 a1 (the
+                  // first k of them), real coefficients c0, c1, c2 at pool[mat] (packed reals)
 };
 
 // Instr::flags

@@ -458,9 +458,9 @@ static std::vector<EOp> fuse_single_qubit(const std::vector<EOp> &in) {
 }
 
 // A unitary single-qubit matrix U = e^{ia} diag(1, e^{i phi}) R diag(1, e^{i lam}) with the real
-// rotation R = [[r, -s], [s, r]], r = |u00|, s = |u10|.  An uncontrolled one-qubit dense op
-// becomes DIAG, real DENSE, DIAG: the rotation needs half the FMAs of a complex matrix, and the
-// emission folds the diagonals into pending per-qubit phases applied in batches.
+// rotation R = [[r, -s], [s, r]], r = |u00|, s = |u10| (an angle in [0, pi/2]).  An uncontrolled
+// one-qubit dense op becomes DIAG, real DENSE, DIAG: the emission runs the rotation as in-place
+// shears and folds the diagonals into pending per-qubit phases applied in batches.
 static std::vector<EOp> split_rotations(const std::vector<EOp> &in) {
     std::vector<EOp> out;
     out.reserve(in.size() * 3);
@@ -470,13 +470,12 @@ static std::vector<EOp> split_rotations(const std::vector<EOp> &in) {
             continue;
         }
         const cd u00(e.m[0], e.m[1]), u01(e.m[2], e.m[3]), u10(e.m[4], e.m[5]), u11(e.m[6], e.m[7]);
-        const bool real = e.m[1] == 0.0 && e.m[3] == 0.0 && e.m[5] == 0.0 && e.m[7] == 0.0;
         const double r = std::abs(u00), s = std::abs(u10);
         // only unitary matrices with both a diagonal and an off-diagonal part
         const bool unitary = std::abs(std::norm(u00) + std::norm(u10) - 1.0) < 1e-12 &&
                              std::abs(std::norm(u01) + std::norm(u11) - 1.0) < 1e-12 &&
                              std::abs(std::conj(u00) * u01 + std::conj(u10) * u11) < 1e-12;
-        if (real || !unitary || r < 1e-150 || s < 1e-150) {
+        if (!unitary || r < 1e-150 || s < 1e-150) {
             out.push_back(e);
             continue;
         }
@@ -498,9 +497,10 @@ static std::vector<EOp> split_rotations(const std::vector<EOp> &in) {
         a.q[0] = e.q[0];
         push(a.m, cd(1, 0));
         push(a.m, eaphi / ea);
-        out.push_back(std::move(b));
+        // real orthogonal inputs (RY, H, ...) give diagonals of +-1; exact identities are dropped
+        if (b.m != std::vector<double>{1.0, 0.0, 1.0, 0.0}) out.push_back(std::move(b));
         out.push_back(std::move(rot));
-        out.push_back(std::move(a));
+        if (a.m != std::vector<double>{1.0, 0.0, 1.0, 0.0}) out.push_back(std::move(a));
         (void)u11;
     }
     return out;
@@ -590,6 +590,10 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
     std::vector<cd> ph(nq > 0 ? nq : 1, cd(1, 0));  // pending phase diag(1, ph[q]) per qubit
     uint64_t pending = 0;
     cd gphase(1, 0);
+    // Rotations may leave a real diagonal scale pending (two shears instead of three) unless the
+    // program takes inner products on the way (gradient ops need the state at its true norm).
+    bool defer_scale = true;
+    for (const EOp &e : eops) defer_scale = defer_scale && e.kind != EOP_GRAD;
     size_t stage_index = 0;
     for (const StageOps &S : stages) {
         StageHeader H;
@@ -964,11 +968,16 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             }
             return group;
         };
+        // the global factor joins a phase group once its magnitude leaves [2^-30, 2^30], so the
+        // stored amplitudes stay within a bounded factor of their true values
         auto flush = [&](uint64_t must) {
             must &= pending;
             while (must) {
                 const uint64_t g = next_group(must);
-                emit_phase_group(g, cd(1, 0));
+                const double mag = std::abs(gphase);
+                const bool fold = !(mag >= 0x1p-30 && mag <= 0x1p30);
+                emit_phase_group(g, fold ? gphase : cd(1, 0));
+                if (fold) gphase = cd(1, 0);
                 must &= ~g;
             }
         };
@@ -979,9 +988,53 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         };
         auto absorb = [&](const EOp &e) {
             const cd d0(e.m[0], e.m[1]), d1(e.m[2], e.m[3]);
+            const int q = e.q[0];
             gphase *= d0;
-            ph[e.q[0]] *= d1 / d0;
-            pending |= 1ull << e.q[0];
+            ph[q] *= d1 / d0;
+            if (ph[q] == cd(1, 0)) pending &= ~(1ull << q);
+            else pending |= 1ull << q;
+        };
+        // uncontrolled real rotation [[r, -s], [s, r]] (r^2 + s^2 = 1)
+        auto is_rotation = [&](const EOp &e) {
+            if (!(e.kind == EOP_DENSE && e.k == 1 && e.cmask == 0)) return false;
+            for (int t = 1; t < 8; t += 2)
+                if (e.m[t] != 0.0) return false;
+            return e.m[0] == e.m[6] && e.m[2] == -e.m[4] && std::abs(e.m[0] * e.m[0] + e.m[4] * e.m[4] - 1.0) < 1e-12;
+        };
+        // A rotation as in-place shears on its register slot.  With r >= 1/4 and a deferred
+        // scale: R = diag(r, 1/r) L U with U = [[1, -s/r], [0, 1]], L = [[1, 0], [s r, 1]], and
+        // diag(r, 1/r) = r diag(1, 1/r^2) joins the pending phases (errors grow at most by 1/r).
+        // Otherwise the three-shear form with t = tan(angle/2) = s / (1 + r) <= 1:
+        // R = [[1, -t], [0, 1]] [[1, 0], [s, 1]] [[1, -t], [0, 1]].  Pending phases on the qubit
+        // have been applied before.
+        auto emit_shear = [&](const EOp &e) {
+            double r = e.m[0], s = e.m[4];
+            if (r < 0) {  // R = -R' with R' an angle in (-pi/2, pi/2]
+                r = -r;
+                s = -s;
+                gphase = -gphase;
+            }
+            Instr I;
+            std::memset(&I, 0, sizeof(I));
+            I.op = I_SHEAR;
+            I.mat = (int32_t)(P.pool.size() / 2);
+            I.pat = 0xFFFFu;
+            I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
+            I.smask = 1 << I.s[0];
+            if (defer_scale && r >= 0.25) {
+                I.k = 2;
+                P.pool.insert(P.pool.end(), {-s / r, s * r, 0.0, 0.0});
+                const int q = e.q[0];
+                gphase *= r;
+                ph[q] /= r * r;
+                if (ph[q] == cd(1, 0)) pending &= ~(1ull << q);
+                else pending |= 1ull << q;
+            } else {
+                const double t = s / (1.0 + r);
+                I.k = 3;
+                P.pool.insert(P.pool.end(), {-t, s, -t, 0.0});
+            }
+            P.instrs.push_back(I);
         };
         size_t left = nops;
         size_t lo_open = 0;  // every op before it is finished
@@ -1020,6 +1073,11 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 absorb(e);
                 continue;
             }
+            if (is_rotation(e)) {
+                flush(e.nondiag());
+                emit_shear(e);
+                continue;
+            }
             // Two uncontrolled one-qubit dense ops on different register slots run as one
             // two-slot op with the Kronecker product: the same 16 FMAs per amplitude, half the
             // register write-backs and one dispatch.
@@ -1028,7 +1086,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 for (size_t t = i + 1; t < nops; t++) {
                     if (fin[t]) continue;
                     const EOp &f = eops[S.ops[t]];
-                    if (f.kind != EOP_DENSE || f.k != 1 || f.cmask != 0 || f.q[0] == e.q[0]) continue;
+                    if (f.kind != EOP_DENSE || f.k != 1 || f.cmask != 0 || f.q[0] == e.q[0] || is_rotation(f)) continue;
                     if ((need[t] & ~regset) == 0 && ready(t)) {
                         j = t;
                         break;
@@ -1063,7 +1121,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 }
             }
             // pending phases pass uncontrolled moves: a swap exchanges the two qubits' phases,
-            // [a0, a1] <- [p0 a1, p1 a0] turns diag(1, f) into f diag(1, conj f)
+            // [a0, a1] <- [p0 a1, p1 a0] turns diag(1, f) into f diag(1, 1/f)
             if (e.kind == EOP_SWAP2 && e.cmask == 0) {
                 const int qa = e.q[0], qb = e.q[1];
                 std::swap(ph[qa], ph[qb]);
@@ -1071,7 +1129,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 pending = (pending & ~((1ull << qa) | (1ull << qb))) | (bb << qa) | (ba << qb);
             } else if (e.kind == EOP_ANTI1 && e.cmask == 0 && (pending >> e.q[0] & 1)) {
                 gphase *= ph[e.q[0]];
-                ph[e.q[0]] = std::conj(ph[e.q[0]]);
+                ph[e.q[0]] = 1.0 / ph[e.q[0]];
             } else {
                 flush(e.nondiag());
             }

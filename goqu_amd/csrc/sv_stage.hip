@@ -171,6 +171,13 @@ __device__ __forceinline__ void cmul_inplace(float &re, float &im, float pr, flo
             : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
             : "v"(pr), "v"(pi));
 }
+// acc <- acc + x * c in place, c uniform (SGPRs)
+__device__ __forceinline__ void fma_inplace(double &acc, double x, double c) {
+    asm volatile("v_fma_f64 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(c));
+}
+__device__ __forceinline__ void fma_inplace(float &acc, float x, float c) {
+    asm volatile("v_fma_f32 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(c));
+}
 __device__ __forceinline__ void swap_inplace(double &x, double &y) {
     double t;
     asm volatile("v_mov_b64 %2, %0\n\tv_mov_b64 %0, %1\n\tv_mov_b64 %1, %2" : "+v"(x), "+v"(y), "=&v"(t));
@@ -240,6 +247,34 @@ __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R
             if constexpr (PAIR) apply(b);
         }
     });
+}
+
+// A real rotation on slot S as NSH in-place shears (lifting steps): a0 += c0 a1, a1 += c1 a0,
+// then (NSH = 3) a0 += c2 a1.  One FMA per amplitude component and shear, no temporaries and no
+// write-backs; the planner picks coefficients whose products stay bounded.
+template <int S, int NSH, bool PAIR, typename R>
+__device__ __forceinline__ void op_shear(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ M) {
+    const R c[3] = {M[0], M[1], M[2]};
+    auto apply = [&](cx<R>(&v)[16]) __attribute__((always_inline)) {
+        static_for<0, NSH>([&](auto pc) __attribute__((always_inline)) {
+            constexpr int p = decltype(pc)::value;
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value;
+                if constexpr (!(r >> S & 1)) {
+                    constexpr int r1 = r | (1 << S);
+                    if constexpr (p & 1) {
+                        fma_inplace(v[r1].x, v[r].x, c[p]);
+                        fma_inplace(v[r1].y, v[r].y, c[p]);
+                    } else {
+                        fma_inplace(v[r].x, v[r1].x, c[p]);
+                        fma_inplace(v[r].y, v[r1].y, c[p]);
+                    }
+                }
+            });
+        });
+    };
+    apply(a);
+    if constexpr (PAIR) apply(b);
 }
 
 // Dense op on three or four register slots: computed for every lane, kept where the
@@ -661,8 +696,20 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                         break;
                     }
 #undef DENSE_CASE
+                } else if (op == I_SHEAR) {
+                    switch (((sw & 255u) << 1) | (I.k == 3 ? 1 : 0)) {
+                    case 0: op_shear<0, 2, PAIR>(a, b, M); break;
+                    case 1: op_shear<0, 3, PAIR>(a, b, M); break;
+                    case 2: op_shear<1, 2, PAIR>(a, b, M); break;
+                    case 3: op_shear<1, 3, PAIR>(a, b, M); break;
+                    case 4: op_shear<2, 2, PAIR>(a, b, M); break;
+                    case 5: op_shear<2, 3, PAIR>(a, b, M); break;
+                    case 6: op_shear<3, 2, PAIR>(a, b, M); break;
+                    case 7: op_shear<3, 3, PAIR>(a, b, M); break;
+                    default: break;
+                    }
                 } else if (op == I_DIAG) {
-#define DIAG_CASE(RM)                                            \
+#define DIAG_CASE(RM)                                          \
         case RM:                                                     \
             op_diag<RM, PAIR>(a, b, I, pool, tid, base, lok);        \
             break;

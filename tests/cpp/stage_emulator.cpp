@@ -70,7 +70,7 @@ enum Cov {
     C_DENSE, C_DENSE_TCTRL, C_DENSE_PAT, C_DENSE_TILE, C_DENSE_BIG, C_DIAG_UNIFORM, C_DIAG_KL1, C_DIAG_KL2,
     C_DIAG_TCTRL, C_DIAG_TILE, C_DIAG_REGS, C_ANTI_PURE, C_ANTI_PHASED, C_ANTI_TCTRL, C_ANTI_PAT, C_SWAP_PURE,
     C_SWAP_PHASED, C_SWAP_TCTRL, C_SWAP_PAT, C_REMAP, C_GRAD, C_GRAD_CTRL, C_RESET, C_PARITY2, C_PARITY4,
-    C_PARITY_REAL, C_NCOV
+    C_PARITY_REAL, C_SHEAR2, C_SHEAR3, C_NCOV
 };
 int64_t g_cov[C_NCOV];
 void count(const Instr &I) {
@@ -101,6 +101,7 @@ void count(const Instr &I) {
     case I_REMAP: g_cov[C_REMAP]++; break;
     case I_GRAD: g_cov[(tc || pt || tl) ? C_GRAD_CTRL : C_GRAD]++; break;
     case I_RESET: g_cov[C_RESET]++; break;
+    case I_SHEAR: g_cov[I.k == 2 ? C_SHEAR2 : C_SHEAR3]++; break;
     case I_PARITY:
         g_cov[I.k == 2 ? C_PARITY2 : C_PARITY4]++;
         if (I.flags & F_REAL) g_cov[C_PARITY_REAL]++;
@@ -361,6 +362,22 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                                 if ((I.flags & F_TCTRL) && !lok) ph = 1;
                                 A[r] *= ph;
                             }
+                            break;
+                        }
+                        case I_SHEAR: {
+                            const int S = I.s[0];
+                            if (S > 3 || I.smask != (1 << S) || I.k < 2 || I.k > 3 || I.flags != 0) {
+                                err = "malformed shear";
+                                return 1;
+                            }
+                            // in-place lifting steps, each rounded as one fused multiply-add
+                            for (int p = 0; p < I.k; p++)
+                                for (int r = 0; r < 16; r++) {
+                                    if (r >> S & 1) continue;
+                                    cd &dst = (p & 1) ? A[r | (1 << S)] : A[r];
+                                    const cd src = (p & 1) ? A[r] : A[r | (1 << S)];
+                                    dst = cd(std::fma(pool[p], src.real(), dst.real()), std::fma(pool[p], src.imag(), dst.imag()));
+                                }
                             break;
                         }
                         case I_ANTI1: {

@@ -69,7 +69,7 @@ def sv_reverse(psi: np.ndarray, lam: np.ndarray, nq_user: int, buf: "N.OpBuffer"
 COVERAGE = ["dense", "dense_tctrl", "dense_pat", "dense_tile", "dense_big", "diag_uniform", "diag_kl1",
             "diag_kl2", "diag_tctrl", "diag_tile", "diag_regs", "anti_pure", "anti_phased", "anti_tctrl",
             "anti_pat", "swap_pure", "swap_phased", "swap_tctrl", "swap_pat", "remap", "grad", "grad_ctrl",
-            "reset", "parity2", "parity4", "parity_real"]
+            "reset", "parity2", "parity4", "parity_real", "shear2", "shear3"]
 
 
 def coverage(reset: bool = True) -> dict:
