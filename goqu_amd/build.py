@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 from pathlib import Path
@@ -36,14 +37,37 @@ def _sources() -> list[Path]:
     return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
 
 
+# kernels whose speed depends on four waves per SIMD (at most 128 VGPRs); the build reports a
+# drop in their occupancy
+OCCUPANCY_WATCH = {"_ZN4goqu7k_stageIdE": 4, "_ZN4goqu7k_stageIfE": 4}
+
+
+def _check_occupancy(src: Path, remarks: str) -> None:
+    current = None
+    for line in remarks.splitlines():
+        m = re.search(r"remark: Function Name: (\S+)", line)
+        if m:
+            current = m.group(1)
+            continue
+        m = re.search(r"remark:\s+Occupancy \[waves/SIMD\]: (\d+)", line)
+        if m and current:
+            for prefix, want in OCCUPANCY_WATCH.items():
+                if current.startswith(prefix) and int(m.group(1)) < want:
+                    print(f"warning: {src.name}: {current} runs {m.group(1)} waves/SIMD (expected {want})",
+                          file=sys.stderr)
+
+
 def _compile(src: Path) -> Path:
     obj = OBJ / (src.name + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if src.name == "sv_stage.hip" else []
+    cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stdout}\n{r.stderr}")
+    if extra:
+        _check_occupancy(src, r.stderr)
     return obj
 
 

@@ -7,6 +7,7 @@
 // threads through LDS when an op needs other qubits in registers), and stores it back.
 #pragma once
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -77,7 +78,8 @@ enum InstrOp : int32_t {
     I_GRAD = 6,   // like I_DENSE, but accumulates Re <b| M |a> into local gradient slot `pad`
     I_PARITY = 7, // parity-block superoperator on 2 or 4 register slots (canonical form below)
     I_SHEAR = 8,  // slot s[0]: k in-place shears, a0 += c0 a1, a1 += c1 a0, a0 += c2 a1 (the
-                  // first k of them), real coefficients c0, c1, c2 at pool[mat] (packed reals)
+                  // first k of them); c0, c1, c2 are doubles stored in the instruction at byte
+                  // offsets 40, 48, 56 (set_shear_coefs), so the op needs no pool load
 };
 
 // Instr::flags
@@ -120,6 +122,18 @@ struct alignas(16) Instr {
     uint16_t kr[16];  // REMAP: LDS byte offset (8-byte elements) of register index r in the new layout
 };
 static_assert(sizeof(Instr) == 96, "Instr must stay 96 bytes");
+
+inline void set_shear_coefs(Instr &I, double c0, double c1, double c2) {
+    unsigned char *b = reinterpret_cast<unsigned char *>(&I);
+    std::memcpy(b + 40, &c0, 8);
+    std::memcpy(b + 48, &c1, 8);
+    std::memcpy(b + 56, &c2, 8);
+}
+inline double shear_coef(const Instr &I, int i) {
+    double c;
+    std::memcpy(&c, reinterpret_cast<const unsigned char *>(&I) + 40 + 8 * i, 8);
+    return c;
+}
 
 // DIAG phase table: entry (lane_index << popcount(smask)) | v, where v holds the values of the
 // smask register slots (ascending slot order, slot bit -> v bit) and lane_index the ls[] bits.

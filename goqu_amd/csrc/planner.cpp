@@ -1017,13 +1017,12 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             Instr I;
             std::memset(&I, 0, sizeof(I));
             I.op = I_SHEAR;
-            I.mat = (int32_t)(P.pool.size() / 2);
             I.pat = 0xFFFFu;
             I.s[0] = (uint8_t)slot_of(loc[e.q[0]]);
             I.smask = 1 << I.s[0];
             if (defer_scale && r >= 0.25) {
                 I.k = 2;
-                P.pool.insert(P.pool.end(), {-s / r, s * r, 0.0, 0.0});
+                set_shear_coefs(I, -s / r, s * r, 0.0);
                 const int q = e.q[0];
                 gphase *= r;
                 ph[q] /= r * r;
@@ -1032,7 +1031,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             } else {
                 const double t = s / (1.0 + r);
                 I.k = 3;
-                P.pool.insert(P.pool.end(), {-t, s, -t, 0.0});
+                set_shear_coefs(I, -t, s, -t);
             }
             P.instrs.push_back(I);
         };

@@ -105,6 +105,46 @@ __device__ __forceinline__ uint32_t koff(const KOff &K) {
     return (K.w[r >> 1] >> (16 * (r & 1))) & 0xFFFFu;
 }
 
+// The first 64 bytes of an Instr (everything the inner op loop reads), fetched as one batch of
+// scalar loads.  I_SHEAR keeps its three coefficients (doubles) at bytes 40..63.
+struct IHead {
+    int32_t op, k, mat, flags;
+    uint32_t pat, tmask;
+    uint64_t omask;
+    uint32_t sw;     // s[0..3]
+    int32_t smask;
+    uint32_t w10;    // pad (I_GRAD slot) / low word of shear coefficient 0
+    uint32_t lsw;    // ls[0..3] / high word of shear coefficient 0
+    uint32_t w12, w13, w14, w15;
+};
+static_assert(offsetof(Instr, lay) == 48 && offsetof(Instr, ls) == 44, "IHead mirrors Instr");
+__device__ __forceinline__ IHead load_head(const Instr *__restrict__ P, int pc) {
+    typedef uint32_t u16v __attribute__((ext_vector_type(16)));
+    const u16v w = *reinterpret_cast<const u16v *>(P + pc);
+    IHead h;
+    h.op = (int32_t)w[0];
+    h.k = (int32_t)w[1];
+    h.mat = (int32_t)w[2];
+    h.flags = (int32_t)w[3];
+    h.pat = w[4];
+    h.tmask = w[5];
+    h.omask = (uint64_t)w[6] | ((uint64_t)w[7] << 32);
+    h.sw = w[8];
+    h.smask = (int32_t)w[9];
+    h.w10 = w[10];
+    h.lsw = w[11];
+    h.w12 = w[12];
+    h.w13 = w[13];
+    h.w14 = w[14];
+    h.w15 = w[15];
+    return h;
+}
+__device__ __forceinline__ double shear_coef(const IHead &h, int i) {
+    const uint32_t lo = i == 0 ? h.w10 : i == 1 ? h.w12 : h.w14;
+    const uint32_t hi = i == 0 ? h.lsw : i == 1 ? h.w13 : h.w15;
+    return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
+}
+
 // compile-time loop helper
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F &&f) {
@@ -123,66 +163,49 @@ __device__ __forceinline__ cx<R> cmulp(cx<R> a, R br, R bi) {
 // it arrived in: results are written back through tied ("+v") operands.  Without this the
 // compiler keeps a second copy of the 16 amplitudes and moves all of them at the loop's
 // control-flow joins, which doubles the VALU work of a gate.
-__device__ __forceinline__ void commit(double &dst, double val) { asm volatile("v_mov_b64 %0, %1" : "+v"(dst) : "v"(val)); }
-__device__ __forceinline__ void commit(float &dst, float val) { asm volatile("v_mov_b32 %0, %1" : "+v"(dst) : "v"(val)); }
+__device__ __forceinline__ void commit(double &dst, double val) { asm("v_mov_b64 %0, %1" : "+v"(dst) : "v"(val)); }
+__device__ __forceinline__ void commit(float &dst, float val) { asm("v_mov_b32 %0, %1" : "+v"(dst) : "v"(val)); }
 template <typename R>
 __device__ __forceinline__ void commit(cx<R> &dst, cx<R> val) {
     commit(dst.x, val.x);
     commit(dst.y, val.y);
 }
-// a <- a * (pr + i pi) in place; P in SGPRs (uniform phase) or VGPRs (per-lane phase)
-template <bool UNIFORM>
-__device__ __forceinline__ void cmul_inplace(double &re, double &im, double pr, double pi) {
-    double t, u;
-    if constexpr (UNIFORM)
-        asm volatile(
-            "v_mul_f64 %2, %1, %5\n\t"
-            "v_mul_f64 %3, %1, %4\n\t"
-            "v_fma_f64 %1, %0, %5, %3\n\t"
-            "v_fma_f64 %0, %0, %4, -%2"
-            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
-            : "s"(pr), "s"(pi));
-    else
-        asm volatile(
-            "v_mul_f64 %2, %1, %5\n\t"
-            "v_mul_f64 %3, %1, %4\n\t"
-            "v_fma_f64 %1, %0, %5, %3\n\t"
-            "v_fma_f64 %0, %0, %4, -%2"
-            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
-            : "v"(pr), "v"(pi));
-}
-template <bool UNIFORM>
-__device__ __forceinline__ void cmul_inplace(float &re, float &im, float pr, float pi) {
-    float t, u;
-    if constexpr (UNIFORM)
-        asm volatile(
-            "v_mul_f32 %2, %1, %5\n\t"
-            "v_mul_f32 %3, %1, %4\n\t"
-            "v_fma_f32 %1, %0, %5, %3\n\t"
-            "v_fma_f32 %0, %0, %4, -%2"
-            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
-            : "s"(pr), "s"(pi));
-    else
-        asm volatile(
-            "v_mul_f32 %2, %1, %5\n\t"
-            "v_mul_f32 %3, %1, %4\n\t"
-            "v_fma_f32 %1, %0, %5, %3\n\t"
-            "v_fma_f32 %0, %0, %4, -%2"
-            : "+v"(re), "+v"(im), "=&v"(t), "=&v"(u)
-            : "v"(pr), "v"(pi));
+// a <- a * (pr + i pi) in place; P in SGPRs (uniform phase) or VGPRs (per-lane phase).  The two
+// products of the old imaginary part are plain code (the compiler schedules them early); the two
+// FMAs that overwrite the amplitude are single tied instructions.
+template <bool UNIFORM, typename R>
+__device__ __forceinline__ void cmul_inplace(R &re, R &im, R pr, R pi) {
+    const R t = im * pi, u = im * pr;
+    if constexpr (sizeof(R) == 8) {
+        if constexpr (UNIFORM) {
+            asm("v_fma_f64 %0, %1, %2, %3" : "+v"(im) : "v"(re), "s"(pi), "v"(u));
+            asm("v_fma_f64 %0, %0, %1, -%2" : "+v"(re) : "s"(pr), "v"(t));
+        } else {
+            asm("v_fma_f64 %0, %1, %2, %3" : "+v"(im) : "v"(re), "v"(pi), "v"(u));
+            asm("v_fma_f64 %0, %0, %1, -%2" : "+v"(re) : "v"(pr), "v"(t));
+        }
+    } else {
+        if constexpr (UNIFORM) {
+            asm("v_fma_f32 %0, %1, %2, %3" : "+v"(im) : "v"(re), "s"(pi), "v"(u));
+            asm("v_fma_f32 %0, %0, %1, -%2" : "+v"(re) : "s"(pr), "v"(t));
+        } else {
+            asm("v_fma_f32 %0, %1, %2, %3" : "+v"(im) : "v"(re), "v"(pi), "v"(u));
+            asm("v_fma_f32 %0, %0, %1, -%2" : "+v"(re) : "v"(pr), "v"(t));
+        }
+    }
 }
 // acc <- acc + x * c in place, c uniform (SGPRs)
 __device__ __forceinline__ void fma_inplace(double &acc, double x, double c) {
-    asm volatile("v_fma_f64 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(c));
+    asm("v_fma_f64 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(c));
 }
 __device__ __forceinline__ void fma_inplace(float &acc, float x, float c) {
-    asm volatile("v_fma_f32 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(c));
+    asm("v_fma_f32 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(c));
 }
 __device__ __forceinline__ void swap_inplace(double &x, double &y) {
     double t;
-    asm volatile("v_mov_b64 %2, %0\n\tv_mov_b64 %0, %1\n\tv_mov_b64 %1, %2" : "+v"(x), "+v"(y), "=&v"(t));
+    asm("v_mov_b64 %2, %0\n\tv_mov_b64 %0, %1\n\tv_mov_b64 %1, %2" : "+v"(x), "+v"(y), "=&v"(t));
 }
-__device__ __forceinline__ void swap_inplace(float &x, float &y) { asm volatile("v_swap_b32 %0, %1" : "+v"(x), "+v"(y)); }
+__device__ __forceinline__ void swap_inplace(float &x, float &y) { asm("v_swap_b32 %0, %1" : "+v"(x), "+v"(y)); }
 template <typename R>
 __device__ __forceinline__ void swap_inplace(cx<R> &x, cx<R> &y) {
     swap_inplace(x.x, y.x);
@@ -253,8 +276,8 @@ __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R
 // then (NSH = 3) a0 += c2 a1.  One FMA per amplitude component and shear, no temporaries and no
 // write-backs; the planner picks coefficients whose products stay bounded.
 template <int S, int NSH, bool PAIR, typename R>
-__device__ __forceinline__ void op_shear(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ M) {
-    const R c[3] = {M[0], M[1], M[2]};
+__device__ __forceinline__ void op_shear(cx<R> (&a)[16], cx<R> (&b)[16], const IHead &h) {
+    const R c[3] = {R(shear_coef(h, 0)), R(shear_coef(h, 1)), R(shear_coef(h, 2))};
     auto apply = [&](cx<R>(&v)[16]) __attribute__((always_inline)) {
         static_for<0, NSH>([&](auto pc) __attribute__((always_inline)) {
             constexpr int p = decltype(pc)::value;
@@ -417,11 +440,33 @@ __device__ __forceinline__ void op_reset(cx<R> (&a)[16]) {
 
 // Diagonal op: phase of register index r = table[(lane_index << popc(RM)) | pext(r, RM)], one
 // for lanes whose thread-bit controls fail.  Table columns without an entry != 1 are skipped.
+// N consecutive complex table entries at a uniform address, fetched as one scalar load
+template <typename R, int N>
+__device__ __forceinline__ void load_cx_batch(const cx<R> *__restrict__ p, cx<R> (&out)[N]) {
+    constexpr int W = N * (int)sizeof(cx<R>) / 4;
+    typedef uint32_t vec __attribute__((ext_vector_type(W)));
+    const vec w = *reinterpret_cast<const vec *>(p);
+    static_for<0, N>([&](auto jc) __attribute__((always_inline)) {
+        constexpr int j = decltype(jc)::value;
+        if constexpr (sizeof(R) == 8) {
+            out[j].x = __builtin_bit_cast(double, (uint64_t)w[4 * j] | ((uint64_t)w[4 * j + 1] << 32));
+            out[j].y = __builtin_bit_cast(double, (uint64_t)w[4 * j + 2] | ((uint64_t)w[4 * j + 3] << 32));
+        } else {
+            // copy the elements to scalars first: a bit_cast applied to the vector element
+            // expressions was compiled to read element 2j for both parts
+            const uint32_t xr = w[2 * j], xi = w[2 * j + 1];
+            out[j].x = __builtin_bit_cast(float, xr);
+            out[j].y = __builtin_bit_cast(float, xi);
+        }
+    });
+}
+
 template <int RM, bool PAIR, typename R>
-__device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const Instr &I, const R *__restrict__ pool,
+__device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const IHead &I, const R *__restrict__ pool,
                                         uint32_t tid, uint64_t base, bool lok) {
     constexpr int KR = popc4(RM);
     constexpr int NV = 1 << KR;
+    constexpr int CH = NV < 4 ? NV : 4;  // table columns per scalar load
     const cx<R> *__restrict__ tab = reinterpret_cast<const cx<R> *>(pool) + I.mat;
     const uint32_t nontriv = I.pat;
     const int kl = I.k;
@@ -439,14 +484,20 @@ __device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const In
         });
     };
     if (kl == 0 && !tc) {  // every lane uses the same phases
-        static_for<0, NV>([&](auto vc) __attribute__((always_inline)) {
-            if ((nontriv >> decltype(vc)::value) & 1u)
-                column(vc, std::integral_constant<bool, true>{}, tab[decltype(vc)::value]);
+        static_for<0, NV / CH>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int c0 = CH * decltype(cc)::value;
+            cx<R> ph[CH];
+            load_cx_batch<R, CH>(tab + c0, ph);
+            static_for<0, CH>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int j = decltype(jc)::value;
+                if ((nontriv >> (c0 + j)) & 1u)
+                    column(std::integral_constant<int, c0 + j>{}, std::integral_constant<bool, true>{}, ph[j]);
+            });
         });
         return;
     }
     // lane index from thread-id bits and (uniform) bits of the tile base
-    const uint32_t lsw = *reinterpret_cast<const uint32_t *>(I.ls);
+    const uint32_t lsw = I.lsw;
     uint32_t li = 0;
     for (int j = 0; j < kl; j++) {
         const uint32_t src = (lsw >> (8 * j)) & 255u;
@@ -663,15 +714,15 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             // (instruction fields then come through scalar loads).
             for (; pc < ninstr; pc++) {
                 pc = __builtin_amdgcn_readfirstlane(pc);
-                const Instr &I = P[pc];
-                const int op = I.op;
-                const int flags = I.flags;
-                if (flags & F_OUTER) break;
-                if ((flags & F_TILE) && (base & I.omask) != I.omask) continue;  // uniform per workgroup
-                const bool lok = (tid & I.tmask) == I.tmask;
-                const uint32_t pat = I.pat;
-                const R *__restrict__ M = pool + 2 * I.mat;
-                const uint32_t sw = *reinterpret_cast<const uint32_t *>(I.s);
+                const IHead cur = load_head(P, pc);  // one batch of scalar loads
+                if (cur.flags & F_OUTER) break;
+                const int op = cur.op;
+                const int flags = cur.flags;
+                if ((flags & F_TILE) && (base & cur.omask) != cur.omask) continue;  // uniform per workgroup
+                const bool lok = (tid & cur.tmask) == cur.tmask;
+                const uint32_t pat = cur.pat;
+                const R *__restrict__ M = pool + 2 * cur.mat;
+                const uint32_t sw = cur.sw;
                 if (op == I_DENSE) {
                     const bool ctrl = flags & (F_TCTRL | F_PAT);
                     const bool real = flags & F_REAL;
@@ -681,7 +732,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
         else if (real) op_dense<MK, false, PAIR, R, true>(a, b, M, pat, lok); \
         else op_dense<MK, false, PAIR>(a, b, M, pat, lok);                    \
         break;
-                    switch (I.smask) {
+                    switch (cur.smask) {
                         DENSE_CASE(1)
                         DENSE_CASE(2)
                         DENSE_CASE(4)
@@ -697,23 +748,23 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     }
 #undef DENSE_CASE
                 } else if (op == I_SHEAR) {
-                    switch (((sw & 255u) << 1) | (I.k == 3 ? 1 : 0)) {
-                    case 0: op_shear<0, 2, PAIR>(a, b, M); break;
-                    case 1: op_shear<0, 3, PAIR>(a, b, M); break;
-                    case 2: op_shear<1, 2, PAIR>(a, b, M); break;
-                    case 3: op_shear<1, 3, PAIR>(a, b, M); break;
-                    case 4: op_shear<2, 2, PAIR>(a, b, M); break;
-                    case 5: op_shear<2, 3, PAIR>(a, b, M); break;
-                    case 6: op_shear<3, 2, PAIR>(a, b, M); break;
-                    case 7: op_shear<3, 3, PAIR>(a, b, M); break;
+                    switch (((sw & 255u) << 1) | (cur.k == 3 ? 1 : 0)) {
+                    case 0: op_shear<0, 2, PAIR>(a, b, cur); break;
+                    case 1: op_shear<0, 3, PAIR>(a, b, cur); break;
+                    case 2: op_shear<1, 2, PAIR>(a, b, cur); break;
+                    case 3: op_shear<1, 3, PAIR>(a, b, cur); break;
+                    case 4: op_shear<2, 2, PAIR>(a, b, cur); break;
+                    case 5: op_shear<2, 3, PAIR>(a, b, cur); break;
+                    case 6: op_shear<3, 2, PAIR>(a, b, cur); break;
+                    case 7: op_shear<3, 3, PAIR>(a, b, cur); break;
                     default: break;
                     }
                 } else if (op == I_DIAG) {
 #define DIAG_CASE(RM)                                          \
         case RM:                                                     \
-            op_diag<RM, PAIR>(a, b, I, pool, tid, base, lok);        \
+            op_diag<RM, PAIR>(a, b, cur, pool, tid, base, lok);        \
             break;
-                    switch (I.smask) {
+                    switch (cur.smask) {
                         DIAG_CASE(0)
                         DIAG_CASE(1)
                         DIAG_CASE(2)
@@ -772,13 +823,13 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 } else if (op == I_PARITY) {
                     if constexpr (!PAIR) {  // density-matrix programs only
                         const bool real = flags & F_REAL;
-                        if (I.k == 2) {
+                        if (cur.k == 2) {
 #define PAR2_CASE(LO, HI)                                                   \
     case (1 << LO) | (1 << HI):                                             \
         if (real) op_parity2<LO, HI, true>(a, M);                           \
         else op_parity2<LO, HI, false>(a, M);                               \
         break;
-                            switch (I.smask) {
+                            switch (cur.smask) {
                                 PAR2_CASE(0, 1)
                                 PAR2_CASE(0, 2)
                                 PAR2_CASE(0, 3)
@@ -814,7 +865,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
         case MK:                                       \
             v = op_grad<MK>(a, b, M, pat, lok);        \
             break;
-                        switch (I.smask) {
+                        switch (cur.smask) {
                             GRAD_CASE(1)
                             GRAD_CASE(2)
                             GRAD_CASE(3)
@@ -833,7 +884,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
     #pragma unroll
                         for (int off = 32; off >= 1; off >>= 1)
                             if (off < (int)blockDim.x) v += __shfl_xor(v, off, 64);
-                        if (lane == 0) gacc[I.pad * 8 + wave] += v;  // one writer per (slot, wave)
+                        if (lane == 0) gacc[cur.w10 * 8 + wave] += v;  // one writer per (slot, wave)
                     }
                 }
             }
@@ -961,6 +1012,8 @@ int upload_program(const Program &P0, int prec, hipStream_t s, DeviceProgram &dp
     dp.hdr_off = 0;
     dp.instr_off = (nh * sizeof(StageHeader) + 255) & ~size_t(255);
     dp.pool_off = (dp.instr_off + ni * sizeof(Instr) + 255) & ~size_t(255);
+    // 64 spare bytes: the kernel's instruction-head fetch (64 B) stays inside the blob even for
+    // a stage without instructions at the end of the array
     const size_t bytes = dp.pool_off + np * rsz + 64;
     std::vector<unsigned char> host(bytes, 0);
     std::memcpy(host.data(), P.stages.data(), nh * sizeof(StageHeader));

@@ -376,7 +376,8 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                                     if (r >> S & 1) continue;
                                     cd &dst = (p & 1) ? A[r | (1 << S)] : A[r];
                                     const cd src = (p & 1) ? A[r] : A[r | (1 << S)];
-                                    dst = cd(std::fma(pool[p], src.real(), dst.real()), std::fma(pool[p], src.imag(), dst.imag()));
+                                    const double c = shear_coef(I, p);
+                                    dst = cd(std::fma(c, src.real(), dst.real()), std::fma(c, src.imag(), dst.imag()));
                                 }
                             break;
                         }
