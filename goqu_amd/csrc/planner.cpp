@@ -521,9 +521,9 @@ struct Layout {
     int lay[24];  // lay[0..3] register local bits; lay[4..m-1] thread local bits
 };
 
-// Thread bits in ascending local order: lanes 0..2 then sit on the lowest tile qubits,
-// which keeps global loads/stores in contiguous runs and LDS accesses bank-spread.
-static Layout make_layout(int m, const int reg[4]) {
+// Thread bits in ascending order of the global qubits the local bits stand for (qmap): lanes
+// 0..2 then sit on the lowest tile qubits, which keeps global loads/stores in contiguous runs.
+static Layout make_layout(int m, const int reg[4], const int *qmap) {
     Layout L;
     L.m = m;
     for (int s = 0; s < kRegBits; s++) L.lay[s] = reg[s];
@@ -533,6 +533,7 @@ static Layout make_layout(int m, const int reg[4]) {
         for (int s = 0; s < kRegBits; s++) in |= (reg[s] == b);
         if (!in) L.lay[u++] = b;
     }
+    std::sort(L.lay + kRegBits, L.lay + m, [&](int x, int y) { return qmap[x] < qmap[y]; });
     return L;
 }
 
@@ -603,16 +604,27 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         H.pool_off = (int32_t)(P.pool.size() / 2);
         H.grad_base = P.num_grad;
         H.ngrad = 0;
-        int loc[64];
+        // loc[q]: the local bit holding global qubit q; qmap[j]: the global qubit local bit j
+        // holds.  Both start as the ascending tile order; an uncontrolled SWAP of two tile qubits
+        // exchanges them instead of moving amplitudes, and the stage stores through qmap.
+        int loc[64], qmap[16];
         for (int i = 0; i < 64; i++) loc[i] = -1;
         {
             int j = 0;
             for (int b = 0; b < nq; b++)
                 if (S.T >> b & 1) {
                     H.T[j] = (uint8_t)b;
+                    qmap[j] = b;
                     loc[b] = j++;
                 }
         }
+        // local bits holding global qubits 0..2 (the coalescing lanes of loads and stores)
+        auto low_local = [&]() {
+            uint32_t lw = 0;
+            if (m - kRegBits >= 3)
+                for (int q = 0; q < 3 && q < nq; q++) lw |= 1u << loc[q];
+            return lw;
+        };
         // targets (local bits) needed by each op, for register choice and look-ahead
         const size_t nops = S.ops.size();
         std::vector<uint32_t> need(nops, 0);
@@ -657,10 +669,10 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             return true;
         };
         // new register set: targets of the first ready op that needs registers, then of the
-        // other ready ops, then of the waiting ops in order; never the lowest 3 local bits
-        // unless an op needs them
+        // other ready ops, then of the waiting ops in order; never the local bits of global
+        // qubits 0..2 unless an op needs them
         auto choose = [&](int out[4], bool initial) {
-            const uint32_t avoid = (m - kRegBits >= 3) ? 0x7u : 0u;
+            const uint32_t avoid = low_local();
             uint32_t set = 0;
             for (int pass = 0; pass < 2 && __builtin_popcount(set) < kRegBits; pass++)
                 for (size_t i = 0; i < nops && __builtin_popcount(set) < kRegBits; i++) {
@@ -679,7 +691,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         };
         int reg[4];
         choose(reg, true);
-        Layout cur = make_layout(m, reg);
+        Layout cur = make_layout(m, reg, qmap);
         for (int i = 0; i < m; i++) {
             H.lay0[i] = (uint8_t)cur.lay[i];
             H.gq0[i] = H.T[cur.lay[i]];
@@ -697,7 +709,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             return -1;
         };
         auto emit_remap = [&](const int nreg[4]) {
-            cur = make_layout(m, nreg);
+            cur = make_layout(m, nreg, qmap);
             Instr I;
             std::memset(&I, 0, sizeof(I));
             I.op = I_REMAP;
@@ -960,7 +972,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             int cnt = 0;
             for (uint64_t t = must; t && cnt < 4; t &= t - 1, cnt++) group |= t & (~t + 1);
             for (int sl = 0; sl < kRegBits && cnt < 4; sl++) {
-                const int qq = H.T[cur.lay[sl]];
+                const int qq = qmap[cur.lay[sl]];
                 if ((pending >> qq & 1) && !(group >> qq & 1)) {
                     group |= 1ull << qq;
                     cnt++;
@@ -981,6 +993,23 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 must &= ~g;
             }
         };
+        // an uncontrolled SWAP of two tile qubits: the two local bits trade global qubits
+        auto relabel_swap = [&](const EOp &e) {
+            const int qa = e.q[0], qb = e.q[1];
+            std::swap(qmap[loc[qa]], qmap[loc[qb]]);
+            std::swap(loc[qa], loc[qb]);
+            std::swap(ph[qa], ph[qb]);
+            const uint64_t ba = pending >> qa & 1, bb = pending >> qb & 1;
+            pending = (pending & ~((1ull << qa) | (1ull << qb))) | (bb << qa) | (ba << qb);
+            for (int q : {qa, qb})
+                for (uint32_t i : touch_list[q]) {
+                    if (fin[i]) continue;
+                    uint32_t l = 0;
+                    for (uint64_t t = eops[S.ops[i]].targets(); t; t &= t - 1) l |= 1u << loc[__builtin_ctzll(t)];
+                    need[i] = l;
+                }
+        };
+        auto is_relabel = [&](const EOp &e) { return e.kind == EOP_SWAP2 && e.cmask == 0 && e.pure_swap; };
         auto absorbable = [&](const EOp &e) {
             if (!(e.kind == EOP_DIAG && e.k == 1 && e.cmask == 0)) return false;
             const cd d0(e.m[0], e.m[1]), d1(e.m[2], e.m[3]);
@@ -1042,13 +1071,18 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             // phases first (free, and they make the ops behind them ready together); the scan
             // looks a bounded window ahead
             bool any = false;
-            for (size_t j = lo_open; j < nops && j < lo_open + 256; j++)
-                if (!fin[j] && absorbable(eops[S.ops[j]]) && ready(j)) {
-                    absorb(eops[S.ops[j]]);
-                    fin[j] = 1;
+            for (size_t j = lo_open; j < nops && j < lo_open + 256; j++) {
+                if (fin[j]) continue;
+                const EOp &x = eops[S.ops[j]];
+                const bool ab = absorbable(x), rl = !ab && is_relabel(x);
+                if ((ab || rl) && ready(j)) {
+                    fin[j] = 1;  // before relabel_swap, which refreshes the unfinished ops
+                    if (ab) absorb(x);
+                    else relabel_swap(x);
                     left--;
                     any = true;
                 }
+            }
             if (any) continue;
             // pick the first ready op that runs on the current registers
             uint32_t regset = 0;
@@ -1070,6 +1104,10 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             const EOp &e = eops[S.ops[i]];
             if (absorbable(e)) {
                 absorb(e);
+                continue;
+            }
+            if (is_relabel(e)) {
+                relabel_swap(e);
                 continue;
             }
             if (is_rotation(e)) {
@@ -1142,21 +1180,30 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 if (final_group) gphase = cd(1, 0);
             }
             if (gphase != cd(1, 0)) {
-                ph[H.T[cur.lay[0]]] = cd(1, 0);
-                emit_phase_group(1ull << H.T[cur.lay[0]], gphase);
+                ph[qmap[cur.lay[0]]] = cd(1, 0);
+                emit_phase_group(1ull << qmap[cur.lay[0]], gphase);
                 gphase = cd(1, 0);
             }
         }
-        // leave the tile with lanes on the lowest local bits for coalesced stores
+        // leave the tile with lanes 0..2 on global qubits 0..2 for coalesced stores: registers
+        // holding them trade places with the highest other local bits
         {
-            bool bad = false;
-            for (int s = 0; s < kRegBits; s++)
-                if (cur.lay[s] < 3 && m - kRegBits >= 3) bad = true;
-            if (bad) emit_remap(reg);
+            const uint32_t lw = low_local();
+            uint32_t rs = 0;
+            for (int s = 0; s < kRegBits; s++) rs |= 1u << cur.lay[s];
+            if (rs & lw) {
+                uint32_t keep = rs & ~lw;
+                for (int b = m - 1; b >= 0 && __builtin_popcount(keep) < kRegBits; b--)
+                    if (!((rs | lw) >> b & 1)) keep |= 1u << b;
+                int nreg[4], t = 0;
+                for (int b = 0; b < m; b++)
+                    if (keep >> b & 1) nreg[t++] = b;
+                emit_remap(nreg);
+            }
         }
         for (int i = 0; i < m; i++) {
             H.layN[i] = (uint8_t)cur.lay[i];
-            H.gqN[i] = H.T[cur.lay[i]];
+            H.gqN[i] = (uint8_t)qmap[cur.lay[i]];
         }
         H.ninstr = (int32_t)P.instrs.size() - H.instr_off;
         P.stages.push_back(H);

@@ -22,10 +22,22 @@ struct GAddr {
     uint64_t tg;
     uint64_t rq[4];
 };
+// strict: gq must be the tile list in layout order (stage loads); otherwise any arrangement of the
+// tile's qubits (stores: uncontrolled swaps inside a stage relabel local bits)
 bool global_layout(GAddr &G, const uint8_t *lay, const uint8_t *gq, const uint8_t *T, int m, uint32_t tid,
-                   uint64_t base) {
+                   uint64_t base, bool strict) {
     bool ok = true;
-    for (int i = 0; i < m; i++) ok &= gq[i] == T[lay[i]];
+    if (strict) {
+        for (int i = 0; i < m; i++) ok &= gq[i] == T[lay[i]];
+    } else {
+        uint64_t want = 0, got = 0;
+        for (int i = 0; i < m; i++) {
+            want |= 1ull << T[i];
+            ok &= !(got >> gq[i] & 1);
+            got |= 1ull << gq[i];
+        }
+        ok &= want == got;
+    }
     for (int s = 0; s < 4; s++) G.rq[s] = 1ull << gq[s];
     G.tg = base;
     for (int u = 0; u < m - 4; u++)
@@ -163,7 +175,7 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
             std::vector<uint32_t> lb(nthr);
             for (int t = 0; t < nthr; t++) {
                 GAddr G;
-                if (!global_layout(G, H.lay0, H.gq0, H.T, m, t, base)) {
+                if (!global_layout(G, H.lay0, H.gq0, H.T, m, t, base, true)) {
                     err = "gq0 disagrees with the tile list";
                     return 1;
                 }
@@ -433,7 +445,7 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
             }
             for (int t = 0; t < nthr; t++) {
                 GAddr G;
-                if (!global_layout(G, H.layN, H.gqN, H.T, m, t, base)) {
+                if (!global_layout(G, H.layN, H.gqN, H.T, m, t, base, false)) {
                     err = "gqN disagrees with the tile list";
                     return 1;
                 }
