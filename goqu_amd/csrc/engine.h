@@ -80,6 +80,9 @@ enum InstrOp : int32_t {
     I_SHEAR = 8,  // slot s[0]: k in-place shears, a0 += c0 a1, a1 += c1 a0, a0 += c2 a1 (the
                   // first k of them); c0, c1, c2 are doubles stored in the instruction at byte
                   // offsets 40, 48, 56 (set_shear_coefs), so the op needs no pool load
+    I_XSWAP = 9,  // register slot s[0] trades places with lane bit k (0..5) by cross-lane moves
+                  // (permlane swaps for lane bits 4, 5; row-shift DPP for 0..3); lay / kr give
+                  // the resulting layout and its LDS offsets, as for I_REMAP
 };
 
 // Instr::flags
@@ -87,7 +90,7 @@ constexpr int32_t F_PURE = 1;   // ANTI1 / SWAP2 with unit phases: a move
 constexpr int32_t F_TCTRL = 2;  // controls on thread-id bits (tmask)
 constexpr int32_t F_PAT = 4;    // controls on register slots (pat)
 constexpr int32_t F_TILE = 8;   // controls on qubits outside the tile (omask), uniform per tile
-constexpr int32_t F_OUTER = 16; // run by the kernel outside its inner op loop (REMAP, RESET, dense on 3-4 slots)
+constexpr int32_t F_OUTER = 16; // run by the kernel outside its inner op loop (REMAP, XSWAP, RESET, dense on 3-4 slots)
 constexpr int32_t F_REAL = 32;  // I_PARITY, I_DENSE: every matrix entry is real (only real parts are read)
 
 // I_PARITY canonical form (the kernel's view).  k = 2: slots lo < hi (smask); class

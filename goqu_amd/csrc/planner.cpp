@@ -523,7 +523,9 @@ struct Layout {
 
 // Thread bits in ascending order of the global qubits the local bits stand for (qmap): lanes
 // 0..2 then sit on the lowest tile qubits, which keeps global loads/stores in contiguous runs.
-static Layout make_layout(int m, const int reg[4], const int *qmap) {
+// pref: up to two local bits for thread bits 4 and 5, the lane bits that trade places with a
+// register slot by one permlane swap per word (bits already on lanes 0..2 stay there).
+static Layout make_layout(int m, const int reg[4], const int *qmap, const int pref[2] = nullptr) {
     Layout L;
     L.m = m;
     for (int s = 0; s < kRegBits; s++) L.lay[s] = reg[s];
@@ -534,6 +536,17 @@ static Layout make_layout(int m, const int reg[4], const int *qmap) {
         if (!in) L.lay[u++] = b;
     }
     std::sort(L.lay + kRegBits, L.lay + m, [&](int x, int y) { return qmap[x] < qmap[y]; });
+    if (pref && m - kRegBits > 5) {
+        for (int k = 0; k < 2; k++) {
+            if (pref[k] < 0) continue;
+            int at = -1;
+            for (int t = kRegBits; t < m; t++)
+                if (L.lay[t] == pref[k]) at = t;
+            if (at < kRegBits + 3) continue;  // a register slot or lanes 0..2
+            const int want = kRegBits + 4 + k;
+            std::swap(L.lay[at], L.lay[want]);
+        }
+    }
     return L;
 }
 
@@ -708,14 +721,62 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 if (cur.lay[u] == localbit) return u - kRegBits;
             return -1;
         };
-        auto emit_remap = [&](const int nreg[4]) {
-            cur = make_layout(m, nreg, qmap);
+        auto emit_remap = [&](const int nreg[4], const int *pref = nullptr) {
+            cur = make_layout(m, nreg, qmap, pref);
             Instr I;
             std::memset(&I, 0, sizeof(I));
             I.op = I_REMAP;
             I.flags = F_OUTER;
             for (int i = 0; i < m; i++) I.lay[i] = (uint8_t)cur.lay[i];
             P.instrs.push_back(I);
+        };
+        // Register slot s trades places with thread-id bit u < 6 (a lane bit) by cross-lane moves.
+        // Costs relative to a permlane swap (one instruction per word and register pair): a
+        // lane bit below 4 takes four instructions per word (two DPP moves, two selects), an LDS
+        // remap about ten (LDS bandwidth bound), whatever its size.
+        auto xswap_cost = [&](int u) { return u >= 4 ? 1 : 4; };
+        constexpr int kXswapBudget = 8;
+        auto emit_xswap = [&](int sl, int u) {
+            std::swap(cur.lay[sl], cur.lay[kRegBits + u]);
+            Instr I;
+            std::memset(&I, 0, sizeof(I));
+            I.op = I_XSWAP;
+            I.flags = F_OUTER;
+            I.k = u;
+            I.pat = 0xFFFFu;
+            I.s[0] = (uint8_t)sl;
+            for (int i = 0; i < m; i++) I.lay[i] = (uint8_t)cur.lay[i];
+            P.instrs.push_back(I);
+        };
+        // reach the register set nreg by lane exchanges when they cost less than an LDS remap
+        auto try_xswaps = [&](const int nreg[4]) {
+            uint32_t rs = 0, ns = 0;
+            for (int s2 = 0; s2 < kRegBits; s2++) {
+                rs |= 1u << cur.lay[s2];
+                ns |= 1u << nreg[s2];
+            }
+            const uint32_t in = ns & ~rs, out = rs & ~ns;
+            int cost = 0;
+            for (uint32_t t = in; t; t &= t - 1) {
+                int u = -1;
+                for (int k = kRegBits; k < m; k++)
+                    if (cur.lay[k] == __builtin_ctz(t)) u = k - kRegBits;
+                if (u < 0 || u > 5) return false;
+                cost += xswap_cost(u);
+            }
+            if (cost > kXswapBudget) return false;
+            uint32_t o = out;
+            for (uint32_t t = in; t; t &= t - 1) {
+                const int b = __builtin_ctz(t), ob = __builtin_ctz(o);
+                o &= o - 1;
+                int sl = -1, u = -1;
+                for (int k = 0; k < kRegBits; k++)
+                    if (cur.lay[k] == ob) sl = k;
+                for (int k = kRegBits; k < m; k++)
+                    if (cur.lay[k] == b) u = k - kRegBits;
+                emit_xswap(sl, u);
+            }
+            return true;
         };
         // controls resolved against the current layout: register pattern, thread-id mask and
         // mask of qubits outside the tile
@@ -1096,7 +1157,21 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             if (i == nops) {
                 int nreg[4];
                 choose(nreg, false);
-                emit_remap(nreg);
+                if (!try_xswaps(nreg)) {
+                    // the next two local bits the waiting ops need go on lanes 4 and 5
+                    uint32_t ns = 0;
+                    for (int k = 0; k < kRegBits; k++) ns |= 1u << nreg[k];
+                    const uint32_t lw = low_local();
+                    int pref[2] = {-1, -1}, np = 0;
+                    for (size_t j = lo_open; j < nops && np < 2; j++) {
+                        if (fin[j]) continue;
+                        for (uint32_t t = need[j] & ~ns & ~lw; t && np < 2; t &= t - 1) {
+                            const int b = __builtin_ctz(t);
+                            if (np == 0 || pref[0] != b) pref[np++] = b;
+                        }
+                    }
+                    emit_remap(nreg, pref);
+                }
                 continue;
             }
             fin[i] = 1;
@@ -1185,13 +1260,41 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                 gphase = cd(1, 0);
             }
         }
-        // leave the tile with lanes 0..2 on global qubits 0..2 for coalesced stores: registers
-        // holding them trade places with the highest other local bits
+        // leave the tile with lanes 0..2 on global qubits 0..2 for coalesced stores: by lane
+        // exchanges when those qubits sit in registers and cheap enough, else by an LDS remap in
+        // which registers holding them trade places with the highest other local bits
         {
             const uint32_t lw = low_local();
+            auto lanes_ok = [&]() {
+                uint32_t l3 = 0;
+                for (int k = 0; k < 3 && kRegBits + k < m; k++) l3 |= 1u << cur.lay[kRegBits + k];
+                return l3 == lw;
+            };
+            if (lw && !lanes_ok()) {
+                // plan exchanges: every register slot holding a low qubit with a lane 0..2 that does not
+                int pairs[3][2], np = 0, cost = 0;
+                uint32_t used = 0;
+                for (int s2 = 0; s2 < kRegBits; s2++)
+                    if (lw >> cur.lay[s2] & 1)
+                        for (int u = 0; u < 3; u++)
+                            if (!(used >> u & 1) && !(lw >> cur.lay[kRegBits + u] & 1)) {
+                                used |= 1u << u;
+                                pairs[np][0] = s2;
+                                pairs[np][1] = u;
+                                np++;
+                                cost += xswap_cost(u);
+                                break;
+                            }
+                Layout trial = cur;
+                for (int k = 0; k < np; k++) std::swap(trial.lay[pairs[k][0]], trial.lay[kRegBits + pairs[k][1]]);
+                uint32_t l3 = 0;
+                for (int k = 0; k < 3; k++) l3 |= 1u << trial.lay[kRegBits + k];
+                if (l3 == lw && cost <= kXswapBudget)
+                    for (int k = 0; k < np; k++) emit_xswap(pairs[k][0], pairs[k][1]);
+            }
             uint32_t rs = 0;
             for (int s = 0; s < kRegBits; s++) rs |= 1u << cur.lay[s];
-            if (rs & lw) {
+            if (lw && !lanes_ok()) {
                 uint32_t keep = rs & ~lw;
                 for (int b = m - 1; b >= 0 && __builtin_popcount(keep) < kRegBits; b--)
                     if (!((rs | lw) >> b & 1)) keep |= 1u << b;
@@ -1224,7 +1327,7 @@ void set_lds_offsets(Program &P) {
         offs(H.lay0, H.k0);
         for (int i = 0; i < H.ninstr; i++) {
             Instr &I = P.instrs[H.instr_off + i];
-            if (I.op == I_REMAP) offs(I.lay, I.kr);
+            if (I.op == I_REMAP || I.op == I_XSWAP) offs(I.lay, I.kr);
         }
     }
 }

@@ -424,6 +424,84 @@ __device__ __forceinline__ void op_swap2(cx<R> (&a)[16], cx<R> (&b)[16], const R
     });
 }
 
+// Register slot S trades places with lane bit U (I_XSWAP): for each register pair (x: bit S = 0,
+// y: bit S = 1), lanes with bit U set take y from lane - 2^U into x, lanes with it clear take x
+// from lane + 2^U into y.  Lane bits 5 and 4 are exactly v_permlane32_swap / v_permlane16_swap
+// (one instruction per 32-bit word and pair); lane bits 0..3 stay inside a 16-lane row, where
+// row_shr / row_shl DPP moves fetch the partner and a select keeps the other half.
+__device__ __forceinline__ void xswap_word(uint32_t &x, uint32_t &y, int U, bool hi) {
+    if (U == 5) {
+        const auto p = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+        x = p[0];
+        y = p[1];
+    } else if (U == 4) {
+        const auto p = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+        x = p[0];
+        y = p[1];
+    } else {
+        // bound_ctrl: lanes without a source in the row read 0; the select discards them
+        uint32_t nx, ny;
+        switch (U) {
+        case 0:
+            nx = __builtin_amdgcn_mov_dpp(y, 0x111, 0xF, 0xF, true);  // row_shr:1
+            ny = __builtin_amdgcn_mov_dpp(x, 0x101, 0xF, 0xF, true);  // row_shl:1
+            break;
+        case 1:
+            nx = __builtin_amdgcn_mov_dpp(y, 0x112, 0xF, 0xF, true);
+            ny = __builtin_amdgcn_mov_dpp(x, 0x102, 0xF, 0xF, true);
+            break;
+        case 2:
+            nx = __builtin_amdgcn_mov_dpp(y, 0x114, 0xF, 0xF, true);
+            ny = __builtin_amdgcn_mov_dpp(x, 0x104, 0xF, 0xF, true);
+            break;
+        default:
+            nx = __builtin_amdgcn_mov_dpp(y, 0x118, 0xF, 0xF, true);
+            ny = __builtin_amdgcn_mov_dpp(x, 0x108, 0xF, 0xF, true);
+            break;
+        }
+        x = hi ? nx : x;
+        y = hi ? y : ny;
+    }
+}
+template <int S, int U, typename R>
+__device__ __forceinline__ void op_xswap(cx<R> (&v)[16], uint32_t lane) {
+    const bool hi = (lane >> U) & 1u;
+    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (!(r >> S & 1)) {
+            constexpr int r1 = r | (1 << S);
+            constexpr int W = (int)sizeof(cx<R>) / 4;
+            typedef uint32_t words __attribute__((ext_vector_type(W)));
+            words x = __builtin_bit_cast(words, v[r]), y = __builtin_bit_cast(words, v[r1]);
+            static_for<0, W>([&](auto wc) __attribute__((always_inline)) {
+                constexpr int w = decltype(wc)::value;
+                uint32_t xw = x[w], yw = y[w];
+                xswap_word(xw, yw, U, hi);
+                x[w] = xw;
+                y[w] = yw;
+            });
+            v[r] = __builtin_bit_cast(cx<R>, x);
+            v[r1] = __builtin_bit_cast(cx<R>, y);
+        }
+    });
+}
+template <typename R>
+__device__ __forceinline__ void xswap_dispatch(cx<R> (&v)[16], int S, int U, uint32_t lane) {
+#define XS_CASE(SS, UU) \
+    case SS * 8 + UU: op_xswap<SS, UU>(v, lane); break;
+#define XS_ROW(SS) XS_CASE(SS, 0) XS_CASE(SS, 1) XS_CASE(SS, 2) XS_CASE(SS, 3) XS_CASE(SS, 4) XS_CASE(SS, 5)
+    switch (S * 8 + U) {
+        XS_ROW(0)
+        XS_ROW(1)
+        XS_ROW(2)
+        XS_ROW(3)
+    default:
+        break;
+    }
+#undef XS_ROW
+#undef XS_CASE
+}
+
 template <int S, typename R>
 __device__ __forceinline__ void op_reset(cx<R> (&a)[16]) {
     static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
@@ -915,6 +993,13 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 }
                 continue;
             }
+            if (I.op == I_XSWAP) {  // cross-lane exchange; the LDS offsets follow the new layout
+                xswap_dispatch(a, I.s[0], I.k, (uint32_t)lane);
+                if constexpr (PAIR) xswap_dispatch(b, I.s[0], I.k, (uint32_t)lane);
+                lb = lds_lane<R>(I.lay, m, tid);
+                load_koff(K, I.kr);
+                continue;
+            }
             // remap: the tile goes through LDS into the next layout
             const uint32_t lbn = lds_lane<R>(I.lay, m, tid);
             KOff Kn;
@@ -1037,6 +1122,12 @@ int upload_program(const Program &P0, int prec, hipStream_t s, DeviceProgram &dp
     return GOQU_OK;
 }
 
+// A stage without instructions that stores in its load order changes nothing (a stage of
+// relabelled SWAPs alone still permutes its tile and runs).
+static bool stage_is_identity(const StageHeader &H) {
+    return H.ninstr == 0 && std::memcmp(H.gq0, H.gqN, (size_t)H.m) == 0;
+}
+
 template <typename R>
 static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
     const unsigned char *blob = static_cast<const unsigned char *>(dp.blob);
@@ -1047,7 +1138,7 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
     std::vector<hipEvent_t> ev;
     for (size_t i = 0; i < dp.stages.size(); i++) {
         const int m = dp.stages[i].m;
-        if (dp.stages[i].ninstr == 0) continue;
+        if (stage_is_identity(dp.stages[i])) continue;
         const uint64_t ntiles = 1ull << (st.nq - m);
         const unsigned threads = 1u << (m - kRegBits);
         const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
@@ -1125,7 +1216,7 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
     }
     for (size_t i = 0; i < P.stages.size() && !rc; i++) {
         const StageHeader &H = P.stages[i];
-        if (H.ninstr == 0) continue;
+        if (stage_is_identity(H)) continue;
         const unsigned threads = 1u << (m - kRegBits);
         const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
         hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),

@@ -82,7 +82,7 @@ enum Cov {
     C_DENSE, C_DENSE_TCTRL, C_DENSE_PAT, C_DENSE_TILE, C_DENSE_BIG, C_DIAG_UNIFORM, C_DIAG_KL1, C_DIAG_KL2,
     C_DIAG_TCTRL, C_DIAG_TILE, C_DIAG_REGS, C_ANTI_PURE, C_ANTI_PHASED, C_ANTI_TCTRL, C_ANTI_PAT, C_SWAP_PURE,
     C_SWAP_PHASED, C_SWAP_TCTRL, C_SWAP_PAT, C_REMAP, C_GRAD, C_GRAD_CTRL, C_RESET, C_PARITY2, C_PARITY4,
-    C_PARITY_REAL, C_SHEAR2, C_SHEAR3, C_NCOV
+    C_PARITY_REAL, C_SHEAR2, C_SHEAR3, C_XSWAP_PERM, C_XSWAP_DPP, C_NCOV
 };
 int64_t g_cov[C_NCOV];
 void count(const Instr &I) {
@@ -114,6 +114,7 @@ void count(const Instr &I) {
     case I_GRAD: g_cov[(tc || pt || tl) ? C_GRAD_CTRL : C_GRAD]++; break;
     case I_RESET: g_cov[C_RESET]++; break;
     case I_SHEAR: g_cov[I.k == 2 ? C_SHEAR2 : C_SHEAR3]++; break;
+    case I_XSWAP: g_cov[I.k >= 4 ? C_XSWAP_PERM : C_XSWAP_DPP]++; break;
     case I_PARITY:
         g_cov[I.k == 2 ? C_PARITY2 : C_PARITY4]++;
         if (I.flags & F_REAL) g_cov[C_PARITY_REAL]++;
@@ -151,6 +152,18 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
         std::memcpy(lay, H.lay0, 16);
         for (int pc = 0; pc < H.ninstr; pc++) {
             const Instr &I = P.instrs[H.instr_off + pc];
+            if (I.op == I_XSWAP) {  // the carried layout must be the exchange of slot s[0] and lane bit k
+                if (I.s[0] > 3 || I.k < 0 || I.k > 5 || 4 + I.k >= m || !(I.flags & F_OUTER)) {
+                    err = "malformed lane exchange";
+                    return 1;
+                }
+                std::swap(lay[I.s[0]], lay[4 + I.k]);
+                for (int i = 0; i < m; i++)
+                    if (lay[i] != I.lay[i]) {
+                        err = "lane exchange layout mismatch";
+                        return 1;
+                    }
+            }
             if (I.op == I_REMAP) std::memcpy(lay, I.lay, 16);
         }
         for (int i = 0; i < m; i++)
@@ -223,6 +236,23 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                                 }
                             }
                         }
+                    for (int t = 0; t < nthr; t++) lb[t] = lds_lane(I.lay, m, t);
+                    K = I.kr;
+                    continue;
+                }
+                if (I.op == I_XSWAP) {
+                    // amplitude (thread t, register r) <- (t with lane bit u = bit S of r,
+                    // r with bit S = lane bit u of t)
+                    const int S = I.s[0], u = I.k;
+                    for (int w = 0; w < npass; w++) {
+                        std::vector<cd> old(arr[w]);
+                        for (int t = 0; t < nthr; t++)
+                            for (int r = 0; r < 16; r++) {
+                                const int tb = t >> u & 1, rb = r >> S & 1;
+                                const int ts = (t & ~(1 << u)) | (rb << u), rs = (r & ~(1 << S)) | (tb << S);
+                                arr[w][t * 16 + r] = old[ts * 16 + rs];
+                            }
+                    }
                     for (int t = 0; t < nthr; t++) lb[t] = lds_lane(I.lay, m, t);
                     K = I.kr;
                     continue;

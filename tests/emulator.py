@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -22,8 +23,11 @@ def lib():
         deps = SRC + [HERE.parent.parent / "goqu_amd" / "csrc" / "engine.h"]
         if not LIB.exists() or any(p.stat().st_mtime > LIB.stat().st_mtime for p in deps):
             LIB.parent.mkdir(exist_ok=True)
-            subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", str(LIB), *map(str, SRC)],
+            # build under a private name, then rename: parallel test workers may build at once
+            tmp = LIB.with_name(f"{LIB.name}.{os.getpid()}.tmp")
+            subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", str(tmp), *map(str, SRC)],
                            check=True)
+            os.replace(tmp, LIB)
         L = C.CDLL(str(LIB))
         L.emu_apply.argtypes = [C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(N.GoquOp), C.c_int, C.c_int,
                                 C.c_int, C.c_int, C.POINTER(C.c_int64)]
@@ -69,7 +73,8 @@ def sv_reverse(psi: np.ndarray, lam: np.ndarray, nq_user: int, buf: "N.OpBuffer"
 COVERAGE = ["dense", "dense_tctrl", "dense_pat", "dense_tile", "dense_big", "diag_uniform", "diag_kl1",
             "diag_kl2", "diag_tctrl", "diag_tile", "diag_regs", "anti_pure", "anti_phased", "anti_tctrl",
             "anti_pat", "swap_pure", "swap_phased", "swap_tctrl", "swap_pat", "remap", "grad", "grad_ctrl",
-            "reset", "parity2", "parity4", "parity_real", "shear2", "shear3"]
+            "reset", "parity2", "parity4", "parity_real", "shear2", "shear3", "xswap_perm",
+            "xswap_dpp"]
 
 
 def coverage(reset: bool = True) -> dict:

@@ -47,6 +47,24 @@ def test_random_circuits_c64(n):
     _check(c, TOL64, N.C64)
 
 
+@pytest.mark.parametrize("precision,tol", [(N.C128, TOL128), (N.C64, TOL64)])
+def test_swap_relabel_stages(precision, tol):
+    """SWAPs among high qubits only (a stage without instructions that stores permuted), SWAPs
+    that move low qubits, and SWAPs mixed with rotations and CNOTs, on random states."""
+    from test_planner_emulated import _swap_circuits, random_state
+
+    n = 14
+    for i, c in enumerate(_swap_circuits(n)):
+        s0 = random_state(n, 40 + i)
+        sim = statevector.New(n, precision)
+        sim.SetStateVector(s0)
+        sim.Apply(c)
+        got = sim.StateVector()
+        sim.Close()
+        want = O.sv_evolve(c, s0.copy())
+        assert np.max(np.abs(got - want)) <= tol, i
+
+
 def test_config1_parity_and_sampling():
     c = config1_circuit(20, 20, seed=11)
     sim, want = _check(c)

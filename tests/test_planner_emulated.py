@@ -43,6 +43,53 @@ def test_planner_high_qubits_and_stage_count():
     assert stages < gates / 4, (stages, gates)
 
 
+def _swap_circuits(n):
+    """SWAPs among high qubits only (a stage without instructions whose store is permuted), SWAPs
+    that also move low qubits (restored by lane exchanges), and SWAPs mixed with rotations and
+    CNOTs on low and high qubits."""
+    from goqu_amd.circuit import NewBuilder
+
+    b = NewBuilder("high", n)
+    for a, c in [(3, n - 1), (n - 1, 5), (4, n - 2), (6, 9)]:
+        b.SWAP(a, c)
+    high = b.Build()
+    b = NewBuilder("low", n)
+    for a, c in [(0, n - 1), (n - 1, 5), (3, n - 2), (1, 2), (0, 1)]:
+        b.SWAP(a, c)
+    low = b.Build()
+    b = NewBuilder("mixed", n)
+    rng = np.random.default_rng(n)
+    for layer in range(3):
+        for q in range(n):
+            b.U3(*rng.uniform(0, 2 * np.pi, 3), q)
+        for a, c in [(0, n - 1), (2, 7), (1, n - 3)]:
+            b.SWAP(a, c).CNOT(c, a)
+    return high, low, b.Build()
+
+
+def random_state(n, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return s / np.linalg.norm(s)
+
+
+@pytest.mark.parametrize("tile,low", [(12, 3), (10, 3), (8, 2)])
+def test_swaps_relabel_and_lane_exchanges(tile, low):
+    """Uncontrolled SWAPs relabel local bits (stores write through the relabelled map), and
+    register sets change by lane exchanges where cheaper than an LDS remap."""
+    n = 14
+    E.coverage(reset=True)
+    for i, c in enumerate(_swap_circuits(n)):
+        s0 = random_state(n, 40 + i)
+        s = s0.copy()
+        E.apply(s, n, O.sv_marshal(c), tile_bits=tile, low_bits=low)
+        assert np.max(np.abs(s - O.sv_evolve(c, s0.copy()))) < 1e-12
+    cov = E.coverage(reset=True)
+    assert cov["swap_pure"] == 0  # every uncontrolled SWAP was a relabelling
+    if tile == 12:
+        assert cov["xswap_perm"] + cov["xswap_dpp"] > 0
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 6])
 def test_dm_lowering_matches_oracle(n):
     """Density-matrix lowering (U on row qubits, conj(U) on column qubits, Kraus channels as
