@@ -55,7 +55,7 @@ struct Scratch {
 };
 
 // ---- launchers (stream-ordered) -------------------------------------------------------------
-PlanConfig default_plan_config(int prec, int nq);
+PlanConfig default_plan_config(int prec, int nq, bool paired);
 int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp);
 int run_program(State &st, const DeviceProgram &dp, hipStream_t s);
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s);  // plan + upload + run

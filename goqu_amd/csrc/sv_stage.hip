@@ -1033,13 +1033,13 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
 // Two instantiations: k_stage runs stages without 3-4-slot dense ops and so needs few enough
 // VGPRs for four waves per SIMD; k_stage_big runs the stages that hold them.
 template <typename R>
-__global__ void __launch_bounds__(256) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+__global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                uint64_t ntiles) {
     stage_body<R, false, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
 }
 template <typename R>
-__global__ void __launch_bounds__(256) k_stage_big(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+__global__ void __launch_bounds__(512) k_stage_big(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                    uint64_t ntiles) {
     stage_body<R, false, true>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
@@ -1070,13 +1070,18 @@ static void profile_add(double ms, int64_t launches, double bytes) {
 // ---------------------------------------------------------------------------------------------
 // Tile size: 2^12 amplitudes (64 KiB of LDS at complex128, two workgroups per CU).
 // GOQU_TILE_BITS (8..12) overrides it, for tuning.
-PlanConfig default_plan_config(int prec, int nq) {
+// Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS).  GOQU_TILE_BITS picks 8..13; 2^13
+// tiles (512 threads, 64 KiB) cut the sweeps of the 30-qubit depth-40 circuit from 58 to 50 but
+// leave its run time within 2 % (the sweeps are bound by FP64 issue, not by HBM), so the default
+// keeps the smaller tile.  Paired programs (adjoint reverse sweeps, two states per tile) stay
+// at most 2^12: their kernel holds twice the amplitudes.
+PlanConfig default_plan_config(int prec, int nq, bool paired) {
     PlanConfig c;
     c.tile_bits = 12;
     c.low_bits = 3;
     if (const char *e = std::getenv("GOQU_TILE_BITS")) {
         const int t = std::atoi(e);
-        if (t >= 8 && t <= 12) c.tile_bits = t;
+        if (t >= 8 && t <= (paired ? 12 : 13)) c.tile_bits = t;
     }
     (void)prec;
     (void)nq;
@@ -1243,7 +1248,7 @@ int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, s
              hipStream_t s) {
     slot_vals.assign(nslots, 0.0);
     if (eops.empty()) return GOQU_OK;
-    Program P = plan_program(eops, psi.nq, default_plan_config(psi.prec, psi.nq));
+    Program P = plan_program(eops, psi.nq, default_plan_config(psi.prec, psi.nq, true));
     std::vector<double> prog_vals;
     int rc = psi.prec == GOQU_C128 ? run_pair_t<double>(psi, lam, P, s, prog_vals)
                                    : run_pair_t<float>(psi, lam, P, s, prog_vals);
@@ -1257,7 +1262,7 @@ int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, s
 
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s) {
     if (eops.empty()) return GOQU_OK;
-    Program P = plan_program(eops, st.nq, default_plan_config(st.prec, st.nq));
+    Program P = plan_program(eops, st.nq, default_plan_config(st.prec, st.nq, false));
     // grow-only program buffer per host thread: every call ends synchronised, so the next call
     // on this thread may overwrite it
     static thread_local DeviceProgram dp;

@@ -19,7 +19,7 @@ def _padded(n, nmin=6):
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 5, 6, 7, 9, 12, 13])
-@pytest.mark.parametrize("tile,low", [(12, 3), (8, 2), (6, 2), (10, 4)])
+@pytest.mark.parametrize("tile,low", [(13, 3), (12, 3), (8, 2), (6, 2), (10, 4)])
 def test_planner_matches_oracle(n, tile, low):
     rng = np.random.default_rng(31 * n + tile)
     c = random_circuit(n, 5, rng)
@@ -73,7 +73,7 @@ def random_state(n, seed):
     return s / np.linalg.norm(s)
 
 
-@pytest.mark.parametrize("tile,low", [(12, 3), (10, 3), (8, 2)])
+@pytest.mark.parametrize("tile,low", [(13, 3), (12, 3), (10, 3), (8, 2)])
 def test_swaps_relabel_and_lane_exchanges(tile, low):
     """Uncontrolled SWAPs relabel local bits (stores write through the relabelled map), and
     register sets change by lane exchanges where cheaper than an LDS remap."""
