@@ -201,9 +201,19 @@ __device__ __forceinline__ void fma_inplace(double &acc, double x, double c) {
 __device__ __forceinline__ void fma_inplace(float &acc, float x, float c) {
     asm("v_fma_f32 %0, %1, %2, %0" : "+v"(acc) : "v"(x), "s"(c));
 }
+// a double swap as two v_swap_b32 on its halves (the halves are the double's own registers)
 __device__ __forceinline__ void swap_inplace(double &x, double &y) {
-    double t;
-    asm("v_mov_b64 %2, %0\n\tv_mov_b64 %0, %1\n\tv_mov_b64 %1, %2" : "+v"(x), "+v"(y), "=&v"(t));
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    u2 a = __builtin_bit_cast(u2, x), b = __builtin_bit_cast(u2, y);
+    uint32_t a0 = a[0], a1 = a[1], b0 = b[0], b1 = b[1];
+    asm("v_swap_b32 %0, %1" : "+v"(a0), "+v"(b0));
+    asm("v_swap_b32 %0, %1" : "+v"(a1), "+v"(b1));
+    a[0] = a0;
+    a[1] = a1;
+    b[0] = b0;
+    b[1] = b1;
+    x = __builtin_bit_cast(double, a);
+    y = __builtin_bit_cast(double, b);
 }
 __device__ __forceinline__ void swap_inplace(float &x, float &y) { asm("v_swap_b32 %0, %1" : "+v"(x), "+v"(y)); }
 template <typename R>
@@ -562,16 +572,22 @@ __device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const IH
         });
     };
     if (kl == 0 && !tc) {  // every lane uses the same phases
-        static_for<0, NV / CH>([&](auto cc) __attribute__((always_inline)) {
-            constexpr int c0 = CH * decltype(cc)::value;
-            cx<R> ph[CH];
-            load_cx_batch<R, CH>(tab + c0, ph);
-            static_for<0, CH>([&](auto jc) __attribute__((always_inline)) {
-                constexpr int j = decltype(jc)::value;
-                if ((nontriv >> (c0 + j)) & 1u)
-                    column(std::integral_constant<int, c0 + j>{}, std::integral_constant<bool, true>{}, ph[j]);
+        // phase tables usually have every column but the first nontrivial: no per-column tests
+        auto run = [&](auto densec) __attribute__((always_inline)) {
+            constexpr bool DC = decltype(densec)::value;
+            static_for<0, NV / CH>([&](auto cc) __attribute__((always_inline)) {
+                constexpr int c0 = CH * decltype(cc)::value;
+                cx<R> ph[CH];
+                load_cx_batch<R, CH>(tab + c0, ph);
+                static_for<0, CH>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int j = decltype(jc)::value;
+                    if ((DC && c0 + j > 0) || ((nontriv >> (c0 + j)) & 1u))
+                        column(std::integral_constant<int, c0 + j>{}, std::integral_constant<bool, true>{}, ph[j]);
+                });
             });
-        });
+        };
+        if ((nontriv | 1u) == (1u << NV) - 1u) run(std::true_type{});
+        else run(std::false_type{});
         return;
     }
     // lane index from thread-id bits and (uniform) bits of the tile base
