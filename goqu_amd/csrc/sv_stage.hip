@@ -79,6 +79,23 @@ __device__ __forceinline__ uint64_t rgof(const GAddr &G) {
            ((r & 8) ? G.rq[3] : 0ull);
 }
 
+// Tile loads and stores stream: each amplitude is read and written once per sweep, so they
+// carry the nontemporal hint.
+template <typename R>
+__device__ __forceinline__ cx<R> ld_stream(const cx<R> *p) {
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2 *>(p));
+    return {v.x, v.y};
+}
+template <typename R>
+__device__ __forceinline__ void st_stream(cx<R> *p, cx<R> a) {
+    typedef R v2 __attribute__((ext_vector_type(2)));
+    v2 v;
+    v.x = a.x;
+    v.y = a.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<v2 *>(p));
+}
+
 // per-thread part of the LDS byte offsets of a layout
 template <typename R>
 __device__ __forceinline__ uint32_t lds_lane(const uint8_t *__restrict__ lay, int m, uint32_t tid) {
@@ -792,8 +809,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             global_layout(G, hdr->gq0, m, tid, base);
             static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
                 constexpr int r = decltype(rc)::value;
-                a[r] = st[G.tg | rgof<r>(G)];
-                if constexpr (PAIR) b[r] = st2[G.tg | rgof<r>(G)];
+                a[r] = ld_stream(st + (G.tg | rgof<r>(G)));
+                if constexpr (PAIR) b[r] = ld_stream(st2 + (G.tg | rgof<r>(G)));
             });
         }
         uint32_t lb = lds_lane<R>(hdr->lay0, m, tid);
@@ -1030,8 +1047,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             global_layout(G, hdr->gqN, m, tid, base);
             static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
                 constexpr int r = decltype(rc)::value;
-                st[G.tg | rgof<r>(G)] = a[r];
-                if constexpr (PAIR) st2[G.tg | rgof<r>(G)] = b[r];
+                st_stream(st + (G.tg | rgof<r>(G)), a[r]);
+                if constexpr (PAIR) st_stream(st2 + (G.tg | rgof<r>(G)), b[r]);
             });
         }
     }
