@@ -33,8 +33,22 @@ HIP_FLAGS = [
 ]
 
 
-def _sources() -> list[Path]:
-    return sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp")))
+# sv_stage.hip holds six large kernel instantiations: each is compiled in an object of its own
+# (GOQU_STAGE_KERNEL = 1..6), next to the object with the file's host code, so they build in
+# parallel.
+STAGE_SPLIT = "sv_stage.hip"
+STAGE_KERNELS = 6
+
+
+def _jobs() -> list[tuple[Path, list[str], str]]:
+    jobs = []
+    for src in sorted(list(CSRC.glob("*.hip")) + list(CSRC.glob("*.cpp"))):
+        jobs.append((src, [], src.name + ".o"))
+        if src.name == STAGE_SPLIT:
+            for k in range(1, STAGE_KERNELS + 1):
+                jobs.append((src, [f"-DGOQU_STAGE_KERNEL={k}"], f"{src.name}.k{k}.o"))
+    # the longest compiles first
+    return sorted(jobs, key=lambda j: (j[0].name != STAGE_SPLIT or not j[1], j[0].name))
 
 
 # kernels whose speed depends on four waves per SIMD (at most 128 VGPRs); the build reports a
@@ -57,10 +71,11 @@ def _check_occupancy(src: Path, remarks: str) -> None:
                           file=sys.stderr)
 
 
-def _compile(src: Path) -> Path:
-    obj = OBJ / (src.name + ".o")
-    extra = ["-Rpass-analysis=kernel-resource-usage"] if src.name == "sv_stage.hip" else []
-    cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", str(src), "-o", str(obj)]
+def _compile(job: tuple[Path, list[str], str]) -> Path:
+    src, defs, name = job
+    obj = OBJ / name
+    extra = ["-Rpass-analysis=kernel-resource-usage"] if defs else []
+    cmd = [HIPCC, *HIP_FLAGS, *defs, *extra, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".cpp":
         cmd = [HIPCC, *HIP_FLAGS, "-x", "hip", "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -73,10 +88,10 @@ def _compile(src: Path) -> Path:
 
 def build_library(jobs: int | None = None) -> Path:
     OBJ.mkdir(exist_ok=True)
-    srcs = _sources()
-    jobs = jobs or min(len(srcs), os.cpu_count() or 4, 8)
+    work = _jobs()
+    jobs = jobs or min(len(work), os.cpu_count() or 4, 8)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        objs = list(ex.map(_compile, work))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -1087,6 +1087,39 @@ __global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<
     stage_body<R, true, true>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
 }
 
+// The six stage kernels take minutes each to compile.  build.py compiles this file once per
+// kernel with GOQU_STAGE_KERNEL = 1..6 (that kernel's code only), in parallel, and once without
+// it: the host code below, which refers to the kernels as extern instantiations.
+#define GOQU_STAGE_INST(EXT, K, R)                                                                \
+    EXT template __global__ void K<R>(cx<R> *__restrict__, const StageHeader *__restrict__,          \
+                                      const Instr *__restrict__, const R *__restrict__, uint64_t);
+#define GOQU_PAIR_INST(EXT, R)                                                                     \
+    EXT template __global__ void k_stage_pair<R>(cx<R> *__restrict__, cx<R> *__restrict__,           \
+                                                 const StageHeader *__restrict__, const Instr *__restrict__, \
+                                                 const R *__restrict__, uint64_t, double *__restrict__);
+#ifdef GOQU_STAGE_KERNEL
+#if GOQU_STAGE_KERNEL == 1
+GOQU_STAGE_INST(, k_stage, double)
+#elif GOQU_STAGE_KERNEL == 2
+GOQU_STAGE_INST(, k_stage, float)
+#elif GOQU_STAGE_KERNEL == 3
+GOQU_STAGE_INST(, k_stage_big, double)
+#elif GOQU_STAGE_KERNEL == 4
+GOQU_STAGE_INST(, k_stage_big, float)
+#elif GOQU_STAGE_KERNEL == 5
+GOQU_PAIR_INST(, double)
+#elif GOQU_STAGE_KERNEL == 6
+GOQU_PAIR_INST(, float)
+#endif
+}  // namespace goqu
+#else  // the host-code object
+GOQU_STAGE_INST(extern, k_stage, double)
+GOQU_STAGE_INST(extern, k_stage, float)
+GOQU_STAGE_INST(extern, k_stage_big, double)
+GOQU_STAGE_INST(extern, k_stage_big, float)
+GOQU_PAIR_INST(extern, double)
+GOQU_PAIR_INST(extern, float)
+
 // ---- profiling ---------------------------------------------------------------------------------
 static std::atomic<bool> g_profile{false};
 static std::mutex g_prof_mu;
@@ -1101,8 +1134,6 @@ static void profile_add(double ms, int64_t launches, double bytes) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Tile size: 2^12 amplitudes (64 KiB of LDS at complex128, two workgroups per CU).
-// GOQU_TILE_BITS (8..12) overrides it, for tuning.
 // Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS).  GOQU_TILE_BITS picks 8..13; 2^13
 // tiles (512 threads, 64 KiB) cut the sweeps of the 30-qubit depth-40 circuit from 58 to 50 but
 // leave its run time within 2 % (the sweeps are bound by FP64 issue, not by HBM), so the default
@@ -1329,3 +1360,4 @@ int goqu_profile_read(double *kernel_ms, int64_t *launches, double *bytes) {
     return GOQU_OK;
 }
 }
+#endif  // GOQU_STAGE_KERNEL
