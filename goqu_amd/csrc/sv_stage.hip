@@ -805,6 +805,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
         }
         cx<R> a[16], b[16];  // b: lambda in paired mode, unused otherwise
         {
+            // the tile's loads issue ahead of other waves' arithmetic (wave priority)
+            __builtin_amdgcn_s_setprio(2);
             GAddr G;
             global_layout(G, hdr->gq0, m, tid, base);
             static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
@@ -812,6 +814,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 a[r] = ld_stream(st + (G.tg | rgof<r>(G)));
                 if constexpr (PAIR) b[r] = ld_stream(st2 + (G.tg | rgof<r>(G)));
             });
+            __builtin_amdgcn_s_setprio(0);
         }
         uint32_t lb = lds_lane<R>(hdr->lay0, m, tid);
         KOff K;
@@ -1043,6 +1046,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             K = Kn;
         }
         {
+            __builtin_amdgcn_s_setprio(2);
             GAddr G;
             global_layout(G, hdr->gqN, m, tid, base);
             static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
@@ -1050,6 +1054,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 st_stream(st + (G.tg | rgof<r>(G)), a[r]);
                 if constexpr (PAIR) st_stream(st2 + (G.tg | rgof<r>(G)), b[r]);
             });
+            __builtin_amdgcn_s_setprio(0);
         }
     }
     if constexpr (PAIR) {
