@@ -77,6 +77,8 @@ _SIGS = {
     "goqu_dm_adjoint": ([_P, C.POINTER(GoquOp), C.c_int32, C.POINTER(GoquDeriv), C.c_int32, C.c_int32, C.c_int32,
                          _U64, _U64, _I32, _D, C.c_int32, _D, _D], C.c_int),
     "goqu_qaoa_batch": ([C.c_int32, _I32, _I32, _I64, _I32, _I64, _D, _D, _D], C.c_int),
+    "goqu_plan_stats": ([C.c_int32, C.c_int32, C.POINTER(GoquOp), C.c_int32, C.c_int32, _I64, _I64], C.c_int),
+    "goqu_debug_inject_fault": ([C.c_int32, C.c_int32], C.c_int),
     "goqu_profile_enable": ([C.c_int32], C.c_int),
     "goqu_profile_reset": ([], C.c_int),
     "goqu_profile_read": ([_D, _I64, _D], C.c_int),
@@ -176,6 +178,17 @@ def add_sv_gate(buf: OpBuffer, g: G.Gate, qubits: Sequence[int]) -> int:
         return buf.add(OP_CONTROLLED, qubits, inner, num_controls=g.NumControls())
     m = g.Matrix()
     return buf.add(OP_GATE, qubits, m, gate_id=g.tier)
+
+
+FAULT_DEVICE_ALLOC, FAULT_HOST_ALLOC = 1, 2
+
+
+def plan_stats(nq: int, buf: OpBuffer, density: bool = False, tile_bits: int = 0) -> Tuple[int, int]:
+    """(stage sweeps, instructions) of the planned program, computed on the host only."""
+    st, ni = C.c_int64(0), C.c_int64(0)
+    check(lib().goqu_plan_stats(nq, 1 if density else 0, buf.array(), len(buf), tile_bits, C.byref(st),
+                                C.byref(ni)))
+    return st.value, ni.value
 
 
 def profile(enable: bool) -> None:

@@ -86,12 +86,29 @@ def _compile(job: tuple[Path, list[str], str]) -> Path:
     return obj
 
 
-def build_library(jobs: int | None = None) -> Path:
+def _stale(job: tuple[Path, list[str], str]) -> bool:
+    """An object is stale when it is missing or older than its source or any header in csrc/ or
+    include/."""
+    obj = OBJ / job[2]
+    if not obj.exists():
+        return True
+    deps = [job[0], *CSRC.glob("*.h"), *(ROOT / "include").glob("*.h")]
+    return any(d.stat().st_mtime > obj.stat().st_mtime for d in deps)
+
+
+def build_library(jobs: int | None = None, incremental: bool = False) -> Path:
+    """Compiles every source (incremental=True: only objects older than their inputs) and links
+    the library."""
     OBJ.mkdir(exist_ok=True)
     work = _jobs()
-    jobs = jobs or min(len(work), os.cpu_count() or 4, 8)
+    if incremental:
+        todo = [j for j in work if _stale(j)]
+        done = [OBJ / j[2] for j in work if not _stale(j)]
+    else:
+        todo, done = work, []
+    jobs = jobs or max(1, min(len(todo), os.cpu_count() or 4, 8))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, work))
+        objs = list(ex.map(_compile, todo)) + done
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -109,7 +126,7 @@ def build_oracle() -> Path:
 
 
 def main() -> int:
-    lib = build_library()
+    lib = build_library(incremental="--incremental" in sys.argv)
     orc = build_oracle()
     print(f"built {lib}\nbuilt {orc}")
     return 0

@@ -86,8 +86,8 @@ int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goq
     lam.nq = work.nq = st.nq;
     lam.prec = work.prec = st.prec;
     lam.elem = work.elem = st.elem;
-    if (hipMalloc(&lam.amps, st.size() * st.elem) != hipSuccess ||
-        hipMalloc(&work.amps, st.size() * st.elem) != hipSuccess) {
+    if (dmalloc((void **)&lam.amps, st.size() * st.elem) != hipSuccess ||
+        dmalloc((void **)&work.amps, st.size() * st.elem) != hipSuccess) {
         (void)hipGetLastError();
         cleanup();
         return fail(GOQU_ERR_OUT_OF_MEMORY, "adjoint: cannot allocate two extra state vectors");
@@ -182,7 +182,7 @@ static int alloc_like(State &s, const State &ref) {
     s.nq = ref.nq;
     s.prec = ref.prec;
     s.elem = ref.elem;
-    if (hipMalloc(&s.amps, ref.size() * ref.elem) != hipSuccess) {
+    if (dmalloc((void **)&s.amps, ref.size() * ref.elem) != hipSuccess) {
         (void)hipGetLastError();
         s.amps = nullptr;
         return fail(GOQU_ERR_OUT_OF_MEMORY, "adjoint: cannot allocate a density-matrix buffer");
@@ -324,24 +324,28 @@ int goqu_sv_adjoint(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops, const g
                     int32_t num_derivs, int32_t num_params, int32_t num_terms, const uint64_t *xmask,
                     const uint64_t *zmask, const int32_t *num_y, const double *coeffs, double *energy,
                     double *grads) {
+    GOQU_GUARD_BEGIN
     if (!h || num_ops < 0 || num_derivs < 0 || num_params < 0 || num_terms < 0 || !energy ||
         (num_ops && !ops) || (num_derivs && !derivs) || (num_params && !grads) ||
         (num_terms && (!xmask || !zmask || !num_y || !coeffs)))
         return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return adjoint_sv(h->st, h->nq_user, ops, num_ops, derivs, num_derivs, num_params, num_terms, xmask, zmask, num_y,
                       coeffs, energy, grads, h->scr, h->stream);
+    GOQU_GUARD_END
 }
 
 int goqu_dm_adjoint(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops, const goqu_deriv_t *derivs,
                     int32_t num_derivs, int32_t num_params, int32_t num_terms, const uint64_t *xmask,
                     const uint64_t *zmask, const int32_t *num_y, const double *coeffs, int32_t checkpoint_every,
                     double *energy, double *grads) {
+    GOQU_GUARD_BEGIN
     if (!h || num_ops < 0 || num_derivs < 0 || num_params < 0 || num_terms < 0 || !energy ||
         (num_ops && !ops) || (num_derivs && !derivs) || (num_params && !grads) ||
         (num_terms && (!xmask || !zmask || !num_y || !coeffs)))
         return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return adjoint_dm(h->st, h->n, ops, num_ops, derivs, num_derivs, num_params, num_terms, xmask, zmask, num_y,
                       coeffs, checkpoint_every, energy, grads, h->scr, h->stream);
+    GOQU_GUARD_END
 }
 
 }  // extern "C"

@@ -1,18 +1,81 @@
 // api.hip — the extern "C" entry points of libgoqu_hip (declared in include/goqu_hip.h).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
+#include <new>
+#include <stdexcept>
 
 #include "runtime.h"
 
 namespace goqu {
 
-static thread_local std::string g_err;
+// The calling thread's last error message, in a fixed buffer: recording an error never allocates.
+static thread_local char g_err[1024];
 
-void set_error(const std::string &msg) { g_err = msg; }
-int fail(int code, const std::string &msg) {
-    g_err = msg;
+int fail_cstr(int code, const char *msg) noexcept {
+    std::strncpy(g_err, msg ? msg : "", sizeof(g_err) - 1);
+    g_err[sizeof(g_err) - 1] = 0;
     return code;
+}
+void set_error(const std::string &msg) { fail_cstr(0, msg.c_str()); }
+int fail(int code, const std::string &msg) { return fail_cstr(code, msg.c_str()); }
+
+int on_exception() noexcept {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return fail_cstr(GOQU_ERR_OUT_OF_MEMORY, "host memory exhausted");
+    } catch (const std::length_error &) {
+        return fail_cstr(GOQU_ERR_OUT_OF_MEMORY, "host container size limit exceeded");
+    } catch (const std::exception &e) {
+        return fail_cstr(GOQU_ERR_DEVICE, e.what());
+    } catch (...) {
+        return fail_cstr(GOQU_ERR_DEVICE, "unknown internal error");
+    }
+}
+
+// ---- fault injection ------------------------------------------------------------------------------
+static std::atomic<int> g_fault_kind{FAULT_NONE};
+static std::atomic<int> g_fault_countdown{0};
+
+static int arm_fault(int kind, int countdown) {
+    if (kind < FAULT_NONE || kind > FAULT_HOST_ALLOC || countdown < 0) return GOQU_ERR_INVALID_ARG;
+    g_fault_countdown.store(countdown);
+    g_fault_kind.store(countdown > 0 ? kind : FAULT_NONE);
+    return GOQU_OK;
+}
+
+// GOQU_FAULT_INJECT=device_alloc:<n> or host_alloc:<n> arms the hook when the library loads
+[[maybe_unused]] static const bool g_fault_env = [] {
+    if (const char *e = std::getenv("GOQU_FAULT_INJECT")) {
+        const char *c = std::strchr(e, ':');
+        const int n = c ? std::atoi(c + 1) : 1;
+        if (!std::strncmp(e, "device_alloc", 12)) arm_fault(FAULT_DEVICE_ALLOC, n);
+        if (!std::strncmp(e, "host_alloc", 10)) arm_fault(FAULT_HOST_ALLOC, n);
+    }
+    return true;
+}();
+
+bool fault_fires(int kind) noexcept {
+    if (g_fault_kind.load() != kind) return false;
+    if (g_fault_countdown.fetch_sub(1) != 1) return false;
+    g_fault_kind.store(FAULT_NONE);
+    return true;
+}
+
+void host_fault_point() {
+    if (fault_fires(FAULT_HOST_ALLOC)) throw std::bad_alloc();
+}
+
+hipError_t dmalloc(void **p, size_t bytes) {
+    if (fault_fires(FAULT_DEVICE_ALLOC)) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+    }
+    return hipMalloc(p, bytes);
 }
 
 static int check_device() {
@@ -26,7 +89,7 @@ static int alloc_state(State &st, int nq, int prec) {
     st.nq = nq;
     st.prec = prec;
     st.elem = prec == GOQU_C128 ? 16 : 8;
-    hipError_t e = hipMalloc(&st.amps, st.size() * st.elem);
+    hipError_t e = dmalloc((void **)&st.amps, st.size() * st.elem);
     if (e != hipSuccess) {
         st.amps = nullptr;
         (void)hipGetLastError();
@@ -98,18 +161,54 @@ using namespace goqu;
 
 extern "C" {
 
-const char *goqu_last_error(void) { return g_err.c_str(); }
+const char *goqu_last_error(void) { return g_err; }
+
+int goqu_debug_inject_fault(int32_t kind, int32_t countdown) {
+    GOQU_GUARD_BEGIN
+    if (arm_fault(kind, countdown)) return fail_cstr(GOQU_ERR_INVALID_ARG, "bad fault kind or countdown");
+    return GOQU_OK;
+    GOQU_GUARD_END
+}
 
 int goqu_device_count(int32_t *out) {
+    GOQU_GUARD_BEGIN
     if (!out) return fail(GOQU_ERR_INVALID_ARG, "null pointer");
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     *out = e == hipSuccess ? n : 0;
     return GOQU_OK;
+    GOQU_GUARD_END
+}
+
+// ---- planning on the host (no device needed) -------------------------------------------------------
+int goqu_plan_stats(int32_t num_qubits, int32_t density, const goqu_op_t *ops, int32_t num_ops, int32_t tile_bits,
+                    int64_t *stages, int64_t *instrs) {
+    GOQU_GUARD_BEGIN
+    if (num_ops < 0 || (num_ops > 0 && !ops) || !stages || !instrs)
+        return fail(GOQU_ERR_INVALID_ARG, "bad argument");
+    const int maxq = density ? 16 : 34;
+    if (num_qubits < 1 || num_qubits > maxq) return fail(GOQU_ERR_INVALID_ARG, "qubit count out of range");
+    if (tile_bits != 0 && (tile_bits < 8 || tile_bits > kMaxTileBits))
+        return fail(GOQU_ERR_INVALID_ARG, "tile_bits must be 0 (default) or 8..13");
+    host_fault_point();
+    std::vector<EOp> eops;
+    std::string err;
+    const int nqi = std::max(density ? 2 * num_qubits : num_qubits, kMinInternalQubits);
+    int rc = density ? lower_dm_ops(ops, num_ops, num_qubits, eops, err)
+                     : lower_sv_ops(ops, num_ops, num_qubits, eops, err);
+    if (rc) return fail(rc, err);
+    PlanConfig cfg = default_plan_config(GOQU_C128, nqi, false);
+    if (tile_bits) cfg.tile_bits = tile_bits;
+    const Program P = plan_program(eops, nqi, cfg);
+    *stages = (int64_t)P.stages.size();
+    *instrs = (int64_t)P.instrs.size();
+    return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 // ---- statevector ----------------------------------------------------------------------------------
 int goqu_sv_create(int32_t num_qubits, int32_t precision, goqu_sv_t **out) {
+    GOQU_GUARD_BEGIN
     if (!out) return fail(GOQU_ERR_INVALID_ARG, "null output handle");
     *out = nullptr;
     if (num_qubits < 1 || num_qubits > 34)
@@ -137,9 +236,11 @@ int goqu_sv_create(int32_t num_qubits, int32_t precision, goqu_sv_t **out) {
     }
     *out = h;
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_sv_destroy(goqu_sv_t *h) {
+    GOQU_GUARD_BEGIN
     if (!h) return GOQU_OK;
     if (h->stream) {
         (void)hipStreamSynchronize(h->stream);
@@ -148,26 +249,33 @@ int goqu_sv_destroy(goqu_sv_t *h) {
     free_state(h->st);
     delete h;
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_sv_reset(goqu_sv_t *h) {
+    GOQU_GUARD_BEGIN
     if (!h) return fail(GOQU_ERR_INVALID_ARG, "null handle");
     int rc = fill_basis(h->st, 0, h->stream);
     if (rc) return rc;
     GOQU_HIP(hipStreamSynchronize(h->stream));
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_sv_apply(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops) {
+    GOQU_GUARD_BEGIN
     if (!h || (num_ops > 0 && !ops)) return fail(GOQU_ERR_INVALID_ARG, "null argument");
+    host_fault_point();
     std::vector<EOp> eops;
     std::string err;
     int rc = lower_sv_ops(ops, num_ops, h->nq_user, eops, err);
     if (rc) return fail(rc, err);
     return run_eops(h->st, eops, h->stream);
+    GOQU_GUARD_END
 }
 
 int goqu_sv_get_state(goqu_sv_t *h, double *amps, uint64_t offset, uint64_t count) {
+    GOQU_GUARD_BEGIN
     if (!h || (count && !amps)) return fail(GOQU_ERR_INVALID_ARG, "null argument");
     const uint64_t N = 1ull << h->nq_user;
     if (offset > N || count > N - offset) return fail(GOQU_ERR_INVALID_ARG, "range beyond the state");
@@ -191,9 +299,11 @@ int goqu_sv_get_state(goqu_sv_t *h, double *amps, uint64_t offset, uint64_t coun
         GOQU_HIP(hipStreamSynchronize(h->stream));
     }
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_sv_set_state(goqu_sv_t *h, const double *amps, uint64_t offset, uint64_t count) {
+    GOQU_GUARD_BEGIN
     if (!h || (count && !amps)) return fail(GOQU_ERR_INVALID_ARG, "null argument");
     const uint64_t N = 1ull << h->nq_user;
     if (offset > N || count > N - offset) return fail(GOQU_ERR_INVALID_ARG, "range beyond the state");
@@ -216,21 +326,27 @@ int goqu_sv_set_state(goqu_sv_t *h, const double *amps, uint64_t offset, uint64_
         GOQU_HIP(hipStreamSynchronize(h->stream));
     }
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_sv_sample(goqu_sv_t *h, const double *uniforms, int64_t shots, uint64_t *out_idx) {
+    GOQU_GUARD_BEGIN
     if (!h || shots < 0 || (shots > 0 && (!uniforms || !out_idx))) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return sample_state(h->st, 1ull << h->nq_user, uniforms, shots, out_idx, h->scr, h->stream);
+    GOQU_GUARD_END
 }
 
 int goqu_sv_expect_paulis(goqu_sv_t *h, int32_t num_terms, const uint64_t *xmask, const uint64_t *zmask,
                           const int32_t *num_y, const double *coeffs, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || num_terms < 0 || (num_terms > 0 && (!xmask || !zmask || !num_y || !coeffs || !out)))
         return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return expect_terms(h->st, h->nq_user, num_terms, xmask, zmask, num_y, coeffs, out, h->scr, h->stream);
+    GOQU_GUARD_END
 }
 
 int goqu_sv_expectation_z(goqu_sv_t *h, uint64_t mask, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || !out) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     const uint64_t x = 0;
     const int32_t ny = 0;
@@ -240,9 +356,11 @@ int goqu_sv_expectation_z(goqu_sv_t *h, uint64_t mask, double *out) {
     if (rc) return rc;
     *out = r[0];
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_sv_measure_qubit(goqu_sv_t *h, int32_t qubit, double u, int32_t *outcome) {
+    GOQU_GUARD_BEGIN
     if (!h || !outcome || qubit < 0 || qubit >= h->nq_user) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     double p0 = 0;
     int rc = prob_zero(h->st, qubit, &p0, h->scr, h->stream);
@@ -255,15 +373,19 @@ int goqu_sv_measure_qubit(goqu_sv_t *h, int32_t qubit, double u, int32_t *outcom
     if (rc) return rc;
     GOQU_HIP(hipStreamSynchronize(h->stream));
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_sv_norm2(goqu_sv_t *h, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || !out) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return prob_zero(h->st, -1, out, h->scr, h->stream);
+    GOQU_GUARD_END
 }
 
 // ---- density matrix -------------------------------------------------------------------------------------
 int goqu_dm_create(int32_t num_qubits, goqu_dm_t **out) {
+    GOQU_GUARD_BEGIN
     if (!out) return fail(GOQU_ERR_INVALID_ARG, "null output handle");
     *out = nullptr;
     if (num_qubits < 1 || num_qubits > 16)
@@ -278,8 +400,8 @@ int goqu_dm_create(int32_t num_qubits, goqu_dm_t **out) {
         return rc;
     }
     const size_t pb = (size_t(1) << num_qubits) * sizeof(double);
-    if (hipStreamCreate(&h->stream) != hipSuccess || hipMalloc(&h->dprobs, pb) != hipSuccess ||
-        hipMalloc(&h->dtmp, pb) != hipSuccess) {
+    if (hipStreamCreate(&h->stream) != hipSuccess || dmalloc((void **)&h->dprobs, pb) != hipSuccess ||
+        dmalloc((void **)&h->dtmp, pb) != hipSuccess) {
         goqu_dm_destroy(h);
         return fail(GOQU_ERR_DEVICE, "density matrix set-up failed");
     }
@@ -291,9 +413,11 @@ int goqu_dm_create(int32_t num_qubits, goqu_dm_t **out) {
     }
     *out = h;
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_destroy(goqu_dm_t *h) {
+    GOQU_GUARD_BEGIN
     if (!h) return GOQU_OK;
     if (h->stream) {
         (void)hipStreamSynchronize(h->stream);
@@ -304,26 +428,33 @@ int goqu_dm_destroy(goqu_dm_t *h) {
     free_state(h->st);
     delete h;
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_reset(goqu_dm_t *h) {
+    GOQU_GUARD_BEGIN
     if (!h) return fail(GOQU_ERR_INVALID_ARG, "null handle");
     int rc = fill_basis(h->st, 0, h->stream);
     if (rc) return rc;
     GOQU_HIP(hipStreamSynchronize(h->stream));
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_apply(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops) {
+    GOQU_GUARD_BEGIN
     if (!h || (num_ops > 0 && !ops)) return fail(GOQU_ERR_INVALID_ARG, "null argument");
+    host_fault_point();
     std::vector<EOp> eops;
     std::string err;
     int rc = lower_dm_ops(ops, num_ops, h->n, eops, err);
     if (rc) return fail(rc, err);
     return run_eops(h->st, eops, h->stream);
+    GOQU_GUARD_END
 }
 
 int goqu_dm_get(goqu_dm_t *h, double *rho, uint64_t offset, uint64_t count) {
+    GOQU_GUARD_BEGIN
     if (!h || (count && !rho)) return fail(GOQU_ERR_INVALID_ARG, "null argument");
     const uint64_t N = 1ull << (2 * h->n);
     if (offset > N || count > N - offset) return fail(GOQU_ERR_INVALID_ARG, "range beyond the matrix");
@@ -332,9 +463,11 @@ int goqu_dm_get(goqu_dm_t *h, double *rho, uint64_t offset, uint64_t count) {
                             h->stream));
     GOQU_HIP(hipStreamSynchronize(h->stream));
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_set(goqu_dm_t *h, const double *rho, uint64_t offset, uint64_t count) {
+    GOQU_GUARD_BEGIN
     if (!h || (count && !rho)) return fail(GOQU_ERR_INVALID_ARG, "null argument");
     const uint64_t N = 1ull << (2 * h->n);
     if (offset > N || count > N - offset) return fail(GOQU_ERR_INVALID_ARG, "range beyond the matrix");
@@ -343,9 +476,11 @@ int goqu_dm_set(goqu_dm_t *h, const double *rho, uint64_t offset, uint64_t count
                             h->stream));
     GOQU_HIP(hipStreamSynchronize(h->stream));
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_probs(goqu_dm_t *h, const double *p01, const double *p10, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || !out) return fail(GOQU_ERR_INVALID_ARG, "null argument");
     int rc = dm_diag(h->st, h->n, h->dprobs, h->stream);
     if (rc) return rc;
@@ -359,18 +494,22 @@ int goqu_dm_probs(goqu_dm_t *h, const double *p01, const double *p10, double *ou
     GOQU_HIP(hipMemcpyAsync(out, h->dprobs, (size_t(1) << h->n) * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     GOQU_HIP(hipStreamSynchronize(h->stream));
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_sample(goqu_dm_t *h, const double *probs, const double *uniforms, int64_t shots, uint64_t *out_idx) {
+    GOQU_GUARD_BEGIN
     if (!h || !probs || shots < 0 || (shots > 0 && (!uniforms || !out_idx)))
         return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     const uint64_t dim = 1ull << h->n;
     GOQU_HIP(hipMemcpyAsync(h->dtmp, probs, dim * sizeof(double), hipMemcpyHostToDevice, h->stream));
     return sample_probs(h->dtmp, dim, uniforms, shots, out_idx, h->scr, h->stream);
+    GOQU_GUARD_END
 }
 
 int goqu_dm_expect_paulis(goqu_dm_t *h, int32_t num_terms, const uint64_t *xmask, const uint64_t *zmask,
                           const int32_t *num_y, const double *coeffs, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || num_terms < 0 || (num_terms > 0 && (!xmask || !zmask || !num_y || !coeffs || !out)))
         return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     const uint64_t valid = (1ull << h->n) - 1;
@@ -388,9 +527,11 @@ int goqu_dm_expect_paulis(goqu_dm_t *h, int32_t num_terms, const uint64_t *xmask
         out[2 * t + 1] = cr * si + ci * sr;
     }
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_expectation_z(goqu_dm_t *h, uint64_t mask, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || !out) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     if (mask & ~((1ull << h->n) - 1)) return fail(GOQU_ERR_INVALID_ARG, "mask beyond the qubit count");
     double r[2];
@@ -398,24 +539,29 @@ int goqu_dm_expectation_z(goqu_dm_t *h, uint64_t mask, double *out) {
     if (rc) return rc;
     *out = r[0];
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 
 int goqu_dm_purity(goqu_dm_t *h, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || !out) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return dm_purity(h->st, h->n, out, h->scr, h->stream);
+    GOQU_GUARD_END
 }
 
 int goqu_dm_fidelity(goqu_dm_t *h, const double *pure, double *out) {
+    GOQU_GUARD_BEGIN
     if (!h || !pure || !out) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     const uint64_t dim = 1ull << h->n;
     void *dpsi = nullptr;
-    GOQU_HIP(hipMalloc(&dpsi, dim * 16));
+    GOQU_HIP(dmalloc((void **)&dpsi, dim * 16));
     hipError_t e = hipMemcpyAsync(dpsi, pure, dim * 16, hipMemcpyHostToDevice, h->stream);
     int rc = e == hipSuccess ? dm_fidelity(h->st, h->n, dpsi, out, h->scr, h->stream)
                              : fail(GOQU_ERR_DEVICE, hipGetErrorString(e));
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(dpsi);
     return rc;
+    GOQU_GUARD_END
 }
 
 }  // extern "C"

@@ -50,6 +50,10 @@ struct EOp {
     // which of the q[] are targets that must sit in registers (DENSE, ANTI1, SWAP2, RESET)
     bool needs_regs() const { return kind != EOP_DIAG; }
     bool is_grad() const { return kind == EOP_GRAD; }
+    // The statevector reset a0 <- |(a0, a1)| is not linear: it discards any phase the two
+    // amplitudes of a pair share, so it commutes with no op on any qubit (not even a diagonal
+    // one on another qubit).  Every reordering treats it as touching all qubits non-diagonally.
+    bool is_barrier() const { return kind == EOP_RESET; }
     uint64_t touch() const {
         uint64_t t = cmask;
         for (int i = 0; i < k; i++) t |= 1ull << q[i];

@@ -424,6 +424,11 @@ static std::vector<EOp> fuse_single_qubit(const std::vector<EOp> &in) {
     int last[64];
     for (int i = 0; i < 64; i++) last[i] = -1;
     for (const EOp &e : in) {
+        if (e.is_barrier()) {  // nothing fuses across a reset, on any qubit
+            out.push_back(e);
+            for (int i = 0; i < 64; i++) last[i] = (int)out.size() - 1;
+            continue;
+        }
         if (fusable1(e)) {
             int l = last[e.q[0]];
             if (l >= 0 && fusable1(out[l]) && out[l].q[0] == e.q[0]) {
@@ -575,7 +580,8 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         for (size_t i = first_open; i < eops.size(); i++) {
             if (done[i]) continue;
             const EOp &e = eops[i];
-            const uint64_t nd = e.nondiag(), dg = e.diag(), tg = e.targets();
+            const uint64_t nd = e.is_barrier() ? all : e.nondiag(), dg = e.is_barrier() ? 0 : e.diag(),
+                           tg = e.targets();
             const bool conflict = (nd & (b_nd | b_dg)) || (dg & b_nd);
             bool take = false;
             if (!conflict && !(e.is_grad() && S.ngrad >= kMaxGradPerStage)) {
@@ -660,12 +666,16 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         // earlier unfinished op acts non-diagonally (ops commute when all shared qubits are
         // diagonal for both).  Per-qubit op lists with lazily advanced heads make a check cost
         // O(qubits of the op).
+        // A reset (a barrier) counts as acting non-diagonally on every qubit: it waits for every
+        // earlier op and every later op waits for it.
+        auto sched_touch = [&](const EOp &x) { return x.is_barrier() ? all : x.touch(); };
+        auto sched_nd = [&](const EOp &x) { return x.is_barrier() ? all : x.nondiag(); };
         std::vector<std::vector<uint32_t>> touch_list(64), nd_list(64);
         std::vector<size_t> touch_head(64, 0), nd_head(64, 0);
         for (size_t i = 0; i < nops; i++) {
             const EOp &x = eops[S.ops[i]];
-            for (uint64_t t = x.touch(); t; t &= t - 1) touch_list[__builtin_ctzll(t)].push_back((uint32_t)i);
-            for (uint64_t t = x.nondiag(); t; t &= t - 1) nd_list[__builtin_ctzll(t)].push_back((uint32_t)i);
+            for (uint64_t t = sched_touch(x); t; t &= t - 1) touch_list[__builtin_ctzll(t)].push_back((uint32_t)i);
+            for (uint64_t t = sched_nd(x); t; t &= t - 1) nd_list[__builtin_ctzll(t)].push_back((uint32_t)i);
         }
         auto first_open = [&](const std::vector<uint32_t> &L, size_t &h) -> size_t {
             while (h < L.size() && fin[L[h]]) h++;
@@ -673,8 +683,8 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         };
         auto ready = [&](size_t i) {
             const EOp &x = eops[S.ops[i]];
-            const uint64_t nd = x.nondiag();
-            for (uint64_t t = x.touch(); t; t &= t - 1) {
+            const uint64_t nd = sched_nd(x);
+            for (uint64_t t = sched_touch(x); t; t &= t - 1) {
                 const int q = __builtin_ctzll(t);
                 const size_t f = (nd >> q & 1) ? first_open(touch_list[q], touch_head[q]) : first_open(nd_list[q], nd_head[q]);
                 if (f < i) return false;
@@ -1234,6 +1244,27 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             }
             // pending phases pass uncontrolled moves: a swap exchanges the two qubits' phases,
             // [a0, a1] <- [p0 a1, p1 a0] turns diag(1, f) into f diag(1, 1/f)
+            if (e.is_barrier()) {
+                // The reset keeps |(a0, a1)| of the state it sees: every pending phase and the
+                // magnitude of the deferred global factor must be in the stored amplitudes first
+                // (the reset's 1e-15 threshold compares true norms).  The global factor's phase
+                // is discarded by the reset itself.
+                const double mag = std::abs(gphase);
+                bool scaled = false;
+                for (uint64_t must = pending; must;) {
+                    const uint64_t g = next_group(must);
+                    emit_phase_group(g, scaled ? cd(1, 0) : cd(mag, 0));
+                    scaled = true;
+                    must &= ~g;
+                }
+                if (!scaled && mag != 1.0) {
+                    ph[e.q[0]] = cd(1, 0);
+                    emit_phase_group(1ull << e.q[0], cd(mag, 0));
+                }
+                gphase = cd(1, 0);
+                emit_eop(e);
+                continue;
+            }
             if (e.kind == EOP_SWAP2 && e.cmask == 0) {
                 const int qa = e.q[0], qb = e.q[1];
                 std::swap(ph[qa], ph[qb]);

@@ -338,6 +338,7 @@ using namespace goqu;
 extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits, const int32_t *layers,
                                const int64_t *edge_offsets, const int32_t *edges, const int64_t *param_offsets,
                                const double *params, double *energies, double *grads) {
+    GOQU_GUARD_BEGIN
     if (num_instances < 0 || (num_instances > 0 && (!num_qubits || !layers || !edge_offsets || !edges ||
                                                      !param_offsets || !params || !energies || !grads)))
         return fail(GOQU_ERR_INVALID_ARG, "bad argument");
@@ -425,21 +426,21 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
         woff[s] = wtotal;
         wtotal += launches[s].size();
     }
-    GOQU_HIP(hipMalloc(&d_inst.p, insts.size() * sizeof(QInst)));
-    GOQU_HIP(hipMalloc(&d_small.p, std::max<size_t>(1, small_ids.size()) * 4));
-    GOQU_HIP(hipMalloc(&d_large.p, std::max<size_t>(1, large_ids.size()) * 4));
-    GOQU_HIP(hipMalloc(&d_goff.p, goff.size() * 8));
-    GOQU_HIP(hipMalloc(&d_e.p, num_instances * 8));
-    GOQU_HIP(hipMalloc(&d_g.p, std::max<size_t>(1, ng) * 8));
-    GOQU_HIP(hipMalloc(&d_work.p, std::max<size_t>(1, wtotal) * sizeof(QWork)));
+    GOQU_HIP(dmalloc((void **)&d_inst.p, insts.size() * sizeof(QInst)));
+    GOQU_HIP(dmalloc((void **)&d_small.p, std::max<size_t>(1, small_ids.size()) * 4));
+    GOQU_HIP(dmalloc((void **)&d_large.p, std::max<size_t>(1, large_ids.size()) * 4));
+    GOQU_HIP(dmalloc((void **)&d_goff.p, goff.size() * 8));
+    GOQU_HIP(dmalloc((void **)&d_e.p, num_instances * 8));
+    GOQU_HIP(dmalloc((void **)&d_g.p, std::max<size_t>(1, ng) * 8));
+    GOQU_HIP(dmalloc((void **)&d_work.p, std::max<size_t>(1, wtotal) * sizeof(QWork)));
     if (pool_elems) {
-        hipError_t e = hipMalloc(&d_pool.p, pool_elems * sizeof(c2d));
+        hipError_t e = dmalloc((void **)&d_pool.p, pool_elems * sizeof(c2d));
         if (e != hipSuccess) {
             (void)hipGetLastError();
             return fail(GOQU_ERR_OUT_OF_MEMORY, "qaoa batch: state pool of " + std::to_string(pool_elems * 16) +
                                                     " bytes does not fit; split the batch");
         }
-        GOQU_HIP(hipMalloc(&d_parts.p, std::max<int64_t>(1, parts) * 8));
+        GOQU_HIP(dmalloc((void **)&d_parts.p, std::max<int64_t>(1, parts) * 8));
     }
     hipStream_t s;
     GOQU_HIP(hipStreamCreate(&s));
@@ -503,4 +504,5 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     GOQU_HIP(hipMemcpy(energies, d_e.p, num_instances * 8, hipMemcpyDeviceToHost));
     if (ng) GOQU_HIP(hipMemcpy(grads + param_offsets[0], d_g.p, ng * 8, hipMemcpyDeviceToHost));
     return GOQU_OK;
+    GOQU_GUARD_END
 }

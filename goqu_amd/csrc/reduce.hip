@@ -15,7 +15,7 @@ int Scratch::ensure(size_t need) {
     if (ptr) GOQU_HIP(hipFree(ptr));
     ptr = nullptr;
     bytes = 0;
-    GOQU_HIP(hipMalloc(&ptr, need));
+    GOQU_HIP(dmalloc((void **)&ptr, need));
     bytes = need;
     return GOQU_OK;
 }

@@ -15,6 +15,27 @@ namespace goqu {
 
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
+// fail() without allocating: usable while std::bad_alloc is in flight
+int fail_cstr(int code, const char *msg) noexcept;
+
+// Entry-point bodies run inside GOQU_GUARD_BEGIN / GOQU_GUARD_END: a C++ exception (host memory
+// exhausted in the planner's or lowering's containers, a length error) becomes a status code and
+// never crosses the C ABI into a Go or Python host.
+int on_exception() noexcept;
+#define GOQU_GUARD_BEGIN try {
+#define GOQU_GUARD_END \
+    }                  \
+    catch (...) { return ::goqu::on_exception(); }
+
+// Fault injection for tests (goqu_debug_inject_fault, or GOQU_FAULT_INJECT=<kind>:<n> in the
+// environment at load time): the n-th device allocation after arming reports a HIP failure, or the
+// n-th lowering of an op list throws std::bad_alloc.
+enum FaultKind { FAULT_NONE = 0, FAULT_DEVICE_ALLOC = 1, FAULT_HOST_ALLOC = 2 };
+bool fault_fires(int kind) noexcept;
+// hipMalloc through the fault-injection hook
+hipError_t dmalloc(void **p, size_t bytes);
+// throws std::bad_alloc when a host-allocation fault is armed and due (before op-list lowering)
+void host_fault_point();
 
 #define GOQU_HIP(call)                                                                        \
     do {                                                                                      \

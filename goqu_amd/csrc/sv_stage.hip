@@ -1186,7 +1186,7 @@ int upload_program(const Program &P0, int prec, hipStream_t s, DeviceProgram &dp
     if (dp.bytes < bytes) {
         if (dp.blob) GOQU_HIP(hipFree(dp.blob));
         dp.blob = nullptr;
-        GOQU_HIP(hipMalloc(&dp.blob, bytes));
+        GOQU_HIP(dmalloc((void **)&dp.blob, bytes));
         dp.bytes = bytes;
     }
     GOQU_HIP(hipMemcpyAsync(dp.blob, host.data(), bytes, hipMemcpyHostToDevice, s));
@@ -1285,7 +1285,7 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
     // per-stage partials (slot-major, one per workgroup), reduced after every stage into gsum
     double *gpart = nullptr, *gsum = nullptr;
     if (ng) {
-        GOQU_HIP(hipMalloc(&gpart, ((size_t)kMaxGradPerStage * grid + ng) * sizeof(double)));
+        GOQU_HIP(dmalloc((void **)&gpart, ((size_t)kMaxGradPerStage * grid + ng) * sizeof(double)));
         gsum = gpart + (size_t)kMaxGradPerStage * grid;
     }
     for (size_t i = 0; i < P.stages.size() && !rc; i++) {
@@ -1348,21 +1348,27 @@ int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s) {
 
 extern "C" {
 int goqu_profile_enable(int32_t on) {
+    GOQU_GUARD_BEGIN
     goqu::g_profile.store(on != 0);
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 int goqu_profile_reset(void) {
+    GOQU_GUARD_BEGIN
     std::lock_guard<std::mutex> lk(goqu::g_prof_mu);
     goqu::g_prof_ms = goqu::g_prof_bytes = 0;
     goqu::g_prof_launches = 0;
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 int goqu_profile_read(double *kernel_ms, int64_t *launches, double *bytes) {
+    GOQU_GUARD_BEGIN
     std::lock_guard<std::mutex> lk(goqu::g_prof_mu);
     if (kernel_ms) *kernel_ms = goqu::g_prof_ms;
     if (launches) *launches = goqu::g_prof_launches;
     if (bytes) *bytes = goqu::g_prof_bytes;
     return GOQU_OK;
+    GOQU_GUARD_END
 }
 }
 #endif  // GOQU_STAGE_KERNEL

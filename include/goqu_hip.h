@@ -182,6 +182,21 @@ int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits, const int3
                     const int64_t *param_offsets, const double *params, double *energies,
                     double *grads);
 
+/* ---- host-side planning and test hooks (no counterpart in the reference) ------------------ */
+/* Lowers and plans an op list on the host only (no device is touched): *stages receives the
+ * number of state sweeps (stage-kernel launches) and *instrs the instruction count.  density != 0
+ * plans the density-matrix lowering of num_qubits qubits.  tile_bits 0 = the default tile. */
+int goqu_plan_stats(int32_t num_qubits, int32_t density, const goqu_op_t *ops, int32_t num_ops,
+                    int32_t tile_bits, int64_t *stages, int64_t *instrs);
+/* Fault injection for tests of the error paths: kind 1 makes the countdown-th device
+ * allocation from now on fail (GOQU_ERR_OUT_OF_MEMORY), kind 2 makes the countdown-th op-list
+ * lowering throw std::bad_alloc inside the library (caught: GOQU_ERR_OUT_OF_MEMORY); kind 0 or
+ * countdown 0 disarms.  GOQU_FAULT_INJECT=device_alloc:<n> / host_alloc:<n> in the environment
+ * arms it when the library loads.  No C++ exception ever crosses this ABI. */
+#define GOQU_FAULT_DEVICE_ALLOC 1
+#define GOQU_FAULT_HOST_ALLOC 2
+int goqu_debug_inject_fault(int32_t kind, int32_t countdown);
+
 /* ---- profiling (no counterpart in the reference; for benchmarks) -------------------------- */
 /* When enabled, every stage-kernel launch is bracketed by HIP events on its own stream; the
  * totals accumulate process-wide until goqu_profile_reset. */

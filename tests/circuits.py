@@ -33,12 +33,17 @@ def rand_2q(rng):
             lambda: G.PSwap(a[0]), lambda: G.Controlled(G.H, 1)][k]()
 
 
-def random_circuit(n: int, depth: int, rng, three_q: bool = True, controlled: bool = True):
-    """Every layer: a random 1q gate per qubit and random multi-qubit gates on random qubits."""
+def random_circuit(n: int, depth: int, rng, three_q: bool = True, controlled: bool = True,
+                   resets: float = 0.0):
+    """Every layer: a random 1q gate per qubit and random multi-qubit gates on random qubits.
+    resets > 0: each qubit is also reset (the Evolve path's deterministic reset) with that
+    probability per layer, between its 1q gate and the multi-qubit gates."""
     b = NewBuilder("random", n)
     for _ in range(depth):
         for q in range(n):
             b.Apply(rand_1q(rng), q)
+            if resets and rng.random() < resets:
+                b.Reset(q)
         for _ in range(max(1, n // 2)):
             if n >= 2:
                 qs = rng.choice(n, 2, replace=False)

@@ -58,3 +58,53 @@ def test_no_device_is_an_error_not_an_abort():
 
     with pytest.raises(N.GoquError):
         statevector.New(3)
+
+
+def _small_buffer(n=5):
+    from goqu_amd.circuit import NewBuilder
+    from oracle import oracle as O
+
+    b = NewBuilder("p", n)
+    for q in range(n):
+        b.H(q).RZ(0.3 * q, q)
+    b.CNOT(0, n - 1).CZ(1, 2)
+    return O.sv_marshal(b.Build())
+
+
+def test_host_planning_needs_no_device():
+    stages, instrs = N.plan_stats(5, _small_buffer())
+    assert stages >= 1 and instrs >= 1
+    with pytest.raises(N.GoquError) as e:
+        N.plan_stats(40, _small_buffer())
+    assert e.value.code == N.ERR_INVALID_ARG
+
+
+def test_host_allocation_failure_is_a_status_code():
+    """A std::bad_alloc inside the library (forced by the fault-injection hook before the
+    lowering) comes back as GOQU_ERR_OUT_OF_MEMORY, not as an abort of the host process; the hook
+    fires once, so the next call succeeds."""
+    L = N.lib()
+    assert L.goqu_debug_inject_fault(N.FAULT_HOST_ALLOC, 1) == N.OK
+    with pytest.raises(N.GoquError) as e:
+        N.plan_stats(5, _small_buffer())
+    assert e.value.code == N.ERR_OOM
+    assert b"host memory" in L.goqu_last_error()
+    N.plan_stats(5, _small_buffer())
+    assert L.goqu_debug_inject_fault(7, 1) == N.ERR_INVALID_ARG
+    assert L.goqu_debug_inject_fault(0, 0) == N.OK
+
+
+def test_oversize_op_records_are_rejected():
+    """Records with too many qubits or qubits out of range are argument errors."""
+    from goqu_amd.circuit import NewBuilder
+    from oracle import oracle as O
+
+    buf = _small_buffer()
+    arr = buf.array()
+    arr[0].num_qubits = 17
+    st, ni = C.c_int64(0), C.c_int64(0)
+    assert N.lib().goqu_plan_stats(5, 0, arr, len(buf), 0, C.byref(st), C.byref(ni)) == N.ERR_INVALID_ARG
+    arr[0].num_qubits = 1
+    arr[0].qubits[0] = 9
+    assert N.lib().goqu_plan_stats(5, 0, arr, len(buf), 0, C.byref(st), C.byref(ni)) == N.ERR_INVALID_ARG
+    assert N.lib().goqu_plan_stats(5, 0, arr, len(buf), 21, C.byref(st), C.byref(ni)) == N.ERR_INVALID_ARG

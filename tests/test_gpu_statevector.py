@@ -190,3 +190,25 @@ def test_dynamic_teleportation_like():
     sim = statevector.New(2)
     counts = sim.Run(c, 20, rng=np.random.default_rng(2))
     assert counts == {"11": 20}
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_reset_known_answers_gpu(k):
+    """H0 H1 Reset1 S0 -> [1, i, 0, 0]/sqrt2 and H0 H1 RZ(0.7, 0) Reset1 -> [1, 1, 0, 0]/sqrt2: no
+    op moves across the (non-linear) reset."""
+    from test_planner_emulated import _reset_kat_circuits
+
+    c, want = _reset_kat_circuits()[k]
+    for prec, tol in [(N.C128, 1e-15), (N.C64, 1e-7)]:
+        sim = statevector.New(2, prec)
+        sim.Evolve(c)
+        assert np.max(np.abs(sim.StateVector() - want)) < tol
+        sim.Close()
+
+
+@pytest.mark.parametrize("n", [3, 7, 12])
+def test_random_circuits_with_resets(n):
+    rng = np.random.default_rng(4200 + n)
+    c = random_circuit(n, 5, rng, resets=0.2)
+    print(f"reset circuit n={n} ops={c.NumOps()}", flush=True)
+    _check(c)

@@ -221,3 +221,37 @@ def test_dm_parity_blocks_complex_and_fallback(n):
     assert np.max(np.abs(s[: 1 << (2 * n)] - want)) < 1e-12
     assert cov["parity2"] > 0 and cov["parity4"] > 0
     assert cov["parity2"] + cov["parity4"] > cov["parity_real"]  # complex blocks ran
+
+
+@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("tile,low", [(12, 3), (8, 2)])
+def test_resets_are_barriers(seed, tile, low):
+    """The Evolve path's reset a0 <- |(a0, a1)| is not linear, so no op on any qubit may be
+    fused, deferred or scheduled across it: random circuits with resets against the oracle."""
+    n = 4 + seed % 7
+    rng = np.random.default_rng(900 + seed)
+    c = random_circuit(n, 4, rng, resets=0.25)
+    s, nqi = _padded(n)
+    E.apply(s, n, O.sv_marshal(c), tile_bits=min(tile, nqi), low_bits=low)
+    assert np.max(np.abs(s[: 1 << n] - O.sv_evolve(c))) < 1e-12
+
+
+def _reset_kat_circuits():
+    """Two known answers: H0 H1 Reset1 S0 -> [1, i, 0, 0]/sqrt2 (the S gate after the reset
+    must act), and H0 H1 RZ(0.7, 0) Reset1 -> [1, 1, 0, 0]/sqrt2 (the RZ phases, including the
+    global one, are discarded by the reset's norm)."""
+    from goqu_amd.circuit import NewBuilder
+
+    a = NewBuilder("kat_s", 2).H(0).H(1).Reset(1).S(0).Build()
+    b = NewBuilder("kat_rz", 2).H(0).H(1).RZ(0.7, 0).Reset(1).Build()
+    r = 1 / np.sqrt(2)
+    return [(a, np.array([r, 1j * r, 0, 0])), (b, np.array([r, r, 0, 0]))]
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_reset_known_answers(k):
+    c, want = _reset_kat_circuits()[k]
+    assert np.max(np.abs(O.sv_evolve(c) - want)) < 1e-15
+    s, nqi = _padded(2)
+    E.apply(s, 2, O.sv_marshal(c), tile_bits=nqi, low_bits=2)
+    assert np.max(np.abs(s[:4] - want)) < 1e-15
