@@ -56,6 +56,7 @@ _SIGS = {
     "goqu_sv_get_state": ([_P, _D, C.c_uint64, C.c_uint64], C.c_int),
     "goqu_sv_set_state": ([_P, _D, C.c_uint64, C.c_uint64], C.c_int),
     "goqu_sv_sample": ([_P, _D, C.c_int64, _U64], C.c_int),
+    "goqu_sv_sample_noalloc": ([_P, _D, C.c_int64, _U64], C.c_int),
     "goqu_sv_expect_paulis": ([_P, C.c_int32, _U64, _U64, _I32, _D, _D], C.c_int),
     "goqu_sv_expectation_z": ([_P, C.c_uint64, _D], C.c_int),
     "goqu_sv_measure_qubit": ([_P, C.c_int32, C.c_double, _I32], C.c_int),

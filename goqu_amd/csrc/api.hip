@@ -218,6 +218,10 @@ int goqu_sv_create(int32_t num_qubits, int32_t precision, goqu_sv_t **out) {
     if (rc) return rc;
     goqu_sv *h = new goqu_sv();
     h->nq_user = num_qubits;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return fail(GOQU_ERR_DEVICE, "hipGetDevice failed");
+    }
     rc = alloc_state(h->st, std::max(num_qubits, kMinInternalQubits), precision);
     if (rc) {
         delete h;
@@ -333,6 +337,17 @@ int goqu_sv_sample(goqu_sv_t *h, const double *uniforms, int64_t shots, uint64_t
     GOQU_GUARD_BEGIN
     if (!h || shots < 0 || (shots > 0 && (!uniforms || !out_idx))) return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return sample_state(h->st, 1ull << h->nq_user, uniforms, shots, out_idx, h->scr, h->stream);
+    GOQU_GUARD_END
+}
+
+int goqu_sv_sample_noalloc(goqu_sv_t *h, const double *uniforms, int64_t shots, uint64_t *out_idx) {
+    GOQU_GUARD_BEGIN
+    if (!h || shots < 0 || (shots > 0 && (!uniforms || !out_idx))) return fail_cstr(GOQU_ERR_INVALID_ARG, "bad argument");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fail_cstr(GOQU_ERR_DEVICE, "hipGetDevice failed");
+    if (dev != h->device)
+        return fail_cstr(GOQU_ERR_INVALID_ARG, "sample without allocation: the handle's device is not the current device");
+    return sample_state_static(h->st, dev, 1ull << h->nq_user, uniforms, shots, out_idx, h->stream);
     GOQU_GUARD_END
 }
 

@@ -3,6 +3,9 @@
 //
 // All reductions are deterministic: each workgroup writes one partial per quantity, and a
 // single-workgroup finisher adds the partials in a fixed order.
+#include <cstdio>
+#include <mutex>
+
 #include "runtime.h"
 
 namespace goqu {
@@ -678,19 +681,31 @@ __global__ void __launch_bounds__(256) k_sample(const void *src, uint64_t n, con
     }
 }
 
+// Bytes of device buffer sample_into needs: chunk prefixes, staged uniforms, sampled indices.
+static size_t sample_buffer_bytes(uint64_t n, int64_t shots, size_t *ub_off, size_t *ob_off) {
+    const uint64_t nchunks = (n + kChunk - 1) / kChunk;
+    const size_t sb = (nchunks + 1) * sizeof(dd), ub = shots * sizeof(double), ob = shots * sizeof(uint64_t);
+    const size_t uoff = (sb + 255) & ~size_t(255), ooff = uoff + ((ub + 255) & ~size_t(255));
+    if (ub_off) *ub_off = uoff;
+    if (ob_off) *ob_off = ooff;
+    return ooff + ((ob + 255) & ~size_t(255));
+}
+
+// The three sampling kernels over a device buffer of sample_buffer_bytes(n, shots) bytes.  Every
+// byte the kernels read is written earlier in the same call (chunk sums by k_chunk_sums, the
+// total by k_scan_chunks, uniforms by the copy, indices by k_sample), so the buffer needs no
+// clearing between calls.
 template <bool FROM_AMPS, typename R>
-static int sample_impl(const void *src, uint64_t n, const double *u, int64_t shots, uint64_t *out, Scratch &scr,
+static int sample_into(const void *src, uint64_t n, const double *u, int64_t shots, uint64_t *out, char *buf,
                        hipStream_t s) {
     if (shots <= 0) return GOQU_OK;
     const uint64_t nchunks = (n + kChunk - 1) / kChunk;
-    const size_t sb = (nchunks + 1) * sizeof(dd), ub = shots * sizeof(double), ob = shots * sizeof(uint64_t);
-    int rc = scr.ensure(sb + ub + ob + 256);
-    if (rc) return rc;
-    char *b = static_cast<char *>(scr.ptr);
-    dd *sums = reinterpret_cast<dd *>(b);
-    double *du = reinterpret_cast<double *>(b + ((sb + 15) & ~size_t(15)));
-    uint64_t *dout = reinterpret_cast<uint64_t *>(reinterpret_cast<char *>(du) + ((ub + 15) & ~size_t(15)));
-    GOQU_HIP(hipMemcpyAsync(du, u, ub, hipMemcpyHostToDevice, s));
+    size_t uoff = 0, ooff = 0;
+    sample_buffer_bytes(n, shots, &uoff, &ooff);
+    dd *sums = reinterpret_cast<dd *>(buf);
+    double *du = reinterpret_cast<double *>(buf + uoff);
+    uint64_t *dout = reinterpret_cast<uint64_t *>(buf + ooff);
+    GOQU_HIP(hipMemcpyAsync(du, u, shots * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_chunk_sums<FROM_AMPS, R>), dim3((unsigned)nchunks), dim3(256), 0, s, src, n, sums);
     GOQU_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(256), 0, s, sums, nchunks);
@@ -698,9 +713,18 @@ static int sample_impl(const void *src, uint64_t n, const double *u, int64_t sho
     const unsigned grid = (unsigned)((shots + 3) / 4);
     hipLaunchKernelGGL((k_sample<FROM_AMPS, R>), dim3(grid), dim3(256), 0, s, src, n, sums, nchunks, du, shots, dout);
     GOQU_HIP(hipGetLastError());
-    GOQU_HIP(hipMemcpyAsync(out, dout, ob, hipMemcpyDeviceToHost, s));
+    GOQU_HIP(hipMemcpyAsync(out, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     GOQU_HIP(hipStreamSynchronize(s));
     return GOQU_OK;
+}
+
+template <bool FROM_AMPS, typename R>
+static int sample_impl(const void *src, uint64_t n, const double *u, int64_t shots, uint64_t *out, Scratch &scr,
+                       hipStream_t s) {
+    if (shots <= 0) return GOQU_OK;
+    int rc = scr.ensure(sample_buffer_bytes(n, shots, nullptr, nullptr));
+    if (rc) return rc;
+    return sample_into<FROM_AMPS, R>(src, n, u, shots, out, static_cast<char *>(scr.ptr), s);
 }
 
 int sample_state(State &st, uint64_t nvalid, const double *u, int64_t shots, uint64_t *out, Scratch &scr,
@@ -712,6 +736,49 @@ int sample_state(State &st, uint64_t nvalid, const double *u, int64_t shots, uin
 int sample_probs(const double *dprobs, uint64_t n, const double *u, int64_t shots, uint64_t *out, Scratch &scr,
                  hipStream_t s) {
     return sample_impl<false, double>(dprobs, n, u, shots, out, scr, s);
+}
+
+// ---- allocation-free sampling ------------------------------------------------------------------------
+// A file-scope device array stands in for the Scratch buffer of sample_state.  Each device holds its
+// own copy until the process ends; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (the array's name is not a device address in host
+// code).  Calls that overlap on one device share the array and race.
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+static std::mutex g_static_mu;
+static void *g_static_ptr[kMaxDevices];
+
+// this device's address of `scratch`, resolved on the first call on the device
+static int static_scratch(int dev, char **out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(GOQU_ERR_DEVICE, "device ordinal beyond the supported range");
+    std::lock_guard<std::mutex> lk(g_static_mu);
+    if (!g_static_ptr[dev]) {
+        void *p = nullptr;
+        GOQU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(GOQU_ERR_DEVICE, "hipGetSymbolAddress returned no address for the static scratch");
+        g_static_ptr[dev] = p;
+    }
+    *out = static_cast<char *>(g_static_ptr[dev]);
+    return GOQU_OK;
+}
+
+int sample_state_static(State &st, int dev, uint64_t nvalid, const double *u, int64_t shots, uint64_t *out,
+                        hipStream_t s) {
+    if (shots <= 0) return GOQU_OK;
+    const size_t need = sample_buffer_bytes(nvalid, shots, nullptr, nullptr);
+    if (need > kStaticScratchBytes) {
+        char msg[256];
+        std::snprintf(msg, sizeof(msg),
+                      "sample without allocation: %lld shots over 2^%d amplitudes need %zu bytes of scratch, "
+                      "the static scratch holds %zu",
+                      (long long)shots, 63 - __builtin_clzll(nvalid), need, kStaticScratchBytes);
+        return fail_cstr(GOQU_ERR_INVALID_ARG, msg);
+    }
+    char *buf = nullptr;
+    int rc = static_scratch(dev, &buf);
+    if (rc) return rc;
+    return st.prec == GOQU_C128 ? sample_into<true, double>(st.amps, nvalid, u, shots, out, buf, s)
+                                : sample_into<true, float>(st.amps, nvalid, u, shots, out, buf, s);
 }
 
 // ---- density matrix ---------------------------------------------------------------------------------

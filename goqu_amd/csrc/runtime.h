@@ -99,6 +99,11 @@ int sample_state(State &st, uint64_t nvalid, const double *u, int64_t shots, uin
                  hipStream_t s);
 int sample_probs(const double *dprobs, uint64_t n, const double *u, int64_t shots, uint64_t *out, Scratch &scr,
                  hipStream_t s);
+// sample_state through the device's static scratch array (no allocation); dev = the state's device
+constexpr size_t kStaticScratchBytes = size_t(8) << 20;
+constexpr int kMaxDevices = 64;
+int sample_state_static(State &st, int dev, uint64_t nvalid, const double *u, int64_t shots, uint64_t *out,
+                        hipStream_t s);
 // lam += coeff * P psi over a group of terms sharing x: lam[i] += sum_t c_t s_t(i^x) psi[i^x]
 int apply_pauli_group(const State &psi, State &lam, uint64_t x, const uint64_t *z, const double *c, int nt,
                       Scratch &scr, hipStream_t s);
@@ -120,6 +125,7 @@ int dm_fidelity(const State &rho, int n, const void *dpsi, double *out, Scratch 
 struct goqu_sv {
     goqu::State st;
     int nq_user = 0;
+    int device = 0;  // the device current when the handle was created
     hipStream_t stream = nullptr;
     goqu::Scratch scr;
 };

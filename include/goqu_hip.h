@@ -98,6 +98,14 @@ int goqu_sv_set_state(goqu_sv_t *h, const double *amps, uint64_t offset, uint64_
 /* Run's sampling (sim.go:46-57, 274-298): out_idx[s] = first i with u[s] < cum_i, where cum
  * is the FP64 prefix sum of |a_i|^2; falls back to the last index. */
 int goqu_sv_sample(goqu_sv_t *h, const double *uniforms, int64_t shots, uint64_t *out_idx);
+/* goqu_sv_sample without any device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result as goqu_sv_sample for the same uniforms.  Its buffers (chunk prefix
+ * sums, staged uniforms, indices) live in an 8 MiB static device array, one per device, whose
+ * address is looked up once per device; a call needing more than 8 MiB (16 * 2^n / 4096 +
+ * 16 * shots bytes, each part rounded up to 256) returns GOQU_ERR_INVALID_ARG.  The handle's
+ * device must be current.  Calls that overlap on one device share the array and race. */
+int goqu_sv_sample_noalloc(goqu_sv_t *h, const double *uniforms, int64_t shots, uint64_t *out_idx);
 /* pauli.Expect per term (sim/pauli/expect_sv.go:21-47): out[2t..2t+1] = coeff_t *
  * sum_i conj(psi[i^x_t]) (-1)^{|i&z_t|} i^{ny_t} psi[i].  Terms sharing an x-mask share a
  * sweep of the state. */
