@@ -164,6 +164,11 @@ struct alignas(16) StageHeader {
 static_assert(sizeof(StageHeader) == 144, "StageHeader layout");
 // StageHeader::flags
 constexpr int32_t SF_BIG = 1;
+// The stage stores through a different global mapping than it loads with (uncontrolled SWAPs
+// relabelled its local bits, or lane exchanges moved amplitudes between threads) and holds no LDS
+// remap: a thread may then store to a position another wave (or lane) has still to load, so the
+// kernel waits for every load of the tile (a workgroup barrier) before the first store.
+constexpr int32_t SF_SYNC_STORE = 2;
 
 // LDS element index swizzle of the stage kernel's remaps, for 8-byte elements (a complex64
 // amplitude, or one component of a complex128 amplitude: those move real parts and imaginary

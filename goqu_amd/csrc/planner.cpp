@@ -1340,6 +1340,11 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             H.gqN[i] = (uint8_t)qmap[cur.lay[i]];
         }
         H.ninstr = (int32_t)P.instrs.size() - H.instr_off;
+        {
+            bool remap = false;
+            for (int i = H.instr_off; i < (int)P.instrs.size(); i++) remap = remap || P.instrs[i].op == I_REMAP;
+            if (!remap && std::memcmp(H.gq0, H.gqN, (size_t)m) != 0) H.flags |= SF_SYNC_STORE;
+        }
         P.stages.push_back(H);
     }
     return P;

@@ -1045,6 +1045,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             lb = lbn;
             K = Kn;
         }
+        // stores through a permuted mapping may hit positions other waves still have to load
+        if (hdr->flags & SF_SYNC_STORE) __syncthreads();
         {
             __builtin_amdgcn_s_setprio(2);
             GAddr G;

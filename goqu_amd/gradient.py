@@ -64,15 +64,73 @@ def gate_derivative(name: str, args: Sequence[float], j: int) -> np.ndarray:
         if j == 1:
             return np.array([[0, 0], [1j * e(ph) * s, 1j * e(ph + la) * c]])
         return np.array([[0, -1j * e(la) * s], [0, 1j * e(ph + la) * c]])
-    # other gates: Richardson-extrapolated central difference (error ~1e-12)
-    def D(h):
-        a1, a0 = list(args), list(args)
-        a1[j] += h
-        a0[j] -= h
-        return (_mat(name, a1) - _mat(name, a0)) / (2 * h)
+    return _analytic_derivative(name, args, j)
 
-    h = 1e-3
-    return (4 * D(h / 2) - D(h)) / 3
+
+def _e(a: float) -> complex:
+    return complex(math.cos(a), math.sin(a))
+
+
+def _analytic_derivative(name: str, args: Sequence[float], j: int) -> np.ndarray:
+    """Entry-wise derivatives of the remaining constructors' matrices (gate.py, following
+    circuit/gate/parameterized.go:91-564): every entry is a product of cos / sin of a half angle
+    and a phase e^{i(k . args)}, differentiated in closed form."""
+    s2 = 1 / math.sqrt(2)
+    if name == "GPI":  # [[0, e(-p)], [e(p), 0]]
+        (p,) = args
+        return np.array([[0, -1j * _e(-p)], [1j * _e(p), 0]])
+    if name == "GPI2":  # s2 [[1, -i e(-p)], [-i e(p), 1]]
+        (p,) = args
+        return np.array([[0, -s2 * _e(-p)], [s2 * _e(p), 0]])
+    if name == "MS":  # off-diagonal -i s2 e(+-(p0 + p1)), -i s2 e(+-(p0 - p1))
+        p0, p1 = args
+        sg = 1.0 if j == 0 else -1.0  # d(p0 - p1)/d(arg j)
+        out = np.zeros((4, 4), dtype=complex)
+        out[0, 3] = -s2 * _e(-(p0 + p1))
+        out[3, 0] = s2 * _e(p0 + p1)
+        out[1, 2] = sg * s2 * _e(p0 - p1)
+        out[2, 1] = -sg * s2 * _e(-(p0 - p1))
+        return out
+    if name == "U2":  # s2 [[1, -e(l)], [e(p), e(p + l)]]
+        p, lam = args
+        if j == 0:
+            return np.array([[0, 0], [1j * s2 * _e(p), 1j * s2 * _e(p + lam)]])
+        return np.array([[0, -1j * s2 * _e(lam)], [0, 1j * s2 * _e(p + lam)]])
+    if name == "Rot":  # [[e(-(p+w)/2) c, -e(-(p-w)/2) s], [e((p-w)/2) s, e((p+w)/2) c]]
+        p, th, w = args
+        c, s = math.cos(th / 2), math.sin(th / 2)
+        if j == 1:
+            return np.array([[-_e(-(p + w) / 2) * s / 2, -_e(-(p - w) / 2) * c / 2],
+                             [_e((p - w) / 2) * c / 2, -_e((p + w) / 2) * s / 2]])
+        M = _mat(name, args)
+        k = np.array([[-0.5, -0.5], [0.5, 0.5]]) if j == 0 else np.array([[-0.5, 0.5], [-0.5, 0.5]])
+        return 1j * k * M  # each entry's phase exponent is k * arg
+    if name == "PhasedXZ":  # [[cx, i sx e(-pi a)], [i sx e(pi (z + a)), cx e(pi z)]]
+        x, z, a = args
+        cx, sx = math.cos(math.pi * x / 2), math.sin(math.pi * x / 2)
+        if j == 0:
+            h = math.pi / 2
+            return np.array([[-h * sx, 1j * h * cx * _e(-math.pi * a)],
+                             [1j * h * cx * _e(math.pi * (z + a)), -h * sx * _e(math.pi * z)]])
+        M = _mat(name, args)
+        k = np.array([[0, 0], [1, 1]]) if j == 1 else np.array([[0, -1], [1, 0]])
+        return 1j * math.pi * k * M
+    if name == "FSim":  # [[1,0,0,0],[0,ct,-i st,0],[0,-i st,ct,0],[0,0,0,e(-phi)]]
+        th, phi = args
+        out = np.zeros((4, 4), dtype=complex)
+        if j == 0:
+            ct, st = math.cos(th), math.sin(th)
+            out[1, 1] = out[2, 2] = -st
+            out[1, 2] = out[2, 1] = -1j * ct
+        else:
+            out[3, 3] = -1j * _e(-phi)
+        return out
+    if name == "PSwap":  # e(phi) on |01><10| and |10><01|
+        (phi,) = args
+        out = np.zeros((4, 4), dtype=complex)
+        out[1, 2] = out[2, 1] = 1j * _e(phi)
+        return out
+    raise ValueError(f"no derivative for parameterised gate {name}")
 
 
 # ---- reference combinators (algorithm/gradient/gradient.go) ------------------------------------

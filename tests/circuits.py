@@ -138,3 +138,104 @@ def efficient_su2(n: int, reps: int):
             if n > 1:
                 b.CNOT(n - 1, 0)
     return b.Build(), nparams
+
+
+def interleaved_product_circuit(n: int, layers: int, seed: int):
+    """Two independent config2_circuit halves on interleaved qubits: half A on the even qubits
+    (local q -> 2q), half B on the odd ones (q -> 2q + 1), ops alternating between the halves.
+    The gates span low and high qubits of the whole register, yet the final state is exactly
+    psi_A (x) psi_B, so every amplitude and every Z-string expectation follows from two small
+    oracle runs.  Returns (circuit, circuit_A, circuit_B)."""
+    na, nb = (n + 1) // 2, n // 2
+    ca, cb = config2_circuit(na, layers, seed), config2_circuit(nb, layers, seed + 1)
+    b = NewBuilder("c2product", n)
+    oa, ob = ca.Ops(), cb.Ops()
+    for i in range(max(len(oa), len(ob))):
+        if i < len(oa):
+            b.Apply(oa[i].Gate, *[2 * q for q in oa[i].Qubits])
+        if i < len(ob):
+            b.Apply(ob[i].Gate, *[2 * q + 1 for q in ob[i].Qubits])
+    return b.Build(), ca, cb
+
+
+def split_bits(idx: np.ndarray, parity: int, nbits: int) -> np.ndarray:
+    """The bits of idx at positions 2j + parity, packed (j = 0..nbits-1)."""
+    out = np.zeros_like(idx)
+    for j in range(nbits):
+        out |= ((idx >> np.uint64(2 * j + parity)) & np.uint64(1)) << np.uint64(j)
+    return out
+
+
+def random_z_masks(n: int, count: int, rng) -> list:
+    """`count` random non-empty Z masks over n qubits."""
+    out = []
+    while len(out) < count:
+        m = int(rng.integers(1, 1 << min(n, 62))) if n <= 62 else int(rng.integers(1, 1 << 62))
+        m &= (1 << n) - 1
+        if m:
+            out.append(m)
+    return out
+
+
+def qaoa_instances(num: int, seed: int, nmin: int = 10, nmax: int = 22):
+    """MaxCut instances: n ~ U{nmin..nmax}, G(n, 0.3) unit-weight edges (resampled if empty),
+    p ~ U{1..4}, gamma, beta ~ U[0, pi), params in FreeParameters order (g0, b0, g1, ...)."""
+    from goqu_amd import qaoa as Q
+
+    rng = np.random.default_rng(seed)
+    insts = []
+    for _ in range(num):
+        n = int(rng.integers(nmin, nmax + 1))
+        while True:
+            edges = [(i, j) for i in range(n) for j in range(i + 1, n) if rng.random() < 0.3]
+            if edges:
+                break
+        p = int(rng.integers(1, 5))
+        insts.append(Q.MaxCutInstance(n, edges, p, rng.uniform(0, math.pi, 2 * p)))
+    return insts
+
+
+def noisy_dm_workload(n: int = 14, layers: int = 20, seed: int = 9):
+    """Per layer U3 (symbolic angles) on every qubit and n/2 CNOTs on random disjoint pairs, then
+    a Delay on every qubit; noise: Depolarizing1Q(1e-3) / Depolarizing2Q(1e-2) defaults and
+    AmplitudeDamping(1e-3) on "delay"; H = sum_i Z_i.  Returns (circuit, noise model, H, names,
+    angles ~ U[0, 2pi))."""
+    from goqu_amd import noise as NZ
+    from goqu_amd.circuit import FreeParameters
+    from goqu_amd.pauli import NewPauliString, NewPauliSum, Z
+
+    rng = np.random.default_rng(seed)
+    th = NewVector("t", 3 * n * layers)
+    b = NewBuilder("dm", n)
+    k = 0
+    for _ in range(layers):
+        for q in range(n):
+            b.SymU3(th.At(k), th.At(k + 1), th.At(k + 2), q)
+            k += 3
+        perm = rng.permutation(n)
+        for i in range(0, n - 1, 2):
+            b.CNOT(int(perm[i]), int(perm[i + 1]))
+        for q in range(n):
+            b.Delay(q, 1.0)
+    c = b.Build()
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(1e-3))
+    nm.AddDefaultError(2, NZ.Depolarizing2Q(1e-2))
+    nm.AddGateError("delay", NZ.AmplitudeDamping(1e-3))
+    H = NewPauliSum([NewPauliString(1.0, {q: Z}, n) for q in range(n)])
+    names = FreeParameters(c)
+    x = rng.uniform(0, 2 * np.pi, len(names))
+    return c, nm, H, names, x
+
+
+def random_pauli_sum(n: int, terms: int, rng, wmax: int = 4):
+    """`terms` random Pauli strings, weight ~ U{1..wmax} on distinct qubits, letters ~ U{X,Y,Z},
+    coefficients ~ N(0, 1)."""
+    from goqu_amd.pauli import NewPauliString, NewPauliSum, X, Y, Z
+
+    out = []
+    for _ in range(terms):
+        w = int(rng.integers(1, min(wmax, n) + 1))
+        ops = {int(q): [X, Y, Z][rng.integers(3)] for q in rng.choice(n, w, replace=False)}
+        out.append(NewPauliString(rng.normal(), ops, n))
+    return NewPauliSum(out)

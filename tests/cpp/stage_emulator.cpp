@@ -133,6 +133,10 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
     if (st2 && vals) vals->assign(P.num_grad, 0.0);
     const uint64_t N = 1ull << nq;
     std::vector<char> visited(N);
+    // the thread that loaded each position: without a barrier between a tile's loads and its
+    // stores (no LDS remap in the stage and no SF_SYNC_STORE), a thread may store only positions
+    // it loaded itself, or it could overwrite data another wave has still to load
+    std::vector<uint32_t> loader(N);
     for (const StageHeader &H : P.stages) {
         const int m = H.m;
         const int nthr = 1 << (m - 4);
@@ -172,6 +176,8 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                 return 1;
             }
         std::fill(visited.begin(), visited.end(), 0);
+        bool barrier = (H.flags & SF_SYNC_STORE) != 0;
+        for (int pc = 0; pc < H.ninstr; pc++) barrier = barrier || P.instrs[H.instr_off + pc].op == I_REMAP;
         const uint64_t ntiles = 1ull << (nq - m);
         std::vector<double> lds(size_t(1) << m);  // 8-byte elements
         std::vector<char> seen(size_t(1) << m);
@@ -199,6 +205,7 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                         return 1;
                     }
                     visited[g] = 1;
+                    loader[g] = (uint32_t)t;
                     arr[0][t * 16 + r] = st[g];
                     if (st2) arr[1][t * 16 + r] = st2[g];
                 }
@@ -483,6 +490,10 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                     const uint64_t g = G.tg | rg(G, r);
                     if (g >= N || visited[g] != 1) {
                         err = "store index out of range or not loaded by this tile";
+                        return 1;
+                    }
+                    if (!barrier && loader[g] != (uint32_t)t) {
+                        err = "a thread stores a position another thread loaded, with no barrier in between";
                         return 1;
                     }
                     visited[g] = 2;

@@ -163,3 +163,51 @@ def test_dm_complex_parity_channels_and_dense_fallback(n):
     want = O.dm_zero(n)
     assert O.dm_apply_ops(want, n, D.marshal(c, nm)) == 0
     assert np.max(np.abs(got - want)) <= TOL
+
+
+def test_dm_reset_known_answers_gpu():
+    """densitymatrix/sim_test.go:550-620 (TestEvolve_Reset) on the device."""
+    zero = np.zeros(4)
+    zero[0] = 1
+    for c in [NewBuilder("xreset", 1).X(0).Reset(0).Build(), NewBuilder("hreset", 1).H(0).Reset(0).Build()]:
+        dm = D.New(1)
+        dm.Evolve(c)
+        assert np.max(np.abs(dm.DensityMatrix() - zero)) < 1e-10
+    dm = D.New(2)
+    dm.Evolve(NewBuilder("bellreset", 2).H(0).CNOT(0, 1).Reset(0).Build())
+    assert np.max(np.abs(dm.DensityMatrix().reshape(4, 4) - np.diag([0.5, 0, 0.5, 0]))) < 1e-10
+
+
+def test_dm_expectation_value_closed_forms_gpu():
+    p = 0.3
+    rho = np.zeros((4, 4), dtype=np.complex128)
+    rho[0, 0], rho[3, 3] = p, 1 - p
+    rho[0, 3] = rho[3, 0] = 0.2
+    dm = D.New(2)
+    dm.SetDensityMatrix(rho.reshape(-1))
+    assert dm.ExpectationValue([0, 1]) == 1.0
+    assert abs(dm.ExpectationValue([0]) - (2 * p - 1)) < 1e-14
+    nm = NZ.New()
+    nm.AddGateError("I", NZ.Depolarizing1Q(0.3))
+    dm = D.New(1).WithNoise(nm)
+    dm.Evolve(NewBuilder("dep", 1).Apply(G.I, 0).Build())
+    assert dm.ExpectationValue([0]) == round((1 - 0.4) * 1e14) / 1e14
+
+
+@pytest.mark.parametrize("n", [3, 9])
+def test_dm_sampled_indices_match_oracle(n):
+    """DM Run's sampling (measure.go:93-103) on the device against the oracle's inverse CDF over
+    the same readout-corrected probabilities and the same uniforms: identical indices."""
+    rng = np.random.default_rng(700 + n)
+    c = random_circuit(n, 3, rng, three_q=False, controlled=False)
+    nm = noise_model()
+    dm = D.New(n).WithNoise(nm)
+    dm.Evolve(c)
+    probs = dm.Probabilities()
+    u = rng.random(5000)
+    out = np.empty(len(u), dtype=np.uint64)
+    from goqu_amd import _native as N
+
+    N.check(N.lib().goqu_dm_sample(dm._h, N.dptr(probs), N.dptr(u), len(u), N.u64ptr(out)))
+    assert np.array_equal(out, O.sample_probs(probs, u))
+    dm.Close()

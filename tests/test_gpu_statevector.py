@@ -212,3 +212,35 @@ def test_random_circuits_with_resets(n):
     c = random_circuit(n, 5, rng, resets=0.2)
     print(f"reset circuit n={n} ops={c.NumOps()}", flush=True)
     _check(c)
+
+
+def test_dynamic_reference_kats_gpu():
+    """The reference's dynamic-circuit tests (dynamic_test.go:14-162) on the device path:
+    conditional, conditional not fired, reset and reuse, 3-qubit teleportation."""
+    from test_oracle_dynamic import teleport_circuit
+
+    c = NewBuilder("cond", 2).WithClbits(1).X(0).Measure(0, 0).If(0, 1, G.X, 1).Build()
+    assert statevector.New(2).Run(c, 200, rng=np.random.default_rng(1)) == {"1": 200}
+    c = NewBuilder("cond_no", 2).WithClbits(1).Measure(0, 0).If(0, 1, G.X, 1).Build()
+    assert statevector.New(2).Run(c, 200, rng=np.random.default_rng(2)) == {"0": 200}
+    c = NewBuilder("reset_reuse", 1).WithClbits(2).H(0).Measure(0, 0).Reset(0).H(0).Measure(0, 1).Build()
+    counts = statevector.New(1).Run(c, 2000, rng=np.random.default_rng(3))
+    assert abs(sum(v for k, v in counts.items() if k[0] == "1") / 2000 - 0.5) < 0.05
+    counts = statevector.New(3).Run(teleport_circuit(), 1000, rng=np.random.default_rng(4))
+    assert all(len(k) == 3 and k[0] == "1" for k in counts) and len(counts) == 4
+
+
+def test_measure_qubit_closed_forms_gpu():
+    """measureQubit on product states: outcome rule r >= P0, collapse and renormalisation exact."""
+    from test_oracle_dynamic import _product
+
+    t, a, b = 0.3, 0.6, 0.8j
+    p0 = math.cos(t) ** 2
+    for prec, tol in [(N.C128, 1e-15), (N.C64, 1e-6)]:  # complex64: a few fp32 ulps
+        for r, outcome in [(0.0, 0), (p0 * 0.999, 0), (p0, 1), (0.999999, 1)]:
+            sim = statevector.New(2, prec)
+            sim.SetStateVector(_product(t, 0.7, a, b))
+            assert sim.MeasureQubit(1, r) == outcome
+            want = np.kron([1, 0], [a, b]) if outcome == 0 else np.kron([0, np.exp(0.7j)], [a, b])
+            assert np.max(np.abs(sim.StateVector() - want)) < tol
+            sim.Close()

@@ -171,3 +171,30 @@ def test_oracle_dm_channels_bruteforce():
     rho = O.dm_zero(n)
     assert O.dm_apply_ops(rho, n, D.marshal(c, nm)) == 0
     assert np.max(np.abs(rho - rho_ref.reshape(-1))) < 1e-12
+
+
+def test_gate_derivatives_are_analytic():
+    """gradient.gate_derivative for every parameterised constructor against 8th-order central
+    differences of the constructor's own matrix (truncation ~h^8, about 1e-13 here): the adjoint
+    consumes these matrices, so a wrong sign or factor in one entry fails here."""
+    from goqu_amd import gradient as GR
+    from goqu_amd.circuit import CONSTRUCTORS
+
+    rng = np.random.default_rng(17)
+    nargs = {"U3": 3, "Rot": 3, "PhasedXZ": 3, "FSim": 2, "MS": 2, "U2": 2}
+    w = np.array([1 / 280, -4 / 105, 1 / 5, -4 / 5, 0, 4 / 5, -1 / 5, 4 / 105, -1 / 280])
+    h = 2e-2
+    for name, ctor in CONSTRUCTORS.items():
+        k = nargs.get(name, 1)
+        for _ in range(3):
+            args = list(rng.uniform(-3, 3, k))
+            for j in range(k):
+                def mat(t):
+                    a = list(args)
+                    a[j] = t
+                    g = ctor(*a)
+                    d = 1 << g.Qubits()
+                    return g.Matrix().reshape(d, d)
+                fd = sum(wi * mat(args[j] + (i - 4) * h) for i, wi in enumerate(w) if wi) / h
+                got = GR.gate_derivative(name, args, j)
+                assert np.max(np.abs(got - fd)) < 1e-11, (name, j, np.max(np.abs(got - fd)))
