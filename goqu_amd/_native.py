@@ -53,6 +53,7 @@ _SIGS = {
     "goqu_sv_destroy": ([_P], C.c_int),
     "goqu_sv_reset": ([_P], C.c_int),
     "goqu_sv_apply": ([_P, C.POINTER(GoquOp), C.c_int32], C.c_int),
+    "goqu_sv_set_plan": ([_P, C.c_int32, C.c_int32], C.c_int),
     "goqu_sv_get_state": ([_P, _D, C.c_uint64, C.c_uint64], C.c_int),
     "goqu_sv_set_state": ([_P, _D, C.c_uint64, C.c_uint64], C.c_int),
     "goqu_sv_sample": ([_P, _D, C.c_int64, _U64], C.c_int),

@@ -274,7 +274,23 @@ int goqu_sv_apply(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops) {
     std::string err;
     int rc = lower_sv_ops(ops, num_ops, h->nq_user, eops, err);
     if (rc) return fail(rc, err);
-    return run_eops(h->st, eops, h->stream);
+    return run_eops(h->st, eops, h->stream, h->plan_set ? &h->plan : nullptr);
+    GOQU_GUARD_END
+}
+
+int goqu_sv_set_plan(goqu_sv_t *h, int32_t tile_bits, int32_t low_bits) {
+    GOQU_GUARD_BEGIN
+    if (!h) return fail_cstr(GOQU_ERR_INVALID_ARG, "null handle");
+    if (tile_bits == 0 && low_bits == 0) {
+        h->plan_set = false;
+        return GOQU_OK;
+    }
+    if (tile_bits < 8 || tile_bits > kMaxTileBits || low_bits < 2 || low_bits > 6 || low_bits > tile_bits - kRegBits)
+        return fail_cstr(GOQU_ERR_INVALID_ARG, "tile_bits must be 8..13 and low_bits 2..min(6, tile_bits - 4)");
+    h->plan.tile_bits = tile_bits;
+    h->plan.low_bits = low_bits;
+    h->plan_set = true;
+    return GOQU_OK;
     GOQU_GUARD_END
 }
 

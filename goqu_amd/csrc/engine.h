@@ -89,13 +89,33 @@ enum InstrOp : int32_t {
                   // the resulting layout and its LDS offsets, as for I_REMAP
 };
 
-// Instr::flags
+// Instr::flags, bits 0..7
 constexpr int32_t F_PURE = 1;   // ANTI1 / SWAP2 with unit phases: a move
 constexpr int32_t F_TCTRL = 2;  // controls on thread-id bits (tmask)
 constexpr int32_t F_PAT = 4;    // controls on register slots (pat)
 constexpr int32_t F_TILE = 8;   // controls on qubits outside the tile (omask), uniform per tile
 constexpr int32_t F_OUTER = 16; // run by the kernel outside its inner op loop (REMAP, XSWAP, RESET, dense on 3-4 slots)
 constexpr int32_t F_REAL = 32;  // I_PARITY, I_DENSE: every matrix entry is real (only real parts are read)
+
+// Instr::flags, bits 8..15: the kernel code path of the instruction (assign_codes), so that the
+// stage kernel dispatches every op with one switch on a small integer.  Ranges:
+enum KCode : int32_t {
+    K_SHEAR = 0,     // 8: 2 * slot + (three shears)
+    K_DIAG_FULL = 8, // uniform phase table, registers 1..15 all nontrivial
+    K_DIAG_MASK = 9, // uniform phase table, the registers of `pat`
+    K_DIAG_TILE = 10,  // phase table selected by bits of the tile base (uniform per workgroup)
+    K_DIAG_LANE = 11,  // phase table indexed by thread-id bits, or under thread-bit controls
+    K_ANTI1 = 12,    // 16: 4 * slot + 2 * pure + thread controls
+    K_SWAP2 = 28,    // 24: 4 * slot pair (01 02 03 12 13 23) + 2 * pure + thread controls
+    K_DENSE1 = 52,   // 12: 3 * slot + (0 controlled, 1 real, 2 complex)
+    K_DENSE2 = 64,   // 12: 2 * slot pair + (0 controlled, 1 complex)
+    K_PARITY2 = 76,  // 12: 2 * slot pair + (0 real, 1 complex)
+    K_PARITY4 = 88,  // 6: 2 * (partner of slot 0 - 1) + (0 real, 1 complex)
+    K_GRAD = 94,     // 10: index of the target slot mask in {1,2,3,4,5,6,8,9,10,12}
+    K_OUTER = 255,   // F_OUTER instructions (the kernel's outer loop)
+};
+constexpr int kCodeShift = 8;
+inline int instr_code(int32_t flags) { return (flags >> kCodeShift) & 255; }
 
 // I_PARITY canonical form (the kernel's view).  k = 2: slots lo < hi (smask); class
 // p = bit lo ^ bit hi; element i of class p has bit lo = i, bit hi = i ^ p; the pool holds
@@ -142,8 +162,9 @@ inline double shear_coef(const Instr &I, int i) {
     return c;
 }
 
-// DIAG phase table: entry (lane_index << popcount(smask)) | v, where v holds the values of the
-// smask register slots (ascending slot order, slot bit -> v bit) and lane_index the ls[] bits.
+// DIAG phase table: 16 entries per lane index, entry (lane_index << 4) | r for register index r
+// (lane_index: the ls[] bits, MSB first); pat bit r = 1 iff some entry of register r is != 1.
+// smask keeps the register slots the op's qubits and controls occupy (informative only).
 
 struct alignas(16) StageHeader {
     int32_t m;            // tile bits
@@ -164,6 +185,8 @@ struct alignas(16) StageHeader {
 static_assert(sizeof(StageHeader) == 144, "StageHeader layout");
 // StageHeader::flags
 constexpr int32_t SF_BIG = 1;
+// the stage holds density-matrix channel ops (I_PARITY): run by the density-matrix kernel
+constexpr int32_t SF_DM = 4;
 // The stage stores through a different global mapping than it loads with (uncontrolled SWAPs
 // relabelled its local bits, or lane exchanges moved amplitudes between threads) and holds no LDS
 // remap: a thread may then store to a position another wave (or lane) has still to load, so the
@@ -191,7 +214,7 @@ struct Program {
 
 struct PlanConfig {
     int tile_bits = 12;   // m
-    int low_bits = 3;     // L: lowest qubits in every tile (contiguous runs of 2^L amps)
+    int low_bits = 4;     // L: lowest qubits in every tile (contiguous runs of 2^L amps)
 };
 
 // Lowering of goqu_op_t records to elementary ops (statevector semantics).  Returns a
@@ -207,6 +230,8 @@ int lower_dm_ops(const void *ops, int nops, int n, std::vector<EOp> &out, std::s
 Program plan_program(const std::vector<EOp> &eops, int nq_internal, const PlanConfig &cfg);
 // Fills Instr::kr and StageHeader::k0: LDS byte offsets of 8-byte elements.
 void set_lds_offsets(Program &P);
+// The kernel code path of every instruction (flags bits 8..15) and the stage kernel flags.
+void assign_codes(Program &P);
 
 // ---- adjoint differentiation programs (adjoint_plan.cpp) ----------------------------------
 // conjugate transpose of a unitary EOp (RESET has none)

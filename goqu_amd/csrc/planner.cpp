@@ -858,24 +858,21 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                     if (I.omask) I.flags |= F_TILE;
                     I.k = kl;
                     I.smask = rm;
-                    const int kr = __builtin_popcount(rm);
+                    // 16 entries per lane index, one per register index r
                     uint32_t nontriv = 0;
-                    std::vector<double> tab(2 << (kl + kr));
+                    std::vector<double> tab((size_t)2 * 16 << kl);
                     for (int li = 0; li < (1 << kl); li++)
-                        for (int v = 0; v < (1 << kr); v++) {
-                            int sval[4] = {0, 0, 0, 0};
-                            for (int sl = 0, t = 0; sl < kRegBits; sl++)
-                                if (rm >> sl & 1) sval[sl] = (v >> t++) & 1;
+                        for (int r = 0; r < kRegAmps; r++) {
                             bool ok = true;
                             for (int sl = 0; sl < kRegBits; sl++)
-                                if ((ctl_slots >> sl & 1) && !sval[sl]) ok = false;
+                                if ((ctl_slots >> sl & 1) && !(r >> sl & 1)) ok = false;
                             int idx = 0;
                             for (int j = 0; j < e.k; j++) {
                                 const int q = e.q[j];
                                 const int sl = loc[q] >= 0 ? slot_of(loc[q]) : -1;
                                 int bit = 0;
                                 if (sl >= 0) {
-                                    bit = sval[sl];
+                                    bit = (r >> sl) & 1;
                                 } else {
                                     for (int p = 0; p < kl; p++)
                                         if (lane_j[p] == j) bit = (li >> (kl - 1 - p)) & 1;
@@ -883,11 +880,41 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                                 idx = (idx << 1) | bit;
                             }
                             const double re = ok ? e.m[2 * idx] : 1.0, im = ok ? e.m[2 * idx + 1] : 0.0;
-                            const int at = (li << kr) | v;
+                            const int at = (li << kRegBits) | r;
                             tab[2 * at] = re;
                             tab[2 * at + 1] = im;
-                            if (re != 1.0 || im != 0.0) nontriv |= 1u << v;
+                            if (re != 1.0 || im != 0.0) nontriv |= 1u << r;
                         }
+                    // Lane index bits that only gate the table (every entry with the bit clear is 1)
+                    // become controls: thread bits join tmask, bits of the tile base join omask.
+                    // Controlled phases between lane qubits (CZ, CP) then need no per-lane table.
+                    for (int p = kl - 1; p >= 0; p--) {
+                        const int bit = kl - 1 - p;  // ls[0] is the most significant index bit
+                        bool gate = true;
+                        for (int li = 0; li < (1 << kl) && gate; li++)
+                            if (!(li >> bit & 1))
+                                for (int r = 0; r < kRegAmps && gate; r++)
+                                    gate = tab[2 * ((li << kRegBits) | r)] == 1.0 && tab[2 * ((li << kRegBits) | r) + 1] == 0.0;
+                        if (!gate) continue;
+                        std::vector<double> kept;
+                        for (int li = 0; li < (1 << kl); li++)
+                            if (li >> bit & 1)
+                                kept.insert(kept.end(), tab.begin() + 2 * (li << kRegBits), tab.begin() + 2 * ((li + 1) << kRegBits));
+                        tab.swap(kept);
+                        const int src = I.ls[p];
+                        if (src < 16) I.tmask |= 1u << src;
+                        else I.omask |= 1ull << (src - 16);
+                        for (int j = p; j + 1 < kl; j++) I.ls[j] = I.ls[j + 1];
+                        I.ls[--kl] = 0;
+                    }
+                    if (I.tmask) I.flags |= F_TCTRL;
+                    if (I.omask) I.flags |= F_TILE;
+                    I.k = kl;
+                    nontriv = 0;
+                    for (int li = 0; li < (1 << kl); li++)
+                        for (int r = 0; r < kRegAmps; r++)
+                            if (tab[2 * ((li << kRegBits) | r)] != 1.0 || tab[2 * ((li << kRegBits) | r) + 1] != 0.0)
+                                nontriv |= 1u << r;
                     I.pat = nontriv;
                     P.pool.insert(P.pool.end(), tab.begin(), tab.end());
                     break;
@@ -1347,7 +1374,61 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
         }
         P.stages.push_back(H);
     }
+    assign_codes(P);
     return P;
+}
+
+void assign_codes(Program &P) {
+    auto pair_index = [](int a, int b) {  // slot pairs 01 02 03 12 13 23
+        static const int tab[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+        return tab[a][b];
+    };
+    static const int grad_masks[10] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12};
+    for (StageHeader &H : P.stages) {
+        for (int i = 0; i < H.ninstr; i++) {
+            Instr &I = P.instrs[H.instr_off + i];
+            int code = K_OUTER;
+            const bool tc = I.flags & F_TCTRL, pure = I.flags & F_PURE, real = I.flags & F_REAL;
+            if (!(I.flags & F_OUTER)) {
+                switch (I.op) {
+                case I_SHEAR: code = K_SHEAR + 2 * I.s[0] + (I.k == 3); break;
+                case I_DIAG: {
+                    bool tile_bits = !tc;
+                    for (int j = 0; j < I.k; j++) tile_bits = tile_bits && I.ls[j] >= 16;
+                    if (I.k == 0 && !tc) code = (I.pat | 1u) == 0xFFFFu ? K_DIAG_FULL : K_DIAG_MASK;
+                    else code = tile_bits ? K_DIAG_TILE : K_DIAG_LANE;
+                    break;
+                }
+                case I_ANTI1: code = K_ANTI1 + 4 * I.s[0] + 2 * pure + tc; break;
+                case I_SWAP2: code = K_SWAP2 + 4 * pair_index(I.s[0], I.s[1]) + 2 * pure + tc; break;
+                case I_DENSE:
+                    if (__builtin_popcount(I.smask) == 1) {
+                        const int v = (I.flags & (F_TCTRL | F_PAT)) ? 0 : real ? 1 : 2;
+                        code = K_DENSE1 + 3 * __builtin_ctz(I.smask) + v;
+                    } else {
+                        const int a = __builtin_ctz(I.smask), b = 31 - __builtin_clz(I.smask);
+                        code = K_DENSE2 + 2 * pair_index(a, b) + ((I.flags & (F_TCTRL | F_PAT)) ? 0 : 1);
+                    }
+                    break;
+                case I_PARITY:
+                    if (I.k == 2) {
+                        const int a = __builtin_ctz(I.smask), b = 31 - __builtin_clz(I.smask);
+                        code = K_PARITY2 + 2 * pair_index(a, b) + !real;
+                    } else {
+                        code = K_PARITY4 + 2 * (I.s[1] - 1) + !real;
+                    }
+                    H.flags |= SF_DM;
+                    break;
+                case I_GRAD:
+                    for (int g = 0; g < 10; g++)
+                        if (grad_masks[g] == I.smask) code = K_GRAD + g;
+                    break;
+                default: break;
+                }
+            }
+            I.flags = (I.flags & ((1 << kCodeShift) - 1)) | (code << kCodeShift);
+        }
+    }
 }
 
 void set_lds_offsets(Program &P) {

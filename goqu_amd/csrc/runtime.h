@@ -79,7 +79,8 @@ struct Scratch {
 PlanConfig default_plan_config(int prec, int nq, bool paired);
 int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp);
 int run_program(State &st, const DeviceProgram &dp, hipStream_t s);
-int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s);  // plan + upload + run
+// plan (with cfg, or the default configuration) + upload + run
+int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg = nullptr);
 // ops applied to psi and lam together; EOP_GRAD ops accumulate Re <lam| D |psi> per slot
 int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, std::vector<double> &slot_vals,
              hipStream_t s);
@@ -126,6 +127,8 @@ struct goqu_sv {
     goqu::State st;
     int nq_user = 0;
     int device = 0;  // the device current when the handle was created
+    bool plan_set = false;  // goqu_sv_set_plan: tile shape of this handle's Apply programs
+    goqu::PlanConfig plan;
     hipStream_t stream = nullptr;
     goqu::Scratch scr;
 };

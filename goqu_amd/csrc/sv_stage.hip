@@ -543,8 +543,6 @@ __device__ __forceinline__ void op_reset(cx<R> (&a)[16]) {
     });
 }
 
-// Diagonal op: phase of register index r = table[(lane_index << popc(RM)) | pext(r, RM)], one
-// for lanes whose thread-bit controls fail.  Table columns without an entry != 1 are skipped.
 // N consecutive complex table entries at a uniform address, fetched as one scalar load
 template <typename R, int N>
 __device__ __forceinline__ void load_cx_batch(const cx<R> *__restrict__ p, cx<R> (&out)[N]) {
@@ -566,48 +564,45 @@ __device__ __forceinline__ void load_cx_batch(const cx<R> *__restrict__ p, cx<R>
     });
 }
 
-template <int RM, bool PAIR, typename R>
+// Diagonal op: register index r is multiplied by table[(lane_index << 4) | r] where bit r of
+// `pat` is set (K_DIAG_*): uniform tables go four entries per scalar load; K_DIAG_LANE selects
+// entries by lane bits (thread-id bits or bits of the tile base) and keeps lanes whose thread-bit
+// controls fail unchanged.
+template <bool PAIR, typename R>
 __device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const IHead &I, const R *__restrict__ pool,
-                                        uint32_t tid, uint64_t base, bool lok) {
-    constexpr int KR = popc4(RM);
-    constexpr int NV = 1 << KR;
-    constexpr int CH = NV < 4 ? NV : 4;  // table columns per scalar load
+                                        uint32_t tid, uint64_t base, bool lok, int code) {
     const cx<R> *__restrict__ tab = reinterpret_cast<const cx<R> *>(pool) + I.mat;
     const uint32_t nontriv = I.pat;
-    const int kl = I.k;
-    const bool tc = I.flags & F_TCTRL;
-    // multiplies the amplitudes of table column v in place (one uniform branch per column)
-    auto column = [&](auto vc, auto uc, cx<R> ph) __attribute__((always_inline)) {
-        constexpr int v = decltype(vc)::value;
+    if (code == K_DIAG_TILE) {  // the table of this tile: index bits of the (uniform) tile base
+        const uint32_t lsw = I.lsw;
+        uint32_t li = 0;
+        for (int j = 0; j < I.k; j++) li = (li << 1) | (uint32_t)((base >> (((lsw >> (8 * j)) & 255u) - 16)) & 1u);
+        tab += __builtin_amdgcn_readfirstlane(li) << 4;
+    }
+    auto mul = [&](auto rc, auto uc, cx<R> ph) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
         constexpr bool U = decltype(uc)::value;
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            if constexpr (pext4(r, RM) == v) {
-                cmul_inplace<U>(a[r].x, a[r].y, ph.x, ph.y);
-                if constexpr (PAIR) cmul_inplace<U>(b[r].x, b[r].y, ph.x, ph.y);
+        cmul_inplace<U>(a[r].x, a[r].y, ph.x, ph.y);
+        if constexpr (PAIR) cmul_inplace<U>(b[r].x, b[r].y, ph.x, ph.y);
+    };
+    if (code != K_DIAG_LANE) {  // a uniform table: four entries per scalar load
+        const bool full = code == K_DIAG_FULL;  // registers 1..15 all nontrivial: no tests
+        static_for<0, 4>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int c0 = 4 * decltype(cc)::value;
+            if (full || ((nontriv >> c0) & 15u)) {
+                cx<R> ph[4];
+                load_cx_batch<R, 4>(tab + c0, ph);
+                static_for<0, 4>([&](auto jc) __attribute__((always_inline)) {
+                    constexpr int r = c0 + decltype(jc)::value;
+                    if ((full && r > 0) || ((nontriv >> r) & 1u))
+                        mul(std::integral_constant<int, r>{}, std::true_type{}, ph[decltype(jc)::value]);
+                });
             }
         });
-    };
-    if (kl == 0 && !tc) {  // every lane uses the same phases
-        // phase tables usually have every column but the first nontrivial: no per-column tests
-        auto run = [&](auto densec) __attribute__((always_inline)) {
-            constexpr bool DC = decltype(densec)::value;
-            static_for<0, NV / CH>([&](auto cc) __attribute__((always_inline)) {
-                constexpr int c0 = CH * decltype(cc)::value;
-                cx<R> ph[CH];
-                load_cx_batch<R, CH>(tab + c0, ph);
-                static_for<0, CH>([&](auto jc) __attribute__((always_inline)) {
-                    constexpr int j = decltype(jc)::value;
-                    if ((DC && c0 + j > 0) || ((nontriv >> (c0 + j)) & 1u))
-                        column(std::integral_constant<int, c0 + j>{}, std::integral_constant<bool, true>{}, ph[j]);
-                });
-            });
-        };
-        if ((nontriv | 1u) == (1u << NV) - 1u) run(std::true_type{});
-        else run(std::false_type{});
         return;
     }
-    // lane index from thread-id bits and (uniform) bits of the tile base
+    const int kl = I.k;
+    const bool tc = I.flags & F_TCTRL;
     const uint32_t lsw = I.lsw;
     uint32_t li = 0;
     for (int j = 0; j < kl; j++) {
@@ -615,24 +610,24 @@ __device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const IH
         const uint32_t bit = src < 16 ? ((tid >> src) & 1u) : (uint32_t)((base >> (src - 16)) & 1u);
         li = (li << 1) | bit;
     }
-    static_for<0, NV>([&](auto vc) __attribute__((always_inline)) {
-        constexpr int v = decltype(vc)::value;
-        if ((nontriv >> v) & 1u) {
+    static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+        constexpr int r = decltype(rc)::value;
+        if ((nontriv >> r) & 1u) {
             cx<R> t;
             if (kl == 0) {
-                t = tab[v];
+                t = tab[r];
             } else if (kl == 1) {  // two uniform candidates
-                const cx<R> t0 = tab[v], t1 = tab[NV + v];
+                const cx<R> t0 = tab[r], t1 = tab[16 + r];
                 t.x = li ? t1.x : t0.x;
                 t.y = li ? t1.y : t0.y;
             } else {
-                t = tab[(li << KR) | v];  // per-lane load from a small, cached table
+                t = tab[(li << 4) | r];  // per-lane load from a small, cached table
             }
             if (tc) {
                 t.x = lok ? t.x : R(1);
                 t.y = lok ? t.y : R(0);
             }
-            column(vc, std::integral_constant<bool, false>{}, t);
+            mul(rc, std::false_type{}, t);
         }
     });
 }
@@ -729,6 +724,10 @@ __device__ __forceinline__ double op_grad(const cx<R> (&a)[16], const cx<R> (&b)
     return lok ? acc : 0.0;
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS accesses, then for every
+// wave.  Unlike __syncthreads it leaves outstanding global memory operations in flight.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // One remap through LDS: the 16 amplitudes leave by the old offsets (lb ^ K[r]) and arrive by
 // the new ones.  complex64: one pass of 8-byte amplitudes.  complex128: the real parts, then the
 // imaginary parts (each pass writes only registers the other pass does not read).
@@ -736,33 +735,33 @@ template <typename R>
 __device__ __forceinline__ void lds_remap(unsigned char *smem, cx<R> (&v)[16], uint32_t lb, const KOff &K,
                                           uint32_t lbn, const KOff &Kn) {
     if constexpr (sizeof(R) == 4) {
-        __syncthreads();
+        lds_barrier();
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             *reinterpret_cast<cx<R> *>(smem + (lb ^ koff<r>(K))) = v[r];
         });
-        __syncthreads();
+        lds_barrier();
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             v[r] = *reinterpret_cast<const cx<R> *>(smem + (lbn ^ koff<r>(Kn)));
         });
     } else {
-        __syncthreads();
+        lds_barrier();
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             *reinterpret_cast<R *>(smem + (lb ^ koff<r>(K))) = v[r].x;
         });
-        __syncthreads();
+        lds_barrier();
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             v[r].x = *reinterpret_cast<const R *>(smem + (lbn ^ koff<r>(Kn)));
         });
-        __syncthreads();
+        lds_barrier();
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             *reinterpret_cast<R *>(smem + (lb ^ koff<r>(K))) = v[r].y;
         });
-        __syncthreads();
+        lds_barrier();
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
             v[r].y = *reinterpret_cast<const R *>(smem + (lbn ^ koff<r>(Kn)));
@@ -772,7 +771,7 @@ __device__ __forceinline__ void lds_remap(unsigned char *smem, cx<R> (&v)[16], u
 
 // One stage over one state (PAIR = false) or over psi and lambda together (PAIR = true: every op
 // acts on both, and I_GRAD accumulates Re <lambda| M |psi> per gradient slot).
-template <typename R, bool PAIR, bool BIG>
+template <typename R, bool PAIR, bool BIG, bool DM>
 __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__restrict__ st2,
                                            const StageHeader *__restrict__ hdr, const Instr *__restrict__ prog,
                                            const R *__restrict__ pool, uint64_t ntiles, double *__restrict__ gpart,
@@ -792,8 +791,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
         __syncthreads();
     }
 
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        // deposit the tile index into the qubits outside the tile (T ascending)
+    // base index of a tile: the tile number deposited into the qubits outside the tile (T ascending)
+    auto tile_base = [&](uint64_t tile) __attribute__((always_inline)) {
         uint64_t base = tile;
 #pragma unroll
         for (int j = 0; j < 16; j++) {
@@ -803,19 +802,29 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 base = ((base >> t) << (t + 1)) | lo;
             }
         }
+        return base;
+    };
+    // the tile's loads issue ahead of other waves' arithmetic (wave priority)
+    auto load_tile = [&](cx<R>(&v)[16], cx<R>(&w)[16], uint64_t base) __attribute__((always_inline)) {
+        __builtin_amdgcn_s_setprio(2);
+        GAddr G;
+        global_layout(G, hdr->gq0, m, tid, base);
+        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+            constexpr int r = decltype(rc)::value;
+            v[r] = ld_stream(st + (G.tg | rgof<r>(G)));
+            if constexpr (PAIR) w[r] = ld_stream(st2 + (G.tg | rgof<r>(G)));
+        });
+        __builtin_amdgcn_s_setprio(0);
+    };
+
+    // Tiles blockIdx.x, + gridDim.x, ... (one per workgroup unless there are more than 2^20).
+    // Fetching the next tile while the ops run on the current one would need a second register set:
+    // two waves per SIMD instead of four, measured 45 % slower (the op stream has too many short
+    // dependencies to run at that occupancy).
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const uint64_t base = tile_base(tile);
         cx<R> a[16], b[16];  // b: lambda in paired mode, unused otherwise
-        {
-            // the tile's loads issue ahead of other waves' arithmetic (wave priority)
-            __builtin_amdgcn_s_setprio(2);
-            GAddr G;
-            global_layout(G, hdr->gq0, m, tid, base);
-            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-                constexpr int r = decltype(rc)::value;
-                a[r] = ld_stream(st + (G.tg | rgof<r>(G)));
-                if constexpr (PAIR) b[r] = ld_stream(st2 + (G.tg | rgof<r>(G)));
-            });
-            __builtin_amdgcn_s_setprio(0);
-        }
+        load_tile(a, b, base);
         uint32_t lb = lds_lane<R>(hdr->lay0, m, tid);
         KOff K;
         load_koff(K, hdr->k0);
@@ -829,40 +838,15 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             for (; pc < ninstr; pc++) {
                 pc = __builtin_amdgcn_readfirstlane(pc);
                 const IHead cur = load_head(P, pc);  // one batch of scalar loads
-                if (cur.flags & F_OUTER) break;
-                const int op = cur.op;
-                const int flags = cur.flags;
-                if ((flags & F_TILE) && (base & cur.omask) != cur.omask) continue;  // uniform per workgroup
+                const int code = (cur.flags >> kCodeShift) & 255;
+                if (code == K_OUTER) break;
+                if ((cur.flags & F_TILE) && (base & cur.omask) != cur.omask) continue;  // uniform per workgroup
                 const bool lok = (tid & cur.tmask) == cur.tmask;
                 const uint32_t pat = cur.pat;
                 const R *__restrict__ M = pool + 2 * cur.mat;
-                const uint32_t sw = cur.sw;
-                if (op == I_DENSE) {
-                    const bool ctrl = flags & (F_TCTRL | F_PAT);
-                    const bool real = flags & F_REAL;
-#define DENSE_CASE(MK)                                                        \
-    case MK:                                                                  \
-        if (ctrl) op_dense<MK, true, PAIR>(a, b, M, pat, lok);                \
-        else if (real) op_dense<MK, false, PAIR, R, true>(a, b, M, pat, lok); \
-        else op_dense<MK, false, PAIR>(a, b, M, pat, lok);                    \
-        break;
-                    switch (cur.smask) {
-                        DENSE_CASE(1)
-                        DENSE_CASE(2)
-                        DENSE_CASE(4)
-                        DENSE_CASE(8)
-                        DENSE_CASE(3)
-                        DENSE_CASE(5)
-                        DENSE_CASE(6)
-                        DENSE_CASE(9)
-                        DENSE_CASE(10)
-                        DENSE_CASE(12)
-                    default:
-                        break;
-                    }
-#undef DENSE_CASE
-                } else if (op == I_SHEAR) {
-                    switch (((sw & 255u) << 1) | (cur.k == 3 ? 1 : 0)) {
+                // one switch per op range; the hot ranges (shears, phase tables) come first
+                if (code < K_DIAG_FULL) {
+                    switch (code) {
                     case 0: op_shear<0, 2, PAIR>(a, b, cur); break;
                     case 1: op_shear<0, 3, PAIR>(a, b, cur); break;
                     case 2: op_shear<1, 2, PAIR>(a, b, cur); break;
@@ -870,130 +854,98 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     case 4: op_shear<2, 2, PAIR>(a, b, cur); break;
                     case 5: op_shear<2, 3, PAIR>(a, b, cur); break;
                     case 6: op_shear<3, 2, PAIR>(a, b, cur); break;
-                    case 7: op_shear<3, 3, PAIR>(a, b, cur); break;
-                    default: break;
+                    default: op_shear<3, 3, PAIR>(a, b, cur); break;
                     }
-                } else if (op == I_DIAG) {
-#define DIAG_CASE(RM)                                          \
-        case RM:                                                     \
-            op_diag<RM, PAIR>(a, b, cur, pool, tid, base, lok);        \
-            break;
-                    switch (cur.smask) {
-                        DIAG_CASE(0)
-                        DIAG_CASE(1)
-                        DIAG_CASE(2)
-                        DIAG_CASE(3)
-                        DIAG_CASE(4)
-                        DIAG_CASE(5)
-                        DIAG_CASE(6)
-                        DIAG_CASE(7)
-                        DIAG_CASE(8)
-                        DIAG_CASE(9)
-                        DIAG_CASE(10)
-                        DIAG_CASE(11)
-                        DIAG_CASE(12)
-                        DIAG_CASE(13)
-                        DIAG_CASE(14)
-                        DIAG_CASE(15)
-                    default:
-                        break;
-                    }
-#undef DIAG_CASE
-                } else if (op == I_ANTI1) {
-                    const int variant = ((sw & 255u) << 2) | ((flags & F_PURE) ? 2 : 0) | ((flags & F_TCTRL) ? 1 : 0);
-#define ANTI_CASE(S)                                                             \
-        case S * 4 + 0: op_anti1<S, false, false, PAIR>(a, b, M, pat, lok); break;   \
-        case S * 4 + 1: op_anti1<S, false, true, PAIR>(a, b, M, pat, lok); break;    \
-        case S * 4 + 2: op_anti1<S, true, false, PAIR>(a, b, M, pat, lok); break;    \
-        case S * 4 + 3: op_anti1<S, true, true, PAIR>(a, b, M, pat, lok); break;
-                    switch (variant) {
+                } else if (code < K_ANTI1) {
+                    op_diag<PAIR>(a, b, cur, pool, tid, base, lok, code);
+                } else if (code < K_DENSE1) {
+#define ANTI_CASE(S)                                                                          \
+    case K_ANTI1 + S * 4 + 0: op_anti1<S, false, false, PAIR>(a, b, M, pat, lok); break;        \
+    case K_ANTI1 + S * 4 + 1: op_anti1<S, false, true, PAIR>(a, b, M, pat, lok); break;         \
+    case K_ANTI1 + S * 4 + 2: op_anti1<S, true, false, PAIR>(a, b, M, pat, lok); break;         \
+    case K_ANTI1 + S * 4 + 3: op_anti1<S, true, true, PAIR>(a, b, M, pat, lok); break;
+#define SWAP_CASE(P, A, B)                                                                    \
+    case K_SWAP2 + P * 4 + 0: op_swap2<A, B, false, false, PAIR>(a, b, M, pat, lok); break;     \
+    case K_SWAP2 + P * 4 + 1: op_swap2<A, B, false, true, PAIR>(a, b, M, pat, lok); break;      \
+    case K_SWAP2 + P * 4 + 2: op_swap2<A, B, true, false, PAIR>(a, b, M, pat, lok); break;      \
+    case K_SWAP2 + P * 4 + 3: op_swap2<A, B, true, true, PAIR>(a, b, M, pat, lok); break;
+                    switch (code) {
                         ANTI_CASE(0)
                         ANTI_CASE(1)
                         ANTI_CASE(2)
                         ANTI_CASE(3)
+                        SWAP_CASE(0, 0, 1)
+                        SWAP_CASE(1, 0, 2)
+                        SWAP_CASE(2, 0, 3)
+                        SWAP_CASE(3, 1, 2)
+                        SWAP_CASE(4, 1, 3)
+                        SWAP_CASE(5, 2, 3)
                     default:
                         break;
                     }
 #undef ANTI_CASE
-                } else if (op == I_SWAP2) {
-                    const int s0 = sw & 255u, s1 = (sw >> 8) & 255u;
-                    const int variant = ((s0 * 4 + s1) << 2) | ((flags & F_PURE) ? 2 : 0) | ((flags & F_TCTRL) ? 1 : 0);
-#define SWAP_CASE(A, B)                                                                       \
-        case (A * 4 + B) * 4 + 0: op_swap2<A, B, false, false, PAIR>(a, b, M, pat, lok); break;   \
-        case (A * 4 + B) * 4 + 1: op_swap2<A, B, false, true, PAIR>(a, b, M, pat, lok); break;    \
-        case (A * 4 + B) * 4 + 2: op_swap2<A, B, true, false, PAIR>(a, b, M, pat, lok); break;    \
-        case (A * 4 + B) * 4 + 3: op_swap2<A, B, true, true, PAIR>(a, b, M, pat, lok); break;
-                    switch (variant) {
-                        SWAP_CASE(0, 1)
-                        SWAP_CASE(0, 2)
-                        SWAP_CASE(0, 3)
-                        SWAP_CASE(1, 2)
-                        SWAP_CASE(1, 3)
-                        SWAP_CASE(2, 3)
+#undef SWAP_CASE
+                } else if (code < K_PARITY2) {
+#define DENSE1_CASE(S)                                                                                   \
+    case K_DENSE1 + 3 * S + 0: op_dense<1 << S, true, PAIR>(a, b, M, pat, lok); break;                     \
+    case K_DENSE1 + 3 * S + 1: op_dense<1 << S, false, PAIR, R, true>(a, b, M, pat, lok); break;           \
+    case K_DENSE1 + 3 * S + 2: op_dense<1 << S, false, PAIR>(a, b, M, pat, lok); break;
+#define DENSE2_CASE(P, MK)                                                                               \
+    case K_DENSE2 + 2 * P + 0: op_dense<MK, true, PAIR>(a, b, M, pat, lok); break;                         \
+    case K_DENSE2 + 2 * P + 1: op_dense<MK, false, PAIR>(a, b, M, pat, lok); break;
+                    switch (code) {
+                        DENSE1_CASE(0)
+                        DENSE1_CASE(1)
+                        DENSE1_CASE(2)
+                        DENSE1_CASE(3)
+                        DENSE2_CASE(0, 3)
+                        DENSE2_CASE(1, 5)
+                        DENSE2_CASE(2, 9)
+                        DENSE2_CASE(3, 6)
+                        DENSE2_CASE(4, 10)
+                        DENSE2_CASE(5, 12)
                     default:
                         break;
                     }
-#undef SWAP_CASE
-                } else if (op == I_PARITY) {
-                    if constexpr (!PAIR) {  // density-matrix programs only
-                        const bool real = flags & F_REAL;
-                        if (cur.k == 2) {
-#define PAR2_CASE(LO, HI)                                                   \
-    case (1 << LO) | (1 << HI):                                             \
-        if (real) op_parity2<LO, HI, true>(a, M);                           \
-        else op_parity2<LO, HI, false>(a, M);                               \
-        break;
-                            switch (cur.smask) {
-                                PAR2_CASE(0, 1)
-                                PAR2_CASE(0, 2)
-                                PAR2_CASE(0, 3)
-                                PAR2_CASE(1, 2)
-                                PAR2_CASE(1, 3)
-                                PAR2_CASE(2, 3)
-                            default:
-                                break;
-                            }
-#undef PAR2_CASE
-                        } else {
-                            // pairing by the partner of slot 0 (the other two slots ascending)
-                            switch ((sw >> 8) & 255u) {
-                            case 1:
-                                if (real) op_parity4<1, 2, 3, true>(a, M);
-                                else op_parity4<1, 2, 3, false>(a, M);
-                                break;
-                            case 2:
-                                if (real) op_parity4<2, 1, 3, true>(a, M);
-                                else op_parity4<2, 1, 3, false>(a, M);
-                                break;
-                            default:
-                                if (real) op_parity4<3, 1, 2, true>(a, M);
-                                else op_parity4<3, 1, 2, false>(a, M);
-                                break;
-                            }
+#undef DENSE1_CASE
+#undef DENSE2_CASE
+                } else if (code < K_GRAD) {
+                    if constexpr (DM) {  // density-matrix channel superoperators
+#define PAR2_CASE(P, LO, HI)                                                          \
+    case K_PARITY2 + 2 * P + 0: op_parity2<LO, HI, true>(a, M); break;                  \
+    case K_PARITY2 + 2 * P + 1: op_parity2<LO, HI, false>(a, M); break;
+                        switch (code) {
+                            PAR2_CASE(0, 0, 1)
+                            PAR2_CASE(1, 0, 2)
+                            PAR2_CASE(2, 0, 3)
+                            PAR2_CASE(3, 1, 2)
+                            PAR2_CASE(4, 1, 3)
+                            PAR2_CASE(5, 2, 3)
+                        case K_PARITY4 + 0: op_parity4<1, 2, 3, true>(a, M); break;
+                        case K_PARITY4 + 1: op_parity4<1, 2, 3, false>(a, M); break;
+                        case K_PARITY4 + 2: op_parity4<2, 1, 3, true>(a, M); break;
+                        case K_PARITY4 + 3: op_parity4<2, 1, 3, false>(a, M); break;
+                        case K_PARITY4 + 4: op_parity4<3, 1, 2, true>(a, M); break;
+                        case K_PARITY4 + 5: op_parity4<3, 1, 2, false>(a, M); break;
+                        default: break;
                         }
+#undef PAR2_CASE
                     }
-                } else if (op == I_GRAD) {
+                } else {
                     if constexpr (PAIR) {
                         double v = 0;
-#define GRAD_CASE(MK)                              \
-        case MK:                                       \
-            v = op_grad<MK>(a, b, M, pat, lok);        \
-            break;
-                        switch (cur.smask) {
-                            GRAD_CASE(1)
-                            GRAD_CASE(2)
-                            GRAD_CASE(3)
-                            GRAD_CASE(4)
-                            GRAD_CASE(5)
-                            GRAD_CASE(6)
-                            GRAD_CASE(8)
-                            GRAD_CASE(9)
-                            GRAD_CASE(10)
-                            GRAD_CASE(12)
-                        default:
-                            break;
+                        switch (code - K_GRAD) {
+                        case 0: v = op_grad<1>(a, b, M, pat, lok); break;
+                        case 1: v = op_grad<2>(a, b, M, pat, lok); break;
+                        case 2: v = op_grad<3>(a, b, M, pat, lok); break;
+                        case 3: v = op_grad<4>(a, b, M, pat, lok); break;
+                        case 4: v = op_grad<5>(a, b, M, pat, lok); break;
+                        case 5: v = op_grad<6>(a, b, M, pat, lok); break;
+                        case 6: v = op_grad<8>(a, b, M, pat, lok); break;
+                        case 7: v = op_grad<9>(a, b, M, pat, lok); break;
+                        case 8: v = op_grad<10>(a, b, M, pat, lok); break;
+                        default: v = op_grad<12>(a, b, M, pat, lok); break;
                         }
-#undef GRAD_CASE
                         // blocks of fewer than 64 threads only read active lanes
     #pragma unroll
                         for (int off = 32; off >= 1; off >>= 1)
@@ -1070,19 +1022,27 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
     }
 }
 
-// Two instantiations: k_stage runs stages without 3-4-slot dense ops and so needs few enough
-// VGPRs for four waves per SIMD; k_stage_big runs the stages that hold them.
+// Three single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops
+// (few enough VGPRs for four waves per SIMD, and code small enough for short branches),
+// k_stage_big the stages that hold them, k_stage_dm density-matrix stages (channel
+// superoperators as well).
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                uint64_t ntiles) {
-    stage_body<R, false, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+    stage_body<R, false, false, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
 }
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage_big(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                    uint64_t ntiles) {
-    stage_body<R, false, true>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+    stage_body<R, false, true, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+}
+template <typename R>
+__global__ void __launch_bounds__(512) k_stage_dm(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+                                                  const Instr *__restrict__ prog, const R *__restrict__ pool,
+                                                  uint64_t ntiles) {
+    stage_body<R, false, true, true>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
 }
 
 template <typename R>
@@ -1091,11 +1051,11 @@ __global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<
                                                     const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                     uint64_t ntiles, double *__restrict__ gpart) {
     __shared__ double gacc[kMaxGradPerStage * 8];
-    stage_body<R, true, true>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
+    stage_body<R, true, true, false>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
 }
 
-// The six stage kernels take minutes each to compile.  build.py compiles this file once per
-// kernel with GOQU_STAGE_KERNEL = 1..6 (that kernel's code only), in parallel, and once without
+// The seven stage kernels take minutes each to compile.  build.py compiles this file once per
+// kernel with GOQU_STAGE_KERNEL = 1..7 (that kernel's code only), in parallel, and once without
 // it: the host code below, which refers to the kernels as extern instantiations.
 #define GOQU_STAGE_INST(EXT, K, R)                                                                \
     EXT template __global__ void K<R>(cx<R> *__restrict__, const StageHeader *__restrict__,          \
@@ -1117,6 +1077,8 @@ GOQU_STAGE_INST(, k_stage_big, float)
 GOQU_PAIR_INST(, double)
 #elif GOQU_STAGE_KERNEL == 6
 GOQU_PAIR_INST(, float)
+#elif GOQU_STAGE_KERNEL == 7
+GOQU_STAGE_INST(, k_stage_dm, double)
 #endif
 }  // namespace goqu
 #else  // the host-code object
@@ -1126,6 +1088,7 @@ GOQU_STAGE_INST(extern, k_stage_big, double)
 GOQU_STAGE_INST(extern, k_stage_big, float)
 GOQU_PAIR_INST(extern, double)
 GOQU_PAIR_INST(extern, float)
+GOQU_STAGE_INST(extern, k_stage_dm, double)
 
 // ---- profiling ---------------------------------------------------------------------------------
 static std::atomic<bool> g_profile{false};
@@ -1141,18 +1104,23 @@ static void profile_add(double ms, int64_t launches, double bytes) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS).  GOQU_TILE_BITS picks 8..13; 2^13
-// tiles (512 threads, 64 KiB) cut the sweeps of the 30-qubit depth-40 circuit from 58 to 50 but
-// leave its run time within 2 % (the sweeps are bound by FP64 issue, not by HBM), so the default
-// keeps the smaller tile.  Paired programs (adjoint reverse sweeps, two states per tile) stay
-// at most 2^12: their kernel holds twice the amplitudes.
+// Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS) holding the 4 lowest qubits.  On the
+// 30-qubit depth-40 circuit: 61 sweeps of 7.85 ms (485 ms per Evolve) against 58 of 8.37 ms (492 ms)
+// with the 3 lowest qubits; 2^13 tiles cut the sweeps to 50-53 but each takes 9.1-9.5 ms.
+// GOQU_TILE_BITS (8..13) and GOQU_LOW_BITS (2..6) override the default, goqu_sv_set_plan sets a
+// handle's shape (the autotuner's choice).  Paired programs (adjoint reverse sweeps, two states
+// per tile) stay at most 2^12: their kernel holds twice the amplitudes.
 PlanConfig default_plan_config(int prec, int nq, bool paired) {
     PlanConfig c;
     c.tile_bits = 12;
-    c.low_bits = 3;
+    c.low_bits = 4;
     if (const char *e = std::getenv("GOQU_TILE_BITS")) {
         const int t = std::atoi(e);
         if (t >= 8 && t <= (paired ? 12 : 13)) c.tile_bits = t;
+    }
+    if (const char *e = std::getenv("GOQU_LOW_BITS")) {
+        const int l = std::atoi(e);
+        if (l >= 2 && l <= 6) c.low_bits = l;
     }
     (void)prec;
     (void)nq;
@@ -1218,6 +1186,8 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
         const uint64_t ntiles = 1ull << (st.nq - m);
         const unsigned threads = 1u << (m - kRegBits);
         const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
+        // one tile per workgroup (grid-stride beyond 2^20 tiles): measured faster than persistent
+        // workgroups sized to the device, whose waves stay in step and load in bursts
         const uint64_t cap = 1ull << 20;
         const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
         if (prof) {
@@ -1226,7 +1196,13 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
             GOQU_HIP(hipEventCreate(&ev[ev.size() - 1]));
             GOQU_HIP(hipEventRecord(ev[ev.size() - 2], s));
         }
-        if (dp.stages[i].flags & SF_BIG)
+        if (dp.stages[i].flags & SF_DM) {
+            if constexpr (sizeof(R) == 8)
+                hipLaunchKernelGGL(k_stage_dm<double>, dim3(grid), dim3(threads), lds, s,
+                                   static_cast<cx<double> *>(st.amps), hdrs + i, instrs, pool, ntiles);
+            else
+                return fail(GOQU_ERR_UNSUPPORTED, "density-matrix stages run in complex128 only");
+        } else if (dp.stages[i].flags & SF_BIG)
             hipLaunchKernelGGL(k_stage_big<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(st.amps),
                                hdrs + i, instrs, pool, ntiles);
         else
@@ -1331,9 +1307,9 @@ int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, s
     return GOQU_OK;
 }
 
-int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s) {
+int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg) {
     if (eops.empty()) return GOQU_OK;
-    Program P = plan_program(eops, st.nq, default_plan_config(st.prec, st.nq, false));
+    Program P = plan_program(eops, st.nq, cfg ? *cfg : default_plan_config(st.prec, st.nq, false));
     // grow-only program buffer per host thread: every call ends synchronised, so the next call
     // on this thread may overwrite it
     static thread_local DeviceProgram dp;

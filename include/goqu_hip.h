@@ -91,6 +91,10 @@ int goqu_sv_destroy(goqu_sv_t *h);                                   /* Close  s
 int goqu_sv_reset(goqu_sv_t *h);                                     /* Reset  sim.go:71-77 */
 /* Apply (sim.go:92-157): ops in order, without resetting.  Evolve = reset + apply. */
 int goqu_sv_apply(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops);
+/* Tile shape of this handle's Apply programs (no counterpart in the reference; the autotuner
+ * sets it): 2^tile_bits amplitudes per tile, of which the low_bits lowest qubits are in every
+ * tile.  tile_bits in 8..13, low_bits in 2..min(6, tile_bits - 4); (0, 0) restores the default. */
+int goqu_sv_set_plan(goqu_sv_t *h, int32_t tile_bits, int32_t low_bits);
 /* StateVector (sim.go:61-65) as a chunked copy-out; set_state is the StatePrep fast path
  * (sim.go:105-120). amps holds 2*count doubles. */
 int goqu_sv_get_state(goqu_sv_t *h, double *amps, uint64_t offset, uint64_t count);

@@ -123,6 +123,52 @@ void count(const Instr &I) {
     }
 }
 
+// The kernel dispatches on the code in flags bits 8..15 alone: decode it the way the kernel's
+// switch does and check that the code path it selects is the one the instruction's fields describe.
+bool check_code(const Instr &I, std::string &err) {
+    const int code = instr_code(I.flags);
+    static const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+    static const int grad_masks[10] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12};
+    const bool tc = I.flags & F_TCTRL, pure = I.flags & F_PURE, real = I.flags & F_REAL;
+    const bool ctrl = I.flags & (F_TCTRL | F_PAT);
+    bool ok = false;
+    if (I.flags & F_OUTER) ok = code == K_OUTER;
+    else if (code < K_DIAG_FULL) ok = I.op == I_SHEAR && I.s[0] == code / 2 && I.k == 2 + code % 2;
+    else if (code < K_ANTI1) {
+        bool tile_bits = I.k > 0 && !tc;
+        for (int j = 0; j < I.k; j++) tile_bits = tile_bits && I.ls[j] >= 16;
+        ok = I.op == I_DIAG && (code == K_DIAG_LANE   ? (I.k > 0 || tc) && !tile_bits
+                                : code == K_DIAG_TILE ? tile_bits
+                                                      : (I.k == 0 && !tc)) &&
+             (code != K_DIAG_FULL || (I.pat | 1u) == 0xFFFFu);
+    }
+    else if (code < K_SWAP2) {
+        const int c = code - K_ANTI1;
+        ok = I.op == I_ANTI1 && I.s[0] == c / 4 && pure == ((c >> 1) & 1) && tc == (c & 1);
+    } else if (code < K_DENSE1) {
+        const int c = code - K_SWAP2;
+        ok = I.op == I_SWAP2 && I.s[0] == pairs[c / 4][0] && I.s[1] == pairs[c / 4][1] && pure == ((c >> 1) & 1) &&
+             tc == (c & 1);
+    } else if (code < K_DENSE2) {
+        const int c = code - K_DENSE1, v = ctrl ? 0 : real ? 1 : 2;
+        ok = I.op == I_DENSE && I.smask == (1 << (c / 3)) && c % 3 == v;
+    } else if (code < K_PARITY2) {
+        const int c = code - K_DENSE2;
+        ok = I.op == I_DENSE && I.smask == ((1 << pairs[c / 2][0]) | (1 << pairs[c / 2][1])) && (c % 2) == !ctrl;
+    } else if (code < K_PARITY4) {
+        const int c = code - K_PARITY2;
+        ok = I.op == I_PARITY && I.k == 2 && I.smask == ((1 << pairs[c / 2][0]) | (1 << pairs[c / 2][1])) &&
+             (c % 2) == !real;
+    } else if (code < K_GRAD) {
+        const int c = code - K_PARITY4;
+        ok = I.op == I_PARITY && I.k == 4 && I.s[0] == 0 && I.s[1] == 1 + c / 2 && (c % 2) == !real;
+    } else if (code < K_GRAD + 10) {
+        ok = I.op == I_GRAD && I.smask == grad_masks[code - K_GRAD];
+    }
+    if (!ok) err = "instruction code " + std::to_string(code) + " does not select op " + std::to_string(I.op);
+    return ok;
+}
+
 // Executes a program exactly as the stage kernel interprets its encoding (complex128: remaps move
 // 8-byte LDS elements, the real parts in one pass and the imaginary parts in a second).
 // st2 != nullptr: paired mode (every op on both states; I_GRAD adds Re <st2| M |st> of each
@@ -140,7 +186,14 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
     for (const StageHeader &H : P.stages) {
         const int m = H.m;
         const int nthr = 1 << (m - 4);
-        for (int pc = 0; pc < H.ninstr; pc++) count(P.instrs[H.instr_off + pc]);
+        for (int pc = 0; pc < H.ninstr; pc++) {
+            count(P.instrs[H.instr_off + pc]);
+            if (!check_code(P.instrs[H.instr_off + pc], err)) return 1;
+            if (P.instrs[H.instr_off + pc].op == I_PARITY && !(H.flags & SF_DM)) {
+                err = "a stage with channel ops is not flagged for the density-matrix kernel";
+                return 1;
+            }
+        }
         for (int j = 0; j < m; j++) {
             if (H.T[j] >= nq || (j && H.T[j] <= H.T[j - 1])) {
                 err = "bad tile qubit list";
@@ -398,12 +451,16 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                                 }
                                 li = (li << 1) | bit;
                             }
+                            (void)KR;
+                            if (I.pat >> 16) {
+                                err = "diag register mask beyond 16 registers";
+                                return 1;
+                            }
                             for (int r = 0; r < 16; r++) {
-                                const int v = pext4(r, RM);
-                                cd ph(pool[2 * ((li << KR) | v)], pool[2 * ((li << KR) | v) + 1]);
-                                if (!(I.pat >> v & 1)) {
+                                cd ph(pool[2 * ((li << 4) | r)], pool[2 * ((li << 4) | r) + 1]);
+                                if (!(I.pat >> r & 1)) {
                                     if (ph != cd(1, 0)) {
-                                        err = "diag column marked trivial holds an entry != 1";
+                                        err = "diag register marked trivial holds an entry != 1";
                                         return 1;
                                     }
                                     continue;
@@ -415,7 +472,7 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                         }
                         case I_SHEAR: {
                             const int S = I.s[0];
-                            if (S > 3 || I.smask != (1 << S) || I.k < 2 || I.k > 3 || I.flags != 0) {
+                            if (S > 3 || I.smask != (1 << S) || I.k < 2 || I.k > 3 || (I.flags & ((1 << kCodeShift) - 1)) != 0) {
                                 err = "malformed shear";
                                 return 1;
                             }
