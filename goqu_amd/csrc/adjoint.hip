@@ -30,6 +30,19 @@ static void ipow(int n, double &re, double &im) {
 // lam = H psi
 static int apply_hamiltonian(const State &psi, State &lam, int nt, const uint64_t *x, const uint64_t *z,
                              const int32_t *ny, const double *coeffs, Scratch &scr, hipStream_t s) {
+    {  // tiled passes (psi and lam move once per pass of up to 12 tile qubits)
+        std::vector<double> cf(2 * (size_t)nt);
+        for (int t = 0; t < nt; t++) {
+            double yr, yi;
+            ipow(ny[t], yr, yi);
+            const double cr = coeffs[2 * t], ci = coeffs[2 * t + 1];
+            cf[2 * t] = cr * yr - ci * yi;
+            cf[2 * t + 1] = cr * yi + ci * yr;
+        }
+        bool done = false;
+        int rc = apply_hamiltonian_tiled(psi, lam, nt, x, z, cf.data(), scr, s, done);
+        if (rc || done) return rc;
+    }
     int rc0 = fill_basis(lam, ~0ull, s);  // all zeros
     if (rc0) return rc0;
     std::map<uint64_t, std::vector<int>> groups;
@@ -51,9 +64,25 @@ static int apply_hamiltonian(const State &psi, State &lam, int nt, const uint64_
     return GOQU_OK;
 }
 
+// a work state shaped like ref, kept across calls (grow-only)
+static int ensure_like(State &buf, const State &ref) {
+    if (buf.amps && buf.nq == ref.nq && buf.prec == ref.prec) return GOQU_OK;
+    if (buf.amps) (void)hipFree(buf.amps);
+    buf.amps = nullptr;
+    buf.nq = ref.nq;
+    buf.prec = ref.prec;
+    buf.elem = ref.elem;
+    if (dmalloc((void **)&buf.amps, ref.size() * ref.elem) != hipSuccess) {
+        (void)hipGetLastError();
+        buf.amps = nullptr;
+        return fail(GOQU_ERR_OUT_OF_MEMORY, "adjoint: cannot allocate a work state vector");
+    }
+    return GOQU_OK;
+}
+
 int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goqu_deriv_t *derivs, int nd, int np,
                int nt, const uint64_t *x, const uint64_t *z, const int32_t *ny, const double *coeffs, double *energy,
-               double *grads, Scratch &scr, hipStream_t s) {
+               double *grads, Scratch &scr, hipStream_t s, State &lam, State &work) {
     const uint64_t valid = (1ull << nq_user) - 1;
     for (int t = 0; t < nt; t++)
         if ((x[t] & ~valid) || (z[t] & ~valid)) return fail(GOQU_ERR_INVALID_ARG, "Pauli mask beyond the qubit count");
@@ -78,20 +107,11 @@ int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goq
     rc = run_eops(st, all, s);
     if (rc) return rc;
 
-    State lam, work;
-    auto cleanup = [&]() {
-        if (lam.amps) (void)hipFree(lam.amps);
-        if (work.amps) (void)hipFree(work.amps);
-    };
-    lam.nq = work.nq = st.nq;
-    lam.prec = work.prec = st.prec;
-    lam.elem = work.elem = st.elem;
-    if (dmalloc((void **)&lam.amps, st.size() * st.elem) != hipSuccess ||
-        dmalloc((void **)&work.amps, st.size() * st.elem) != hipSuccess) {
-        (void)hipGetLastError();
-        cleanup();
-        return fail(GOQU_ERR_OUT_OF_MEMORY, "adjoint: cannot allocate two extra state vectors");
-    }
+    // lambda and the uncomputed copy of psi live in the handle between calls
+    auto cleanup = [&]() {};
+    rc = ensure_like(lam, st);
+    if (!rc) rc = ensure_like(work, st);
+    if (rc) return rc;
     rc = apply_hamiltonian(st, lam, nt, x, z, ny, coeffs, scr, s);
     double e2[2];
     if (!rc) rc = inner_product(st, lam, e2, scr, s);
@@ -330,7 +350,7 @@ int goqu_sv_adjoint(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops, const g
         (num_terms && (!xmask || !zmask || !num_y || !coeffs)))
         return fail(GOQU_ERR_INVALID_ARG, "bad argument");
     return adjoint_sv(h->st, h->nq_user, ops, num_ops, derivs, num_derivs, num_params, num_terms, xmask, zmask, num_y,
-                      coeffs, energy, grads, h->scr, h->stream);
+                      coeffs, energy, grads, h->scr, h->stream, h->adj_lam, h->adj_work);
     GOQU_GUARD_END
 }
 

@@ -251,6 +251,8 @@ int goqu_sv_destroy(goqu_sv_t *h) {
         (void)hipStreamDestroy(h->stream);
     }
     free_state(h->st);
+    free_state(h->adj_lam);
+    free_state(h->adj_work);
     delete h;
     return GOQU_OK;
     GOQU_GUARD_END

@@ -4,6 +4,7 @@
 // All reductions are deterministic: each workgroup writes one partial per quantity, and a
 // single-workgroup finisher adds the partials in a fixed order.
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 
 #include "runtime.h"
@@ -458,6 +459,195 @@ int apply_pauli_group(const State &psi, State &lam, uint64_t x, const uint64_t *
                            nt);
     GOQU_HIP(hipGetLastError());
     GOQU_HIP(hipStreamSynchronize(s));  // z/c staging is reused by the next call
+    return GOQU_OK;
+}
+
+// ---- H|psi> in passes over tiles -------------------------------------------------------------------
+// lam (+)= sum over a pass's x-groups g of D_g(i ^ x_g) psi[i ^ x_g], D_g(j) = sum_t c_t (-1)^{|j & z_t|}.
+// A pass holds groups whose x-masks all lie inside 12 tile qubits tq[] (ascending, the 3 lowest
+// qubits included); one workgroup per tile stages psi's tile in LDS and reads every flipped
+// partner from there, so a pass moves psi once and lam once instead of once per group.
+constexpr int kHTileBits = 12;
+struct HGroup {
+    uint64_t x;   // global x-mask
+    uint32_t xl;  // x-mask on the tile's local bits
+    int32_t t0;   // first term (index into z[] and c[])
+    int32_t nt;   // terms
+    int32_t pad;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_pauli_apply_tile(const cx<R> *__restrict__ psi, cx<R> *__restrict__ lam,
+                                                          const uint8_t *__restrict__ tq, const HGroup *__restrict__ grp,
+                                                          int ng, const uint64_t *__restrict__ z,
+                                                          const double *__restrict__ c, int accumulate) {
+    __shared__ cx<R> sm[1 << kHTileBits];
+    const uint32_t t = threadIdx.x;
+    uint8_t T[kHTileBits];
+#pragma unroll
+    for (int j = 0; j < kHTileBits; j++) T[j] = tq[j];
+    // the tile's base: blockIdx deposited into the qubits outside T
+    uint64_t base = blockIdx.x;
+#pragma unroll
+    for (int j = 0; j < kHTileBits; j++) {
+        const uint64_t lo = base & ((1ull << T[j]) - 1);
+        base = ((base >> T[j]) << (T[j] + 1)) | lo;
+    }
+    // local index j = t + 256 r: thread bits on T[0..7], register bits on T[8..11]
+    uint64_t gt = base;
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+        if (t >> u & 1) gt |= 1ull << T[u];
+    uint64_t gr[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        uint64_t g = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (r >> u & 1) g |= 1ull << T[8 + u];
+        gr[r] = g;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) sm[t + 256 * r] = psi[gt | gr[r]];
+    __syncthreads();
+    double ar[16], ai[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) ar[r] = ai[r] = 0.0;
+    for (int g = 0; g < ng; g++) {
+        const HGroup G = grp[g];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const uint64_t j = (gt | gr[r]) ^ G.x;  // global index of the partner
+            const cx<R> p = sm[(t + 256 * r) ^ G.xl];
+            double fr = 0, fi = 0;
+            for (int k = 0; k < G.nt; k++) {
+                const bool neg = __popcll(j & z[G.t0 + k]) & 1;
+                fr += neg ? -c[2 * (G.t0 + k)] : c[2 * (G.t0 + k)];
+                fi += neg ? -c[2 * (G.t0 + k) + 1] : c[2 * (G.t0 + k) + 1];
+            }
+            ar[r] += fr * (double)p.x - fi * (double)p.y;
+            ai[r] += fr * (double)p.y + fi * (double)p.x;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        cx<R> *d = lam + (gt | gr[r]);
+        if (accumulate) {
+            const cx<R> o = *d;
+            *d = cx<R>{(R)(o.x + ar[r]), (R)(o.y + ai[r])};
+        } else {
+            *d = cx<R>{(R)ar[r], (R)ai[r]};
+        }
+    }
+}
+
+int apply_hamiltonian_tiled(const State &psi, State &lam, int nt, const uint64_t *x, const uint64_t *z,
+                            const double *cfold, Scratch &scr, hipStream_t s, bool &done) {
+    done = false;
+    if (psi.nq < kHTileBits || nt == 0) return GOQU_OK;
+    // groups by x-mask, in first-appearance order of their x-mask
+    std::vector<uint64_t> gx;
+    std::vector<std::vector<int>> gterms;
+    for (int t = 0; t < nt; t++) {
+        size_t g = 0;
+        while (g < gx.size() && gx[g] != x[t]) g++;
+        if (g == gx.size()) {
+            gx.push_back(x[t]);
+            gterms.emplace_back();
+        }
+        gterms[g].push_back(t);
+    }
+    const uint64_t low = 7;  // qubits 0..2: runs of 8 amplitudes per global access
+    for (uint64_t m : gx)
+        if (__builtin_popcountll(m | low) > kHTileBits) return GOQU_OK;  // a group needs more tile qubits
+    // greedy passes: a group joins the first pass whose qubit set stays within 12 qubits
+    std::vector<uint64_t> pass_q;
+    std::vector<std::vector<int>> pass_g;
+    for (size_t g = 0; g < gx.size(); g++) {
+        size_t p = 0;
+        while (p < pass_q.size() && __builtin_popcountll(pass_q[p] | gx[g]) > kHTileBits) p++;
+        if (p == pass_q.size()) {
+            pass_q.push_back(low);
+            pass_g.emplace_back();
+        }
+        pass_q[p] |= gx[g];
+        pass_g[p].push_back((int)g);
+    }
+    // device staging: per pass 16 bytes of tile qubits + groups, then all terms
+    const size_t np = pass_q.size(), ngt = gx.size();
+    std::vector<unsigned char> host;
+    auto put = [&](const void *p, size_t b) {
+        const size_t at = (host.size() + 15) & ~size_t(15);
+        host.resize(at + b);
+        std::memcpy(host.data() + at, p, b);
+        return at;
+    };
+    std::vector<uint64_t> zt(nt);
+    std::vector<double> ct(2 * (size_t)nt);
+    std::vector<size_t> off_q(np), off_g(np);
+    int32_t k = 0;
+    std::vector<int32_t> first(ngt);
+    for (size_t g = 0; g < ngt; g++) {
+        first[g] = k;
+        for (int t : gterms[g]) {
+            zt[k] = z[t];
+            ct[2 * k] = cfold[2 * t];
+            ct[2 * k + 1] = cfold[2 * t + 1];
+            k++;
+        }
+    }
+    const size_t off_z = put(zt.data(), zt.size() * 8), off_c = put(ct.data(), ct.size() * 8);
+    for (size_t p = 0; p < np; p++) {
+        uint64_t q = pass_q[p];
+        for (int b = 0; b < psi.nq && __builtin_popcountll(q) < kHTileBits; b++) q |= 1ull << b;
+        uint8_t tq[16] = {0};
+        int loc[64];
+        for (int b = 0, j = 0; b < psi.nq; b++)
+            if (q >> b & 1) {
+                loc[b] = j;
+                tq[j++] = (uint8_t)b;
+            }
+        std::vector<HGroup> hg;
+        for (int g : pass_g[p]) {
+            HGroup G{};
+            G.x = gx[g];
+            uint32_t xl = 0;
+            for (uint64_t m = gx[g]; m; m &= m - 1) {
+                const int j = loc[__builtin_ctzll(m)];
+                // local index bits: thread bits T[0..7] -> index bits 0..7, register bits T[8..11] -> 8..11
+                xl |= 1u << j;
+            }
+            G.xl = xl;
+            G.t0 = first[g];
+            G.nt = (int32_t)gterms[g].size();
+            hg.push_back(G);
+        }
+        off_q[p] = put(tq, 16);
+        off_g[p] = put(hg.data(), hg.size() * sizeof(HGroup));
+    }
+    int rc = scr.ensure(host.size() + 64);
+    if (rc) return rc;
+    char *dev = static_cast<char *>(scr.ptr);
+    GOQU_HIP(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, s));
+    const unsigned grid = (unsigned)(psi.size() >> kHTileBits);
+    for (size_t p = 0; p < np; p++) {
+        const int ng = (int)pass_g[p].size();
+        const uint8_t *dq = reinterpret_cast<const uint8_t *>(dev + off_q[p]);
+        const HGroup *dg = reinterpret_cast<const HGroup *>(dev + off_g[p]);
+        const uint64_t *dz = reinterpret_cast<const uint64_t *>(dev + off_z);
+        const double *dc = reinterpret_cast<const double *>(dev + off_c);
+        if (psi.prec == GOQU_C128)
+            hipLaunchKernelGGL(k_pauli_apply_tile<double>, dim3(grid), dim3(256), 0, s,
+                               static_cast<const cx<double> *>(psi.amps), static_cast<cx<double> *>(lam.amps), dq, dg,
+                               ng, dz, dc, p > 0 ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_pauli_apply_tile<float>, dim3(grid), dim3(256), 0, s,
+                               static_cast<const cx<float> *>(psi.amps), static_cast<cx<float> *>(lam.amps), dq, dg, ng,
+                               dz, dc, p > 0 ? 1 : 0);
+        GOQU_HIP(hipGetLastError());
+    }
+    GOQU_HIP(hipStreamSynchronize(s));  // the host staging buffer is released on return
+    done = true;
     return GOQU_OK;
 }
 

@@ -108,6 +108,10 @@ int sample_state_static(State &st, int dev, uint64_t nvalid, const double *u, in
 // lam += coeff * P psi over a group of terms sharing x: lam[i] += sum_t c_t s_t(i^x) psi[i^x]
 int apply_pauli_group(const State &psi, State &lam, uint64_t x, const uint64_t *z, const double *c, int nt,
                       Scratch &scr, hipStream_t s);
+// lam = sum_t c_t P_t psi (c_t with i^ny folded in) in passes over 2^12-amplitude tiles; done = false
+// (lam untouched) when the state has fewer than 12 qubits or an x-group spans too many qubits
+int apply_hamiltonian_tiled(const State &psi, State &lam, int nt, const uint64_t *x, const uint64_t *z,
+                            const double *cfold, Scratch &scr, hipStream_t s, bool &done);
 // <lam| (dM on targets, under cmask) |psi>, dM dense k-qubit (k<=4) in gate order
 int sandwich(const State &lam, const State &psi, const EOp &dop, double out[2], Scratch &scr, hipStream_t s);
 int scale_state(State &st, double re, double im, hipStream_t s);
@@ -129,6 +133,7 @@ struct goqu_sv {
     int device = 0;  // the device current when the handle was created
     bool plan_set = false;  // goqu_sv_set_plan: tile shape of this handle's Apply programs
     goqu::PlanConfig plan;
+    goqu::State adj_lam, adj_work;  // adjoint work states, allocated on first use
     hipStream_t stream = nullptr;
     goqu::Scratch scr;
 };

@@ -1260,10 +1260,14 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
     const uint64_t ntiles = 1ull << (psi.nq - m);
     const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 1ull << 20);
     const int ng = P.num_grad;
-    // per-stage partials (slot-major, one per workgroup), reduced after every stage into gsum
+    // per-stage partials (slot-major, one per workgroup), reduced after every stage into gsum;
+    // grow-only per host thread (every call ends synchronised)
+    static thread_local Scratch gbuf;
     double *gpart = nullptr, *gsum = nullptr;
     if (ng) {
-        GOQU_HIP(dmalloc((void **)&gpart, ((size_t)kMaxGradPerStage * grid + ng) * sizeof(double)));
+        rc = gbuf.ensure(((size_t)kMaxGradPerStage * grid + ng) * sizeof(double));
+        if (rc) return rc;
+        gpart = static_cast<double *>(gbuf.ptr);
         gsum = gpart + (size_t)kMaxGradPerStage * grid;
     }
     for (size_t i = 0; i < P.stages.size() && !rc; i++) {
@@ -1283,7 +1287,6 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
     hipError_t e = hipSuccess;
     if (!rc && ng) e = hipMemcpyAsync(slot_vals.data(), gsum, ng * sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (gpart) (void)hipFree(gpart);
     if (!rc && e != hipSuccess) rc = fail(GOQU_ERR_DEVICE, std::string("paired stages: ") + hipGetErrorString(e));
     return rc;
 }
