@@ -22,6 +22,7 @@ int invert_eop(const EOp &e, EOp &inv, std::string &err) {
     inv = e;
     switch (e.kind) {
     case EOP_DENSE:
+    case EOP_WIDE:
     case EOP_PARITY: {  // conjugate transpose of the full matrix (keeps the parity blocks)
         const int d = 1 << e.k;
         for (int r = 0; r < d; r++)
@@ -114,6 +115,11 @@ int build_sv_reverse(const goqu_op_t *ops, int nops, int nq_user, const goqu_der
             return rc;
         }
         for (int j = (int)low.size() - 1; j >= 0; j--) {
+            if (low[j].kind == EOP_WIDE) {  // paired stage programs hold no wide ops
+                fused = false;
+                rev.clear();
+                return GOQU_OK;
+            }
             EOp ie;
             rc = invert_eop(low[j], ie, err);
             if (rc) return rc;

@@ -34,7 +34,10 @@ enum EOpKind : int32_t {
     EOP_PARITY = 6, // density-matrix channel superoperator that preserves each qubit's row^column
                     // parity: k = 2 (q = row, col) or 4 (q = row a, row b, col a, col b); m holds
                     // the full 2^k x 2^k superoperator (q[0] = MSB), zero across parity classes
+    EOP_WIDE = 7,   // dense on k = 5..kMaxWideTargets targets listed in qw (qw[0] = MSB), under
+                    // cmask: not part of any stage; run by the MFMA kernel between stage programs
 };
+constexpr int kMaxWideTargets = 9;
 
 // Gradient slots a single stage may hold (per-wave partials live in LDS).
 constexpr int kMaxGradPerStage = 64;
@@ -45,6 +48,7 @@ struct EOp {
     int32_t q[4] = {0, 0, 0, 0};
     uint64_t cmask = 0;   // control qubits (all must be 1)
     std::vector<double> m;  // interleaved complex payload (dense matrix / diag / phases)
+    std::vector<int> qw;    // EOP_WIDE: target qubits, qw[0] = MSB of the matrix index
     bool pure_swap = false; // ANTI1 / SWAP2 with unit phases: move only
     int32_t slot = -1;      // EOP_GRAD: global gradient slot
     // which of the q[] are targets that must sit in registers (DENSE, ANTI1, SWAP2, RESET)
@@ -54,15 +58,14 @@ struct EOp {
     // amplitudes of a pair share, so it commutes with no op on any qubit (not even a diagonal
     // one on another qubit).  Every reordering treats it as touching all qubits non-diagonally.
     bool is_barrier() const { return kind == EOP_RESET; }
-    uint64_t touch() const {
-        uint64_t t = cmask;
-        for (int i = 0; i < k; i++) t |= 1ull << q[i];
-        return t;
-    }
-    uint64_t targets() const {
-        if (!needs_regs()) return 0;
+    uint64_t touch() const { return cmask | targets_of(); }
+    uint64_t targets() const { return needs_regs() ? targets_of() : 0; }
+    uint64_t targets_of() const {
         uint64_t t = 0;
-        for (int i = 0; i < k; i++) t |= 1ull << q[i];
+        if (kind == EOP_WIDE)
+            for (int x : qw) t |= 1ull << x;
+        else
+            for (int i = 0; i < k; i++) t |= 1ull << q[i];
         return t;
     }
     // qubits acted on non-diagonally (targets) and only diagonally (controls, diagonal gates):
@@ -226,7 +229,7 @@ int lower_sv_ops(const void *ops /* goqu_op_t* */, int nops, int nq_user,
 // Lowering for a density matrix of n qubits stored as a 2n-qubit vector.
 int lower_dm_ops(const void *ops, int nops, int n, std::vector<EOp> &out, std::string &err);
 
-// Greedy stage planner + register allocation.
+// Greedy stage planner + register allocation (no EOP_WIDE ops: run_eops splits at them).
 Program plan_program(const std::vector<EOp> &eops, int nq_internal, const PlanConfig &cfg);
 // Fills Instr::kr and StageHeader::k0: LDS byte offsets of 8-byte elements.
 void set_lds_offsets(Program &P);

@@ -42,6 +42,17 @@ static EOp make_diag(const int *q, int k, uint64_t cmask, const double *m, int d
     return e;
 }
 
+static EOp make_wide(const int *q, int k, uint64_t cmask, const double *m) {
+    EOp e;
+    e.kind = EOP_WIDE;
+    e.k = k;
+    e.qw.assign(q, q + k);
+    e.cmask = cmask;
+    const size_t d = size_t(1) << k;
+    e.m.assign(m, m + 2 * d * d);
+    return e;
+}
+
 static EOp make_dense(const int *q, int k, uint64_t cmask, const double *m, int dim) {
     EOp e;
     e.kind = EOP_DENSE;
@@ -131,12 +142,16 @@ int lower_sv_ops(const void *ops_v, int nops, int nq, std::vector<EOp> &out, std
                 err = "controlled gate with " + std::to_string(op.num_qubits) + " qubits exceeds 10";
                 return GOQU_ERR_UNSUPPORTED;
             }
-            if (nt > kRegBits) {
-                err = "controlled gate with more than 4 target qubits";
-                return GOQU_ERR_UNSUPPORTED;
-            }
             uint64_t cm = 0;
             for (int j = 0; j < nc; j++) cm |= 1ull << q[j];
+            if (nt > kRegBits) {  // kernel_controlled.go:249-292: full inner matrix on 5..9 targets
+                if (is_diagonal(m, 1 << nt)) {
+                    err = "controlled diagonal gate with more than 4 target qubits";
+                    return GOQU_ERR_UNSUPPORTED;
+                }
+                out.push_back(make_wide(q + nc, nt, cm, m));
+                continue;
+            }
             lower_matrix(q + nc, nt, cm, m, out);
             continue;
         }
@@ -319,13 +334,13 @@ int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std:
                 err = "op " + std::to_string(i) + ": bad control count";
                 return GOQU_ERR_INVALID_ARG;
             }
-            if (nt > kRegBits) {
-                err = "op " + std::to_string(i) + ": more than 4 target qubits";
+            if (nt > kMaxWideTargets) {
+                err = "op " + std::to_string(i) + ": more than 9 target qubits";
                 return GOQU_ERR_UNSUPPORTED;
             }
             if (!op.matrix) continue;
             uint64_t rc = 0, cc = 0;
-            int rt[4], ct[4];
+            int rt[kMaxWideTargets], ct[kMaxWideTargets];
             for (int j = 0; j < nc; j++) {
                 rc |= 1ull << (op.qubits[j] + n);
                 cc |= 1ull << op.qubits[j];
@@ -335,8 +350,13 @@ int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std:
                 ct[j] = op.qubits[nc + j];
             }
             const int d = 1 << nt;
-            lower_matrix(rt, nt, rc, op.matrix, out);
             std::vector<double> cm = conj_of(op.matrix, d * d);
+            if (nt > kRegBits) {  // wide: U on the row qubits, conj(U) on the column qubits
+                out.push_back(make_wide(rt, nt, rc, op.matrix));
+                out.push_back(make_wide(ct, nt, cc, cm.data()));
+                continue;
+            }
+            lower_matrix(rt, nt, rc, op.matrix, out);
             lower_matrix(ct, nt, cc, cm.data(), out);
             continue;
         }

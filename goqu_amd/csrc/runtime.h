@@ -85,6 +85,9 @@ int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanC
 int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, std::vector<double> &slot_vals,
              hipStream_t s);
 
+// a dense op on 5..9 target qubits (EOP_WIDE) as FP64 MFMA contractions (wide.hip)
+int run_wide(State &st, const EOp &e, Scratch &scr, hipStream_t s);
+
 int fill_basis(State &st, uint64_t index, hipStream_t s);  // |index>
 int copy_state(State &dst, const State &src, hipStream_t s);
 

@@ -1310,7 +1310,7 @@ int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, s
     return GOQU_OK;
 }
 
-int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg) {
+static int run_segment(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg) {
     if (eops.empty()) return GOQU_OK;
     Program P = plan_program(eops, st.nq, cfg ? *cfg : default_plan_config(st.prec, st.nq, false));
     // grow-only program buffer per host thread: every call ends synchronised, so the next call
@@ -1322,6 +1322,30 @@ int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanC
     hipError_t e = hipStreamSynchronize(s);
     if (rc) return rc;
     if (e != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("stage kernels: ") + hipGetErrorString(e));
+    return GOQU_OK;
+}
+
+// Stage programs between wide ops (dense on 5..9 targets), which run as MFMA contractions.
+int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg) {
+    static thread_local Scratch wscr;
+    bool wide = false;
+    for (const EOp &e : eops) wide = wide || e.kind == EOP_WIDE;
+    if (!wide) return run_segment(st, eops, s, cfg);
+    size_t i = 0;
+    while (i < eops.size()) {
+        size_t j = i;
+        while (j < eops.size() && eops[j].kind != EOP_WIDE) j++;
+        if (j > i) {
+            int rc = run_segment(st, std::vector<EOp>(eops.begin() + i, eops.begin() + j), s, cfg);
+            if (rc) return rc;
+        }
+        if (j < eops.size()) {
+            int rc = run_wide(st, eops[j], wscr, s);
+            if (rc) return rc;
+            j++;
+        }
+        i = j;
+    }
     return GOQU_OK;
 }
 

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import cmath
 import math
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -116,6 +116,94 @@ class StatePrepGate(Gate):
 
     def Amplitudes(self) -> np.ndarray:
         return self._amps
+
+    def Decompose(self, qubits: Sequence[int]) -> List[Tuple["Gate", List[int]]]:
+        """The reference's RY + CNOT cascade (circuit/gate/stateprep.go:68-106), the path
+        statevector.Apply takes when the qubits are not exactly [0..n-1] (sim.go:121-135)."""
+        if len(qubits) != self.Qubits():
+            return []
+        return mottonen_decompose(self._amps, list(qubits))
+
+
+def _decompose_1q(amps: np.ndarray, q: int) -> List[Tuple[Gate, List[int]]]:
+    """stateprep.go:185-211: RY(2 atan2(|a1|, |a0|)) and an RZ for the relative phase."""
+    ops: List[Tuple[Gate, List[int]]] = []
+    theta = 2 * math.atan2(abs(amps[1]), abs(amps[0]))
+    if abs(theta) > 1e-10:
+        ops.append((RY(theta), [q]))
+    if abs(amps[0]) > 1e-10 and abs(amps[1]) > 1e-10:
+        d = cmath.phase(amps[1]) - cmath.phase(amps[0])
+        if abs(d) > 1e-10:
+            ops.append((RZ(d), [q]))
+    elif abs(amps[1]) > 1e-10:
+        ph = cmath.phase(amps[1])
+        if abs(ph) > 1e-10:
+            ops.append((RZ(ph), [q]))
+    return ops
+
+
+def _uniformly_controlled_ry(angles: Sequence[float], controls: List[int], target: int):
+    """stateprep.go:213-259: recursive split into (even, odd) half-sum / half-difference angles
+    around two CNOTs from the last control."""
+    n = len(angles)
+    if n == 0:
+        return []
+    if n == 1:
+        return [(RY(angles[0]), [target])] if abs(angles[0]) > 1e-10 else []
+    half = n // 2
+    even = [(angles[2 * k] + angles[2 * k + 1]) / 2 for k in range(half)]
+    odd = [(angles[2 * k] - angles[2 * k + 1]) / 2 for k in range(half)]
+    ops = []
+    for part in (even, odd):
+        if len(controls) > 1:
+            ops += _uniformly_controlled_ry(part, controls[:-1], target)
+        elif abs(part[0]) > 1e-10:
+            ops.append((RY(part[0]), [target]))
+        ops.append((CNOT, [controls[-1], target]))
+    return ops
+
+
+def mottonen_decompose(amplitudes: Sequence[complex], qubits: List[int]) -> List[Tuple[Gate, List[int]]]:
+    """stateprep.go:106-181: for level = 0..n-1 (target qubits[n-1-level], controls the qubits
+    before it), RY angles 2 atan2(|b|, |a|) of the paired halves, the pairs merged into the
+    first half scaled to the pair norm; a final RZ for the phase of the last amplitude."""
+    n = len(qubits)
+    if n == 0:
+        return []
+    amps = np.asarray(amplitudes, dtype=np.complex128)
+    if n == 1:
+        return _decompose_1q(amps, qubits[0])
+    ops: List[Tuple[Gate, List[int]]] = []
+    for level in range(n):
+        target = qubits[n - 1 - level]
+        hs = 1 << level
+        npairs = len(amps) // (2 * hs)
+        angles = []
+        new = np.zeros(npairs * hs, dtype=np.complex128)
+        for p in range(npairs):
+            g0 = amps[p * 2 * hs: p * 2 * hs + hs]
+            g1 = amps[p * 2 * hs + hs: p * 2 * hs + 2 * hs]
+            n0 = math.sqrt(float(np.sum(g0.real * g0.real + g0.imag * g0.imag)))
+            n1 = math.sqrt(float(np.sum(g1.real * g1.real + g1.imag * g1.imag)))
+            angles.append(2 * math.atan2(n1, n0))
+            tot = math.sqrt(n0 * n0 + n1 * n1)
+            if tot > 1e-15:
+                if n0 > 1e-15:
+                    new[p * hs: (p + 1) * hs] = g0 * (tot / n0)
+                else:
+                    new[p * hs] = tot
+        controls = list(qubits[: n - 1 - level])
+        if npairs == 1:
+            if abs(angles[0]) > 1e-10:
+                ops.append((RY(angles[0]), [target]))
+        else:
+            ops += _uniformly_controlled_ry(angles, controls, target)
+        amps = new
+    if len(amps) == 1:
+        ph = cmath.phase(amps[0])
+        if abs(ph) > 1e-10:
+            ops.append((RZ(ph), [qubits[0]]))
+    return ops
 
 
 s2 = 1.0 / math.sqrt(2.0)

@@ -101,7 +101,12 @@ class Sim:
         if list(qubits) == list(range(self.num_qubits)):
             self.SetStateVector(g.Amplitudes())
             return
-        raise NotImplementedError("StatePrep on a qubit subset (Möttönen decomposition) is not supported")
+        # slow path (sim.go:121-135): the gate's RY + CNOT cascade; gates without a matrix are skipped
+        buf = N.OpBuffer()
+        for dg, dq in g.Decompose(qubits):
+            if dg.Matrix() is not None:
+                N.add_sv_gate(buf, dg, dq)
+        self._flush(buf)
 
     def ApplyOps(self, buf: "N.OpBuffer") -> None:
         """Apply pre-marshalled op records."""

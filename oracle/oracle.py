@@ -87,12 +87,33 @@ def sv_marshal(c: Circuit) -> "N.OpBuffer":
 
 
 def sv_evolve(c: Circuit, state: Optional[np.ndarray] = None) -> np.ndarray:
-    """Evolve (sim/statevector/sim.go:82-85) on the CPU oracle."""
+    """Evolve (sim/statevector/sim.go:82-85) on the CPU oracle.  StatePrep follows sim.go:105-136:
+    a full-register copy on qubits [0..n-1], else the gate's decomposition."""
     n = c.NumQubits()
     s = zero_state(n) if state is None else np.ascontiguousarray(state, dtype=np.complex128).copy()
-    rc = sv_apply_ops(s, n, sv_marshal(c))
-    if rc:
-        raise RuntimeError(f"oracle apply failed with status {rc}")
+    buf = N.OpBuffer()
+
+    def flush():
+        rc = sv_apply_ops(s, n, buf)
+        if rc:
+            raise RuntimeError(f"oracle apply failed with status {rc}")
+
+    for op in c.Ops():
+        g = op.Gate
+        if g is None or op.ControlFlow is not None:
+            continue
+        if isinstance(g, G.StatePrepGate):
+            flush()
+            buf = N.OpBuffer()
+            if list(op.Qubits) == list(range(n)):
+                s[:] = g.Amplitudes()
+            else:
+                for dg, dq in g.Decompose(op.Qubits):
+                    if dg.Matrix() is not None:
+                        N.add_sv_gate(buf, dg, dq)
+            continue
+        N.add_sv_gate(buf, g, op.Qubits)
+    flush()
     return s
 
 

@@ -581,11 +581,49 @@ int emu_apply(double *state, int nq_internal, int nq_user, const goqu_op_t *ops,
     PlanConfig cfg;
     cfg.tile_bits = tile_bits;
     cfg.low_bits = low_bits;
-    Program P = plan_program(eops, nq_internal, cfg);
-    if (num_stages) *num_stages = (int64_t)P.stages.size();
-    rc = run(P, reinterpret_cast<cd *>(state), nq_internal, err);
-    if (rc) g_err = err;
-    return rc ? 100 : 0;
+    // stage programs between wide ops, as the device library runs them; a wide op is applied as
+    // its dense matrix over every column whose controls hold
+    cd *st = reinterpret_cast<cd *>(state);
+    if (num_stages) *num_stages = 0;
+    size_t i = 0;
+    while (i < eops.size()) {
+        size_t j = i;
+        while (j < eops.size() && eops[j].kind != EOP_WIDE) j++;
+        if (j > i) {
+            Program P = plan_program(std::vector<EOp>(eops.begin() + i, eops.begin() + j), nq_internal, cfg);
+            if (num_stages) *num_stages += (int64_t)P.stages.size();
+            if (run(P, st, nq_internal, err)) {
+                g_err = err;
+                return 100;
+            }
+        }
+        if (j < eops.size()) {
+            const EOp &e = eops[j];
+            const int k = e.k, D = 1 << k;
+            uint64_t tmask = 0;
+            for (int x : e.qw) tmask |= 1ull << x;
+            std::vector<cd> v(D), w(D);
+            for (uint64_t g = 0; g < (1ull << nq_internal); g++) {
+                if ((g & tmask) || (g & e.cmask) != e.cmask) continue;  // one column per base index
+                auto idx = [&](int r) {
+                    uint64_t x = g;
+                    for (int b = 0; b < k; b++)
+                        if (r >> (k - 1 - b) & 1) x |= 1ull << e.qw[b];
+                    return x;
+                };
+                for (int r = 0; r < D; r++) v[r] = st[idx(r)];
+                for (int r = 0; r < D; r++) {
+                    cd acc = 0;
+                    for (int c = 0; c < D; c++) acc += cd(e.m[2 * (r * D + c)], e.m[2 * (r * D + c) + 1]) * v[c];
+                    w[r] = acc;
+                }
+                for (int r = 0; r < D; r++) st[idx(r)] = w[r];
+            }
+            j++;
+        }
+        i = j;
+    }
+    return 0;
 }
 const char *emu_error(void) { return g_err.c_str(); }
 

@@ -211,3 +211,22 @@ def test_dm_sampled_indices_match_oracle(n):
     N.check(N.lib().goqu_dm_sample(dm._h, N.dptr(probs), N.dptr(u), len(u), N.u64ptr(out)))
     assert np.array_equal(out, O.sample_probs(probs, u))
     dm.Close()
+
+
+def test_dm_wide_controlled_gate():
+    """A controlled gate on 5 targets acts on rho exactly: U on the row qubits and conj(U) on the
+    column qubits, each a wide MFMA op; rho = |psi><psi| of the statevector."""
+    from scipy.stats import unitary_group
+
+    n = 7
+    rng = np.random.default_rng(75)
+    U = unitary_group.rvs(32, random_state=5)
+    b = NewBuilder("dmwide", n)
+    for q in range(n):
+        b.U3(*rng.uniform(0, 2 * np.pi, 3), q)
+    b.Apply(G.Controlled(G.Unitary("U", U.reshape(-1)), 1), 6, 0, 2, 3, 4, 5)
+    c = b.Build()
+    dm = D.New(n)
+    dm.Evolve(c)
+    psi = O.sv_evolve(c)
+    assert np.max(np.abs(dm.DensityMatrix() - np.outer(psi, psi.conj()).reshape(-1))) <= TOL

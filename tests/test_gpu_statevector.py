@@ -244,3 +244,47 @@ def test_measure_qubit_closed_forms_gpu():
             want = np.kron([1, 0], [a, b]) if outcome == 0 else np.kron([0, np.exp(0.7j)], [a, b])
             assert np.max(np.abs(sim.StateVector() - want)) < tol
             sim.Close()
+
+
+def test_stateprep_kats_and_subset_parity():
+    """StatePrep (sim.go:105-136): the reference's known answers (sim_test.go:775-872) on the
+    device, and random StatePrep on qubit subsets (the gate's RY + CNOT cascade) against the
+    oracle."""
+    from test_oracle_kats import _stateprep_kat_circuits
+
+    for c, want in _stateprep_kat_circuits():
+        sim = statevector.New(c.NumQubits())
+        sim.Evolve(c)
+        assert np.max(np.abs(sim.StateVector() - np.asarray(want))) < 1e-12, c.Name()
+        sim.Close()
+    rng = np.random.default_rng(12)
+    for k, qs in [(1, [3]), (2, [4, 1]), (3, [0, 5, 2]), (4, [6, 1, 3, 0])]:
+        v = rng.normal(size=1 << k) + 1j * rng.normal(size=1 << k)
+        v /= np.linalg.norm(v)
+        b = NewBuilder("sp", 7)
+        for q in range(7):
+            b.H(q)
+        b.Apply(G.StatePrep(v), *qs).CNOT(0, 6)
+        c = b.Build()
+        _check(c)
+
+
+@pytest.mark.parametrize("n,nt,nc", [(11, 5, 1), (12, 6, 2), (16, 9, 1), (22, 7, 3)])
+def test_wide_controlled_gates_mfma(n, nt, nc):
+    """Controlled gates with 5..9 target qubits (the reference's full controlled-matrix path,
+    kernel_controlled.go:249-292) run as FP64 MFMA contractions: against the oracle, complex128
+    and complex64."""
+    from scipy.stats import unitary_group
+
+    rng = np.random.default_rng(100 * nt + n)
+    U = unitary_group.rvs(1 << nt, random_state=int(rng.integers(1 << 30)))
+    qs = [int(x) for x in rng.choice(n, nt + nc, replace=False)]
+    b = NewBuilder("wide", n)
+    for q in range(n):
+        b.U3(*rng.uniform(0, 2 * np.pi, 3), q)
+    b.Apply(G.Controlled(G.Unitary("U", U.reshape(-1)), nc), *qs)
+    b.CNOT(0, n - 1).RZ(0.4, 2)
+    c = b.Build()
+    print(f"wide n={n} targets={nt} controls={nc}", flush=True)
+    _check(c)
+    _check(c, TOL64, N.C64)

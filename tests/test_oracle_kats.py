@@ -225,3 +225,29 @@ def test_dm_fidelity_pure():  # kernel2q_test.go:11-84 (fidelity 1 against the s
     c = NewBuilder("f", 4).H(0).CNOT(0, 1).RY(0.3, 2).CNOT(2, 3).Apply(G.RXX(0.4), 1, 3).Build()
     psi = O.sv_evolve(c)
     assert abs(O.dm_fidelity(_dm(c, 4), 4, psi) - 1) < EPS
+
+
+def _stateprep_kat_circuits():  # sim_test.go:775-872
+    from goqu_amd import gate as G
+
+    out = []
+    out.append((NewBuilder("sp1", 1).Apply(G.StatePrep([s2, s2]), 0).Build(), [s2, s2]))
+    out.append((NewBuilder("sp-bell", 2).Apply(G.StatePrep([s2, 0, 0, s2]), 0, 1).Build(), [s2, 0, 0, s2]))
+    out.append((NewBuilder("sp-decompose", 2).Apply(G.StatePrep([s2, s2]), 1).Build(), [s2, 0, s2, 0]))
+    out.append((NewBuilder("sp-one", 1).Apply(G.StatePrep([0, 1]), 0).Build(), [0, 1]))
+    return out
+
+
+def test_stateprep_kats():
+    for c, want in _stateprep_kat_circuits():
+        assert close(O.sv_evolve(c), want), c.Name()
+    # TestStatePrep_Normalized: |0> + i|1> over sqrt2 stays normalised
+    from goqu_amd import gate as G
+
+    sv = O.sv_evolve(NewBuilder("sp-arb", 2).Apply(G.StatePrep([s2, 1j * s2]), 1).Build())
+    assert abs(np.vdot(sv, sv).real - 1) < EPS
+    # the one-qubit cascade: RY(2 atan2(|a1|, |a0|)) then RZ(phase difference) (stateprep.go:185-211)
+    a = np.array([0.6, 0.8 * np.exp(0.9j)])
+    sv = O.sv_evolve(NewBuilder("sp-ph", 2).Apply(G.StatePrep(a), 0).Build())
+    assert abs(abs(sv[0]) - 0.6) < EPS and abs(abs(sv[1]) - 0.8) < EPS
+    assert abs(np.angle(sv[1] / sv[0]) - 0.9) < EPS

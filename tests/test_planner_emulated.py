@@ -255,3 +255,27 @@ def test_reset_known_answers(k):
     s, nqi = _padded(2)
     E.apply(s, 2, O.sv_marshal(c), tile_bits=nqi, low_bits=2)
     assert np.max(np.abs(s[:4] - want)) < 1e-15
+
+
+@pytest.mark.parametrize("nt,nc", [(5, 1), (6, 2), (8, 2), (9, 1)])
+def test_wide_controlled_gates(nt, nc):
+    """Controlled gates whose inner matrix acts on 5..9 targets (the reference's full controlled
+    matrix path, kernel_controlled.go:249-292, up to 10 qubits): lowered to wide ops between
+    stage programs, against the oracle."""
+    from goqu_amd import gate as G
+    from goqu_amd.circuit import NewBuilder
+    from scipy.stats import unitary_group
+
+    n = 11
+    rng = np.random.default_rng(10 * nt + nc)
+    U = unitary_group.rvs(1 << nt, random_state=int(rng.integers(1 << 30)))
+    qs = [int(x) for x in rng.choice(n, nt + nc, replace=False)]
+    b = NewBuilder("wide", n)
+    for q in range(n):
+        b.U3(*rng.uniform(0, 2 * np.pi, 3), q)
+    b.Apply(G.Controlled(G.Unitary("U", U.reshape(-1)), nc), *qs)
+    b.CNOT(0, 1).RZ(0.4, 2)
+    c = b.Build()
+    s, _ = _padded(n)
+    E.apply(s, n, O.sv_marshal(c), tile_bits=10, low_bits=3)
+    assert np.max(np.abs(s - O.sv_evolve(c))) < 1e-12
