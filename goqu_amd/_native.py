@@ -32,6 +32,13 @@ class GoquDeriv(C.Structure):
     _fields_ = [("op", C.c_int32), ("param", C.c_int32), ("scale", C.c_double), ("dmatrix", C.POINTER(C.c_double))]
 
 
+class GoquDynInstr(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("a", C.c_int32), ("b", C.c_int32), ("value", C.c_int32)]
+
+
+DYN_GATES, DYN_MEASURE, DYN_RESET, DYN_BRANCH, DYN_CASE, DYN_DEFAULT, DYN_END, DYN_WHILE, DYN_FOR = range(9)
+
+
 class GoquError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"goqu_hip error {code}: {msg}")
@@ -79,6 +86,10 @@ _SIGS = {
     "goqu_dm_adjoint": ([_P, C.POINTER(GoquOp), C.c_int32, C.POINTER(GoquDeriv), C.c_int32, C.c_int32, C.c_int32,
                          _U64, _U64, _I32, _D, C.c_int32, _D, _D], C.c_int),
     "goqu_qaoa_batch": ([C.c_int32, _I32, _I32, _I64, _I32, _I64, _D, _D, _D], C.c_int),
+    "goqu_sv_run_dynamic": ([_P, C.POINTER(GoquOp), C.c_int32, C.POINTER(GoquDynInstr), C.c_int32, _I32, C.c_int32,
+                             C.c_int32, C.c_int64, _D, C.c_int32, C.POINTER(C.c_uint8)], C.c_int),
+    "goqu_dm_run_dynamic": ([_P, C.POINTER(GoquOp), C.c_int32, C.POINTER(GoquDynInstr), C.c_int32, _I32, C.c_int32,
+                             C.c_int32, C.c_int64, _D, C.c_int32, C.POINTER(C.c_uint8)], C.c_int),
     "goqu_plan_stats": ([C.c_int32, C.c_int32, C.POINTER(GoquOp), C.c_int32, C.c_int32, _I64, _I64], C.c_int),
     "goqu_debug_inject_fault": ([C.c_int32, C.c_int32], C.c_int),
     "goqu_profile_enable": ([C.c_int32], C.c_int),

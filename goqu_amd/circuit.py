@@ -282,6 +282,12 @@ class Builder:
         if len(set(qs)) != len(qs):
             raise ValueError("builder: repeated qubit")
 
+    def _cl(self, *cs):
+        """validateClbit (builder.go): a clbit outside [0, numClbits) is an error."""
+        for c in cs:
+            if not 0 <= c < self.nc:
+                raise ValueError(f"builder: clbit {c} out of range [0, {self.nc})")
+
     def Apply(self, g: G.Gate, *qubits: int) -> "Builder":
         self._q(*qubits)
         if g.Qubits() != len(qubits):
@@ -391,6 +397,7 @@ class Builder:
 
     def If(self, clbit, value, g: G.Gate, *qubits):
         self._q(*qubits)
+        self._cl(clbit)
         self.ops.append(Operation(g, list(qubits), [], Condition(clbit, value)))
         return self
 
@@ -400,6 +407,7 @@ class Builder:
         return b
 
     def IfElseBlock(self, clbit, value, if_body, else_body=None):
+        self._cl(clbit)
         bi = self._sub()
         if_body(bi)
         bodies = [bi.ops]
@@ -414,15 +422,37 @@ class Builder:
         return self.IfElseBlock(clbit, value, fn, None)
 
     def While(self, clbit, value, body):
+        self._cl(clbit)
         b = self._sub()
         body(b)
         self.ops.append(Operation(None, [], [], None, ControlFlow("while", [b.ops], Condition(clbit, value))))
         return self
 
     def For(self, clbit, start, end, step, body):
+        """builder.For (builder.go:537-567): step 0 is an error."""
+        self._cl(clbit)
+        if step == 0:
+            raise ValueError("for loop step must be non-zero")
         b = self._sub()
         body(b)
         self.ops.append(Operation(None, [], [], None, ControlFlow("for", [b.ops], None, (start, end, step))))
+        return self
+
+    def Switch(self, clbits, cases: Dict[int, object], default_body=None):
+        """builder.Switch (builder.go:570-626): case bodies in ascending order of their values, then
+        the default body if given; the switch value reads the clbits little-endian."""
+        self._cl(*clbits)
+        vals = sorted(cases)
+        bodies = []
+        for v in vals:
+            b = self._sub()
+            cases[v](b)
+            bodies.append(b.ops)
+        if default_body is not None:
+            b = self._sub()
+            default_body(b)
+            bodies.append(b.ops)
+        self.ops.append(Operation(None, [], [], None, ControlFlow("switch", bodies, None, None, list(clbits), vals)))
         return self
 
     def Compose(self, c: Circuit, qubit_map: Optional[Dict[int, int]] = None):

@@ -75,18 +75,60 @@ struct Scratch {
     ~Scratch();
 };
 
+// Shots batched along gridDim.y: shot y's state starts y * stride amplitudes after the first;
+// a shot whose act byte is 0 is skipped by every kernel launched with the batch (act == nullptr:
+// all shots run).
+struct ShotBatch {
+    int nshots = 1;
+    uint64_t stride = 0;
+    const uint8_t *act = nullptr;
+};
+
+// A dense op on 5..9 target qubits (EOP_WIDE) staged on the device for k_wide_mfma (wide.hip):
+// dev holds the row bits, the free qubits and Re U | Im U | -Im U.
+struct WideOp {
+    void *dev = nullptr;
+    int k = 0, nfree = 0;
+    uint64_t cmask = 0, ncols = 0;
+    size_t off_f = 0, off_m = 0;
+    void release();
+};
+int prepare_wide(int nq, const EOp &e, hipStream_t s, WideOp &w);
+int launch_wide(void *amps, int prec, const WideOp &w, hipStream_t s, const ShotBatch *b);
+int run_wide(State &st, const EOp &e, hipStream_t s);  // prepare + launch + synchronise + release
+
+// An op list planned and uploaded once, launchable any number of times: stage programs between
+// wide ops.
+struct PreparedOps {
+    struct Piece {
+        bool wide = false;
+        DeviceProgram prog;
+        WideOp w;
+    };
+    std::vector<Piece> pieces;
+    int64_t stages = 0;  // stage-kernel launches per run
+    void release();
+    PreparedOps() = default;
+    PreparedOps(const PreparedOps &) = delete;
+    PreparedOps &operator=(const PreparedOps &) = delete;
+    PreparedOps(PreparedOps &&o) noexcept : pieces(std::move(o.pieces)), stages(o.stages) { o.pieces.clear(); }
+    ~PreparedOps() { release(); }
+};
+
 // ---- launchers (stream-ordered) -------------------------------------------------------------
 PlanConfig default_plan_config(int prec, int nq, bool paired);
 int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp);
-int run_program(State &st, const DeviceProgram &dp, hipStream_t s);
+int run_program(State &st, const DeviceProgram &dp, hipStream_t s, const ShotBatch *b = nullptr);
 // plan (with cfg, or the default configuration) + upload + run
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg = nullptr);
+// plan + upload only; st gives the qubit count and precision
+int prepare_eops(const State &st, const std::vector<EOp> &eops, const PlanConfig &cfg, hipStream_t s,
+                 PreparedOps &out);
+// launches a prepared list (no synchronisation); b batches shots as above
+int launch_prepared(State &st, const PreparedOps &p, hipStream_t s, const ShotBatch *b = nullptr);
 // ops applied to psi and lam together; EOP_GRAD ops accumulate Re <lam| D |psi> per slot
 int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, std::vector<double> &slot_vals,
              hipStream_t s);
-
-// a dense op on 5..9 target qubits (EOP_WIDE) as FP64 MFMA contractions (wide.hip)
-int run_wide(State &st, const EOp &e, Scratch &scr, hipStream_t s);
 
 int fill_basis(State &st, uint64_t index, hipStream_t s);  // |index>
 int copy_state(State &dst, const State &src, hipStream_t s);

@@ -1029,20 +1029,23 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                               uint64_t ntiles) {
-    stage_body<R, false, false, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+                                               uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride) {
+    if (act && !act[blockIdx.y]) return;  // a shot that this program does not reach
+    stage_body<R, false, false, false>(st + blockIdx.y * shot_stride, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
 }
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage_big(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                                   uint64_t ntiles) {
-    stage_body<R, false, true, false>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+                                                   uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride) {
+    if (act && !act[blockIdx.y]) return;  // a shot that this program does not reach
+    stage_body<R, false, true, false>(st + blockIdx.y * shot_stride, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
 }
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage_dm(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                   const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                                  uint64_t ntiles) {
-    stage_body<R, false, true, true>(st, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+                                                  uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride) {
+    if (act && !act[blockIdx.y]) return;  // a shot that this program does not reach
+    stage_body<R, false, true, true>(st + blockIdx.y * shot_stride, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
 }
 
 template <typename R>
@@ -1059,7 +1062,8 @@ __global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<
 // it: the host code below, which refers to the kernels as extern instantiations.
 #define GOQU_STAGE_INST(EXT, K, R)                                                                \
     EXT template __global__ void K<R>(cx<R> *__restrict__, const StageHeader *__restrict__,          \
-                                      const Instr *__restrict__, const R *__restrict__, uint64_t);
+                                      const Instr *__restrict__, const R *__restrict__, uint64_t,      \
+                                      const uint8_t *__restrict__, uint64_t);
 #define GOQU_PAIR_INST(EXT, R)                                                                     \
     EXT template __global__ void k_stage_pair<R>(cx<R> *__restrict__, cx<R> *__restrict__,           \
                                                  const StageHeader *__restrict__, const Instr *__restrict__, \
@@ -1173,12 +1177,15 @@ static bool stage_is_identity(const StageHeader &H) {
 }
 
 template <typename R>
-static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
+static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, const ShotBatch *b) {
     const unsigned char *blob = static_cast<const unsigned char *>(dp.blob);
     const StageHeader *hdrs = reinterpret_cast<const StageHeader *>(blob + dp.hdr_off);
     const Instr *instrs = reinterpret_cast<const Instr *>(blob + dp.instr_off);
     const R *pool = reinterpret_cast<const R *>(blob + dp.pool_off);
     const bool prof = g_profile.load();
+    const unsigned nshots = b ? (unsigned)b->nshots : 1u;
+    const uint8_t *act = b ? b->act : nullptr;
+    const uint64_t stride = b ? b->stride : 0;
     std::vector<hipEvent_t> ev;
     for (size_t i = 0; i < dp.stages.size(); i++) {
         const int m = dp.stages[i].m;
@@ -1196,18 +1203,19 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
             GOQU_HIP(hipEventCreate(&ev[ev.size() - 1]));
             GOQU_HIP(hipEventRecord(ev[ev.size() - 2], s));
         }
+        const dim3 g(grid, nshots);
         if (dp.stages[i].flags & SF_DM) {
             if constexpr (sizeof(R) == 8)
-                hipLaunchKernelGGL(k_stage_dm<double>, dim3(grid), dim3(threads), lds, s,
-                                   static_cast<cx<double> *>(st.amps), hdrs + i, instrs, pool, ntiles);
+                hipLaunchKernelGGL(k_stage_dm<double>, g, dim3(threads), lds, s, static_cast<cx<double> *>(st.amps),
+                                   hdrs + i, instrs, pool, ntiles, act, stride);
             else
                 return fail(GOQU_ERR_UNSUPPORTED, "density-matrix stages run in complex128 only");
         } else if (dp.stages[i].flags & SF_BIG)
-            hipLaunchKernelGGL(k_stage_big<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(st.amps),
-                               hdrs + i, instrs, pool, ntiles);
+            hipLaunchKernelGGL(k_stage_big<R>, g, dim3(threads), lds, s, static_cast<cx<R> *>(st.amps), hdrs + i,
+                               instrs, pool, ntiles, act, stride);
         else
-            hipLaunchKernelGGL(k_stage<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(st.amps),
-                               hdrs + i, instrs, pool, ntiles);
+            hipLaunchKernelGGL(k_stage<R>, g, dim3(threads), lds, s, static_cast<cx<R> *>(st.amps), hdrs + i, instrs,
+                               pool, ntiles, act, stride);
         GOQU_HIP(hipGetLastError());
         if (prof) GOQU_HIP(hipEventRecord(ev.back(), s));
     }
@@ -1221,13 +1229,14 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s) {
             (void)hipEventDestroy(ev[i]);
             (void)hipEventDestroy(ev[i + 1]);
         }
-        profile_add(ms_total, (int64_t)(ev.size() / 2), (double)(ev.size() / 2) * 2.0 * st.size() * sizeof(cx<R>));
+        profile_add(ms_total, (int64_t)(ev.size() / 2),
+                    (double)(ev.size() / 2) * 2.0 * st.size() * sizeof(cx<R>) * nshots);
     }
     return GOQU_OK;
 }
 
-int run_program(State &st, const DeviceProgram &dp, hipStream_t s) {
-    return st.prec == GOQU_C128 ? run_program_t<double>(st, dp, s) : run_program_t<float>(st, dp, s);
+int run_program(State &st, const DeviceProgram &dp, hipStream_t s, const ShotBatch *b) {
+    return st.prec == GOQU_C128 ? run_program_t<double>(st, dp, s, b) : run_program_t<float>(st, dp, s, b);
 }
 
 // sums[slot] = sum over b of part[slot * nb + b], in a fixed order
@@ -1327,7 +1336,6 @@ static int run_segment(State &st, const std::vector<EOp> &eops, hipStream_t s, c
 
 // Stage programs between wide ops (dense on 5..9 targets), which run as MFMA contractions.
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg) {
-    static thread_local Scratch wscr;
     bool wide = false;
     for (const EOp &e : eops) wide = wide || e.kind == EOP_WIDE;
     if (!wide) return run_segment(st, eops, s, cfg);
@@ -1340,11 +1348,55 @@ int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanC
             if (rc) return rc;
         }
         if (j < eops.size()) {
-            int rc = run_wide(st, eops[j], wscr, s);
+            int rc = run_wide(st, eops[j], s);
             if (rc) return rc;
             j++;
         }
         i = j;
+    }
+    return GOQU_OK;
+}
+
+void PreparedOps::release() {
+    for (Piece &p : pieces) {
+        if (p.prog.blob) (void)hipFree(p.prog.blob);
+        p.prog.blob = nullptr;
+        p.w.release();
+    }
+    pieces.clear();
+    stages = 0;
+}
+
+int prepare_eops(const State &st, const std::vector<EOp> &eops, const PlanConfig &cfg, hipStream_t s,
+                 PreparedOps &out) {
+    out.release();
+    size_t i = 0;
+    while (i < eops.size()) {
+        size_t j = i;
+        while (j < eops.size() && eops[j].kind != EOP_WIDE) j++;
+        if (j > i) {
+            Program P = plan_program(std::vector<EOp>(eops.begin() + i, eops.begin() + j), st.nq, cfg);
+            out.pieces.emplace_back();
+            int rc = upload_program(P, st.prec, s, out.pieces.back().prog);
+            if (rc) return rc;
+            for (const StageHeader &H : P.stages) out.stages += stage_is_identity(H) ? 0 : 1;
+        }
+        if (j < eops.size()) {
+            out.pieces.emplace_back();
+            out.pieces.back().wide = true;
+            int rc = prepare_wide(st.nq, eops[j], s, out.pieces.back().w);
+            if (rc) return rc;
+            j++;
+        }
+        i = j;
+    }
+    return GOQU_OK;
+}
+
+int launch_prepared(State &st, const PreparedOps &p, hipStream_t s, const ShotBatch *b) {
+    for (const PreparedOps::Piece &pc : p.pieces) {
+        int rc = pc.wide ? launch_wide(st.amps, st.prec, pc.w, s, b) : run_program(st, pc.prog, s, b);
+        if (rc) return rc;
     }
     return GOQU_OK;
 }

@@ -26,7 +26,10 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_wide_mfma(cx<R> *__restrict__ st, int k, const uint64_t *__restrict__ rowbit,
                                                    const uint8_t *__restrict__ freeq, int nfree, uint64_t cmask,
                                                    uint64_t ncols, const double *__restrict__ Ur,
-                                                   const double *__restrict__ Ui, const double *__restrict__ Uin) {
+                                                   const double *__restrict__ Ui, const double *__restrict__ Uin,
+                                                   const uint8_t *__restrict__ act, uint64_t shot_stride) {
+    if (act && !act[blockIdx.y]) return;  // a shot that this op does not reach
+    st += blockIdx.y * shot_stride;
     extern __shared__ double lds[];
     const int D = 1 << k;
     double *Br = lds, *Bi = lds + (size_t)D * 16;
@@ -89,7 +92,7 @@ __global__ void __launch_bounds__(256) k_wide_mfma(cx<R> *__restrict__ st, int k
     }
 }
 
-int run_wide(State &st, const EOp &e, Scratch &scr, hipStream_t s) {
+int prepare_wide(int nq, const EOp &e, hipStream_t s, WideOp &w) {
     const int k = e.k;
     if (e.kind != EOP_WIDE || k < 5 || k > kMaxWideTargets || (int)e.qw.size() != k)
         return fail(GOQU_ERR_INVALID_ARG, "malformed wide op");
@@ -99,15 +102,14 @@ int run_wide(State &st, const EOp &e, Scratch &scr, hipStream_t s) {
     uint64_t used = e.cmask;
     for (int b = 0; b < k; b++) {
         const int q = e.qw[k - 1 - b];
-        if (q < 0 || q >= st.nq) return fail(GOQU_ERR_INVALID_ARG, "wide op qubit out of range");
+        if (q < 0 || q >= nq) return fail(GOQU_ERR_INVALID_ARG, "wide op qubit out of range");
         rowbit[b] = 1ull << q;
         used |= rowbit[b];
     }
-    if (e.cmask >> st.nq) return fail(GOQU_ERR_INVALID_ARG, "wide op control out of range");
+    if (e.cmask >> nq) return fail(GOQU_ERR_INVALID_ARG, "wide op control out of range");
     std::vector<uint8_t> freeq;
-    for (int q = 0; q < st.nq; q++)
+    for (int q = 0; q < nq; q++)
         if (!(used >> q & 1)) freeq.push_back((uint8_t)q);
-    const uint64_t ncols = 1ull << freeq.size();
     // device staging: row bits | free qubits | Ur | Ui | -Ui
     const size_t mat = (size_t)D * D;
     const size_t off_f = (size_t)k * 8, off_m = (off_f + 64 + 255) & ~size_t(255);
@@ -121,25 +123,54 @@ int run_wide(State &st, const EOp &e, Scratch &scr, hipStream_t s) {
         ui[i] = e.m[2 * i + 1];
         uin[i] = -e.m[2 * i + 1];
     }
-    int rc = scr.ensure(bytes);
-    if (rc) return rc;
-    char *dev = static_cast<char *>(scr.ptr);
-    GOQU_HIP(hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, s));
-    const double *dur = reinterpret_cast<const double *>(dev + off_m);
-    const uint64_t ngroups = (ncols + 15) / 16;
-    const unsigned grid = (unsigned)std::min<uint64_t>(ngroups, 1u << 20);
-    const size_t lds = (size_t)D * 16 * 2 * sizeof(double);
-    if (st.prec == GOQU_C128)
-        hipLaunchKernelGGL(k_wide_mfma<double>, dim3(grid), dim3(256), lds, s, static_cast<cx<double> *>(st.amps), k,
-                           reinterpret_cast<const uint64_t *>(dev), reinterpret_cast<const uint8_t *>(dev + off_f),
-                           (int)freeq.size(), e.cmask, ncols, dur, dur + mat, dur + 2 * mat);
-    else
-        hipLaunchKernelGGL(k_wide_mfma<float>, dim3(grid), dim3(256), lds, s, static_cast<cx<float> *>(st.amps), k,
-                           reinterpret_cast<const uint64_t *>(dev), reinterpret_cast<const uint8_t *>(dev + off_f),
-                           (int)freeq.size(), e.cmask, ncols, dur, dur + mat, dur + 2 * mat);
-    GOQU_HIP(hipGetLastError());
+    w.release();
+    GOQU_HIP(dmalloc(&w.dev, bytes));
+    w.k = k;
+    w.nfree = (int)freeq.size();
+    w.cmask = e.cmask;
+    w.ncols = 1ull << freeq.size();
+    w.off_f = off_f;
+    w.off_m = off_m;
+    GOQU_HIP(hipMemcpyAsync(w.dev, host.data(), bytes, hipMemcpyHostToDevice, s));
     GOQU_HIP(hipStreamSynchronize(s));  // the host staging buffer is released on return
     return GOQU_OK;
+}
+
+void WideOp::release() {
+    if (dev) (void)hipFree(dev);
+    dev = nullptr;
+}
+
+int launch_wide(void *amps, int prec, const WideOp &w, hipStream_t s, const ShotBatch *b) {
+    const int D = 1 << w.k;
+    const size_t mat = (size_t)D * D;
+    const char *dev = static_cast<const char *>(w.dev);
+    const double *dur = reinterpret_cast<const double *>(dev + w.off_m);
+    const uint64_t ngroups = (w.ncols + 15) / 16;
+    const unsigned grid = (unsigned)std::min<uint64_t>(ngroups, 1u << 20);
+    const unsigned shots = b ? (unsigned)b->nshots : 1u;
+    const uint8_t *act = b ? b->act : nullptr;
+    const uint64_t stride = b ? b->stride : 0;
+    const size_t lds = (size_t)D * 16 * 2 * sizeof(double);
+    const uint64_t *rb = reinterpret_cast<const uint64_t *>(dev);
+    const uint8_t *fq = reinterpret_cast<const uint8_t *>(dev + w.off_f);
+    if (prec == GOQU_C128)
+        hipLaunchKernelGGL(k_wide_mfma<double>, dim3(grid, shots), dim3(256), lds, s, static_cast<cx<double> *>(amps),
+                           w.k, rb, fq, w.nfree, w.cmask, w.ncols, dur, dur + mat, dur + 2 * mat, act, stride);
+    else
+        hipLaunchKernelGGL(k_wide_mfma<float>, dim3(grid, shots), dim3(256), lds, s, static_cast<cx<float> *>(amps),
+                           w.k, rb, fq, w.nfree, w.cmask, w.ncols, dur, dur + mat, dur + 2 * mat, act, stride);
+    GOQU_HIP(hipGetLastError());
+    return GOQU_OK;
+}
+
+int run_wide(State &st, const EOp &e, hipStream_t s) {
+    WideOp w;
+    int rc = prepare_wide(st.nq, e, s, w);
+    if (!rc) rc = launch_wide(st.amps, st.prec, w, s, nullptr);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "wide op failed");
+    w.release();
+    return rc;
 }
 
 }  // namespace goqu

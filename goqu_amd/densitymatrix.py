@@ -16,6 +16,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from . import _native as N
+from . import dynamic as DY
 from . import gate as G
 from . import noise as NZ
 from .circuit import Circuit, Operation
@@ -178,63 +179,33 @@ class Sim:
         self.ApplyOps(buf)
         return outcome
 
-    def RunDynamic(self, c: Circuit, shots: int, rng=None) -> Dict[str, int]:
+    def DynamicProgram(self, c: Circuit) -> "DY.DynamicProgram":
+        """The circuit as a device program (dynamic.go's executeOps cases, flattened): gates with
+        their noise channels, delay as its channel, the "reset" channel after each reset."""
+        nm = self.noise
+
+        def gate(buf, op):
+            _marshal_op(buf, op, nm)
+
+        def reset_tail(buf, op):
+            if nm is not None:
+                ch = nm.Lookup("reset", op.Qubits)
+                if ch is not None:
+                    _add_channel(buf, ch, op.Qubits)
+
+        return DY.flatten(c, gate, emit_delay=gate, emit_reset_tail=reset_tail)
+
+    def RunDynamicShots(self, c: Circuit, shots: int, rng=None, uniforms=None) -> np.ndarray:
+        """All shots batched on the device; each shot's clbits, one row per shot."""
         if c.NumQubits() != self.num_qubits:
             raise ValueError(f"circuit has {c.NumQubits()} qubits, simulator has {self.num_qubits}")
         if shots <= 0:
             raise ValueError(f"shots must be positive, got {shots}")
-        rng = rng if rng is not None else np.random.default_rng()
-        counts: Dict[str, int] = {}
-        nc = c.NumClbits()
-        for _ in range(shots):
-            self.Reset()
-            clbits = [0] * nc
-            self._exec(c.Ops(), clbits, rng)
-            bs = "".join("1" if clbits[nc - 1 - i] else "0" for i in range(nc))
-            counts[bs] = counts.get(bs, 0) + 1
-        return counts
+        return DY.run(N.lib().goqu_dm_run_dynamic, self._h, self.DynamicProgram(c), shots, rng, uniforms)
 
-    def _exec(self, ops: List[Operation], clbits: List[int], rng) -> None:
-        for op in ops:
-            cf = op.ControlFlow
-            if cf is not None:
-                if cf.Type == "while":
-                    from .circuit import MaxControlFlowIterations
-
-                    for _ in range(MaxControlFlowIterations):
-                        if clbits[cf.Condition.Clbit] != cf.Condition.Value:
-                            break
-                        self._exec(cf.Bodies[0], clbits, rng)
-                elif cf.Type == "for":
-                    start, end, step = cf.ForRange
-                    for _ in range(start, end, step):
-                        self._exec(cf.Bodies[0], clbits, rng)
-                elif cf.Type == "if_else":
-                    if clbits[cf.Condition.Clbit] == cf.Condition.Value:
-                        self._exec(cf.Bodies[0], clbits, rng)
-                    elif len(cf.Bodies) > 1:
-                        self._exec(cf.Bodies[1], clbits, rng)
-                continue
-            g = op.Gate
-            if g is None:
-                if op.Clbits and op.Qubits:
-                    clbits[op.Clbits[0]] = self.MeasureQubit(op.Qubits[0], rng.random())
-                continue
-            if g.Name() == "reset":  # measure, X on outcome 1, then the "reset" channel
-                buf = N.OpBuffer()
-                if self.MeasureQubit(op.Qubits[0], rng.random()) == 1:
-                    buf.add(N.OP_GATE, op.Qubits[:1], G.X.Matrix())
-                if self.noise is not None:
-                    ch = self.noise.Lookup("reset", op.Qubits)
-                    if ch is not None:
-                        _add_channel(buf, ch, op.Qubits)
-                self.ApplyOps(buf)
-                continue
-            if op.Condition is not None and clbits[op.Condition.Clbit] != op.Condition.Value:
-                continue
-            buf = N.OpBuffer()
-            _marshal_op(buf, op, self.noise)
-            self.ApplyOps(buf)
+    def RunDynamic(self, c: Circuit, shots: int, rng=None, uniforms=None) -> Dict[str, int]:
+        """RunDynamic (dynamic.go:13-51)."""
+        return DY.counts_of(self.RunDynamicShots(c, shots, rng, uniforms))
 
     def Run(self, c: Circuit, shots: int, uniforms=None, rng=None) -> Dict[str, int]:
         """Run (sim.go:51-67): dynamic circuits go to RunDynamic; otherwise evolve, apply the

@@ -6,6 +6,8 @@ Differences from the CPU reference, by design:
     panics above 28 (sim.go:28-30).
   * Sampling takes its uniforms from a caller-supplied array or numpy Generator, so runs are
     reproducible; the reference draws from an unseeded PCG per Run (sim.go:51).
+  * RunDynamic runs all shots together on the device (goqu_amd.dynamic); shot s draws from its
+    own row of uniforms, as the reference draws from a fresh generator per shot (dynamic.go:26).
 """
 from __future__ import annotations
 
@@ -15,8 +17,9 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 
 from . import _native as N
+from . import dynamic as D
 from . import gate as G
-from .circuit import Circuit, Operation, MaxControlFlowIterations
+from .circuit import Circuit
 from .pauli import PauliString, PauliSum, masks
 
 
@@ -158,64 +161,22 @@ class Sim:
         return out.value
 
     # -- dynamic circuits (dynamic.go:14-230) --
-    def RunDynamic(self, c: Circuit, shots: int, rng=None) -> Dict[str, int]:
+    def DynamicProgram(self, c: Circuit) -> "D.DynamicProgram":
+        """The circuit as a device program (dynamic.go's executeOps cases, flattened)."""
+        return D.flatten(c, lambda buf, op: N.add_sv_gate(buf, op.Gate, op.Qubits))
+
+    def RunDynamicShots(self, c: Circuit, shots: int, rng=None, uniforms=None) -> np.ndarray:
+        """All shots batched on the device; returns each shot's clbits (one row per shot).  Shot s
+        takes its uniforms in order from row s of `uniforms` when given."""
         if c.NumQubits() != self.num_qubits:
             raise ValueError(f"circuit has {c.NumQubits()} qubits, simulator has {self.num_qubits}")
         if shots <= 0:
             raise ValueError(f"shots must be positive, got {shots}")
-        rng = rng if rng is not None else np.random.default_rng()
-        counts: Dict[str, int] = {}
-        nc = c.NumClbits()
-        for _ in range(shots):
-            self.Reset()
-            clbits = [0] * nc
-            self._exec(c.Ops(), clbits, rng)
-            bs = "".join("1" if clbits[nc - 1 - i] else "0" for i in range(nc))
-            counts[bs] = counts.get(bs, 0) + 1
-        return counts
+        return D.run(N.lib().goqu_sv_run_dynamic, self._h, self.DynamicProgram(c), shots, rng, uniforms)
 
-    def _apply_one(self, g: G.Gate, qubits: Sequence[int]) -> None:
-        buf = N.OpBuffer()
-        N.add_sv_gate(buf, g, qubits)
-        self._flush(buf)
-
-    def _exec(self, ops: List[Operation], clbits: List[int], rng) -> None:
-        for op in ops:
-            cf = op.ControlFlow
-            if cf is not None:
-                self._control_flow(cf, clbits, rng)
-                continue
-            g = op.Gate
-            if g is None:
-                if op.Clbits and op.Qubits:
-                    clbits[op.Clbits[0]] = self.MeasureQubit(op.Qubits[0], rng.random())
-                continue
-            name = g.Name()
-            if name in ("barrier", "delay"):
-                continue
-            if name == "reset":  # measure, then X on outcome 1 (dynamic.go:79-85)
-                if self.MeasureQubit(op.Qubits[0], rng.random()) == 1:
-                    self._apply_one(G.X, op.Qubits)
-                continue
-            if op.Condition is not None and clbits[op.Condition.Clbit] != op.Condition.Value:
-                continue
-            self._apply_one(g, op.Qubits)
-
-    def _control_flow(self, cf, clbits, rng) -> None:
-        if cf.Type == "while":
-            for _ in range(MaxControlFlowIterations):
-                if clbits[cf.Condition.Clbit] != cf.Condition.Value:
-                    break
-                self._exec(cf.Bodies[0], clbits, rng)
-        elif cf.Type == "for":
-            start, end, step = cf.ForRange
-            for _ in range(start, end, step):
-                self._exec(cf.Bodies[0], clbits, rng)
-        elif cf.Type == "if_else":
-            if clbits[cf.Condition.Clbit] == cf.Condition.Value:
-                self._exec(cf.Bodies[0], clbits, rng)
-            elif len(cf.Bodies) > 1:
-                self._exec(cf.Bodies[1], clbits, rng)
+    def RunDynamic(self, c: Circuit, shots: int, rng=None, uniforms=None) -> Dict[str, int]:
+        """RunDynamic (dynamic.go:14-52): counts of the clbit strings, clbit n_c-1 leftmost."""
+        return D.counts_of(self.RunDynamicShots(c, shots, rng, uniforms))
 
     # -- observables --
     def _check_pauli(self, nq: int, what: str) -> None:

@@ -194,6 +194,43 @@ int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits, const int3
                     const int64_t *param_offsets, const double *params, double *energies,
                     double *grads);
 
+/* ---- dynamic circuits with the shots batched on the device (replaces RunDynamic /
+ *      executeOps / executeControlFlow / measureQubit, sim/statevector/dynamic.go:14-230 and
+ *      sim/densitymatrix/dynamic.go:13-255) ------------------------------------------------ */
+/* A dynamic circuit is a flat program of instructions over an op array (goqu_op_t, as for Apply;
+ * for the density matrix with the noise channels already resolved, as for goqu_dm_apply).
+ * Control flow is structured: BRANCH, WHILE and FOR each open a group that END closes. */
+#define GOQU_DYN_GATES 0   /* ops[a .. a+b) on every shot that reaches the instruction         */
+#define GOQU_DYN_MEASURE 1 /* measureQubit(a) into clbit b: one uniform u, outcome = (u >= P0) */
+#define GOQU_DYN_RESET 2   /* measure qubit a (one uniform), then X on outcome 1               */
+#define GOQU_DYN_BRANCH 3  /* opens an if/else or switch group of CASE / DEFAULT bodies        */
+#define GOQU_DYN_CASE 4    /* body for shots whose clbits cond_bits[a .. a+b) read, little-endian,
+                              as `value` and that took no earlier CASE of the group            */
+#define GOQU_DYN_DEFAULT 5 /* body for shots that took no CASE (the group's last body)         */
+#define GOQU_DYN_END 6     /* closes BRANCH, WHILE or FOR                                       */
+#define GOQU_DYN_WHILE 7   /* body repeats while cond_bits[a .. a+b) read `value`, at most 1000
+                              times (ir.MaxControlFlowIterations, circuit/ir/circuit.go:129)   */
+#define GOQU_DYN_FOR 8     /* body repeats a times                                              */
+typedef struct goqu_dyn_instr {
+    int32_t kind; /* GOQU_DYN_* */
+    int32_t a;
+    int32_t b;
+    int32_t value;
+} goqu_dyn_instr_t;
+
+/* RunDynamic: every shot starts from |0...0> (|0><0|), runs the program and leaves its clbits in
+ * clbits_out[shot * num_clbits + c] (0 or 1).  Shot s draws its uniforms in order from
+ * uniforms[s * draws_per_shot ...] (the reference draws from a fresh generator per shot); a shot
+ * that needs more returns GOQU_ERR_INVALID_ARG.  Shots run in batches side by side on the device;
+ * the handle is left holding the last shot's final state.  P0 is an FP64 sum of the same terms as
+ * the reference's in another order, so outcomes agree except where u - P0 is within rounding. */
+int goqu_sv_run_dynamic(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops, const goqu_dyn_instr_t *prog,
+                        int32_t num_instr, const int32_t *cond_bits, int32_t num_cond_bits, int32_t num_clbits,
+                        int64_t shots, const double *uniforms, int32_t draws_per_shot, uint8_t *clbits_out);
+int goqu_dm_run_dynamic(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops, const goqu_dyn_instr_t *prog,
+                        int32_t num_instr, const int32_t *cond_bits, int32_t num_cond_bits, int32_t num_clbits,
+                        int64_t shots, const double *uniforms, int32_t draws_per_shot, uint8_t *clbits_out);
+
 /* ---- host-side planning and test hooks (no counterpart in the reference) ------------------ */
 /* Lowers and plans an op list on the host only (no device is touched): *stages receives the
  * number of state sweeps (stage-kernel launches) and *instrs the instruction count.  density != 0

@@ -591,6 +591,31 @@ static void dm_reset_qubit(cx *rho, int nq, int q) {
             if (((r >> q) & 1) || ((c >> q) & 1)) rho[r * dim + c] = C(0, 0);
 }
 
+/* measureQubitDM — sim/densitymatrix/dynamic.go:211-255 */
+int orc_dm_measure_qubit(double *rho_d, int nq, int q, double r) {
+    cx *rho = (cx *)rho_d;
+    const int64_t dim = 1LL << nq, mask = 1LL << q;
+    double prob0 = 0.0;
+    for (int64_t i = 0; i < dim; i++)
+        if ((i & mask) == 0) prob0 += rho[i * dim + i].re;
+    int outcome = (r >= prob0) ? 1 : 0;
+    double p = outcome == 0 ? prob0 : 1 - prob0;
+    if (p < 1e-15) return outcome;
+    cx scale = C(1.0 / p, 0);
+    for (int64_t rr = 0; rr < dim; rr++) {
+        int rb = (int)((rr >> q) & 1);
+        for (int64_t c = 0; c < dim; c++) {
+            int cb = (int)((c >> q) & 1);
+            int64_t idx = rr * dim + c;
+            if (rb == outcome && cb == outcome)
+                rho[idx] = cmul(rho[idx], scale);
+            else
+                rho[idx] = C(0, 0);
+        }
+    }
+    return outcome;
+}
+
 /* applyChannel — sim/densitymatrix/sim.go:220-255 */
 static void dm_channel(cx *rho, int nq, const goqu_op_t *op) {
     const int k = op->num_qubits, d = 1 << k;

@@ -42,6 +42,8 @@ def lib():
         L.orc_sv_expectation_value.restype = C.c_double
         L.orc_sv_measure_qubit.argtypes = [_D, C.c_int, C.c_int, C.c_double]
         L.orc_sv_measure_qubit.restype = C.c_int
+        L.orc_dm_measure_qubit.argtypes = [_D, C.c_int, C.c_int, C.c_double]
+        L.orc_dm_measure_qubit.restype = C.c_int
         L.orc_dm_apply.argtypes = [_D, C.c_int, C.POINTER(N.GoquOp), C.c_int]
         L.orc_dm_apply.restype = C.c_int
         L.orc_dm_diag_probs.argtypes = [_D, C.c_int, _D]
@@ -163,6 +165,10 @@ def dm_apply_ops(rho: np.ndarray, n: int, buf: "N.OpBuffer") -> int:
     if not len(buf):
         return 0
     return lib().orc_dm_apply(_dp(rho), n, buf.array(), len(buf))
+
+
+def dm_measure_qubit(rho: np.ndarray, n: int, q: int, r: float) -> int:
+    return lib().orc_dm_measure_qubit(_dp(rho), n, q, r)
 
 
 def dm_probs(rho: np.ndarray, n: int, p01=None, p10=None) -> np.ndarray:

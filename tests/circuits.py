@@ -239,3 +239,57 @@ def random_pauli_sum(n: int, terms: int, rng, wmax: int = 4):
         ops = {int(q): [X, Y, Z][rng.integers(3)] for q in rng.choice(n, w, replace=False)}
         out.append(NewPauliString(rng.normal(), ops, n))
     return NewPauliSum(out)
+
+
+def random_dynamic_circuit(n: int, nclbits: int, depth: int, rng, density: bool = False):
+    """Mid-circuit measurements, resets, conditioned gates and every control-flow form (if/else,
+    switch with and without a default, for, while loops that end with probability 1/2 per
+    iteration), nested up to two deep, between layers of random gates.  density=True keeps to
+    1- and 2-qubit gates and adds delays (noise hooks)."""
+    b = NewBuilder("dynamic", n).WithClbits(nclbits)
+
+    def gates(bb, k):
+        for _ in range(k):
+            if n >= 2 and rng.random() < 0.4:
+                qs = rng.choice(n, 2, replace=False)
+                bb.Apply(rand_2q(rng), int(qs[0]), int(qs[1]))
+            else:
+                bb.Apply(rand_1q(rng), int(rng.integers(n)))
+            if density and rng.random() < 0.2:
+                bb.Delay(int(rng.integers(n)), 1.0)
+
+    def measure(bb):
+        bb.Measure(int(rng.integers(n)), int(rng.integers(nclbits)))
+
+    def block(bb, level):
+        k = int(rng.integers(9))
+        c0 = int(rng.integers(nclbits))
+        if k == 0:
+            measure(bb)
+        elif k == 1:
+            bb.Reset(int(rng.integers(n)))
+        elif k == 2:
+            bb.If(c0, int(rng.integers(2)), rand_1q(rng), int(rng.integers(n)))
+        elif k == 3 and level < 2:
+            bb.IfElseBlock(c0, int(rng.integers(2)), lambda x: (gates(x, 2), block(x, level + 1)),
+                           (lambda x: (gates(x, 1), measure(x))) if rng.random() < 0.7 else None)
+        elif k == 4 and level < 2 and nclbits >= 2:
+            bits = [int(x) for x in rng.choice(nclbits, 2, replace=False)]
+            vals = [int(v) for v in rng.choice(4, int(rng.integers(1, 4)), replace=False)]
+            cases = {v: (lambda x, v=v: (gates(x, 1 + v % 2), block(x, level + 1))) for v in vals}
+            bb.Switch(bits, cases, (lambda x: gates(x, 2)) if rng.random() < 0.5 else None)
+        elif k == 5 and level < 2:
+            bb.For(c0, 0, int(rng.integers(1, 4)), 1, lambda x: (gates(x, 1), block(x, level + 1)))
+        elif k == 6 and level < 2:
+            q = int(rng.integers(n))
+            bb.Measure(q, c0)  # loop while the clbit reads 1: H then measure again
+            bb.While(c0, 1, lambda x: (x.H(q), x.Measure(q, c0), gates(x, 1)))
+        else:
+            gates(bb, 2)
+
+    for _ in range(depth):
+        gates(b, n)
+        block(b, 0)
+    for q in range(min(n, nclbits)):
+        b.Measure(q, q)
+    return b.Build()
