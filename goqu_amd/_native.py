@@ -90,6 +90,13 @@ _SIGS = {
                              C.c_int32, C.c_int64, _D, C.c_int32, C.POINTER(C.c_uint8)], C.c_int),
     "goqu_dm_run_dynamic": ([_P, C.POINTER(GoquOp), C.c_int32, C.POINTER(GoquDynInstr), C.c_int32, _I32, C.c_int32,
                              C.c_int32, C.c_int64, _D, C.c_int32, C.POINTER(C.c_uint8)], C.c_int),
+    "goqu_sv_batch_expect": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(GoquOp), _I64,
+                              C.c_int32, _U64, _U64, _I32, _D, _D], C.c_int),
+    "goqu_sv_batch_sample": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(GoquOp), _I64,
+                              C.c_int64, _D, _U64], C.c_int),
+    "goqu_dm_batch_expect": ([C.c_int32, C.c_int32, C.POINTER(GoquOp), _I64, C.c_int32, _U64, _U64, _I32, _D, _D],
+                             C.c_int),
+    "goqu_dm_batch_sample": ([C.c_int32, C.c_int32, C.POINTER(GoquOp), _I64, _D, _D, C.c_int64, _D, _U64], C.c_int),
     "goqu_plan_stats": ([C.c_int32, C.c_int32, C.POINTER(GoquOp), C.c_int32, C.c_int32, _I64, _I64], C.c_int),
     "goqu_debug_inject_fault": ([C.c_int32, C.c_int32], C.c_int),
     "goqu_profile_enable": ([C.c_int32], C.c_int),

@@ -47,12 +47,6 @@ __device__ __forceinline__ double block_total(double v, double *sh) {
     return t;
 }
 
-template <typename R>
-__global__ void k_dyn_init(cx<R> *__restrict__ a, uint64_t n, uint64_t stride) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        a[i] = cx<R>{(i & (stride - 1)) == 0 ? R(1) : R(0), R(0)};
-}
-
 __global__ void k_dyn_fill(uint8_t *__restrict__ p, uint64_t n, uint8_t v) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -556,13 +550,8 @@ int run_dynamic(bool density, int n, State &handle_state, hipStream_t s, const P
     for (int64_t s0 = 0; s0 < shots; s0 += B) {
         const int nbatch = (int)std::min<int64_t>(B, shots - s0);
         R.nshots = nbatch;
-        const uint64_t total = (uint64_t)nbatch * stride;
-        if (handle_state.prec == GOQU_C128)
-            hipLaunchKernelGGL(k_dyn_init<double>, dim3(std::min<uint64_t>(blocks_for(total, 256), 8192)), dim3(256),
-                               0, s, static_cast<cx<double> *>(buf.amps), total, stride);
-        else
-            hipLaunchKernelGGL(k_dyn_init<float>, dim3(std::min<uint64_t>(blocks_for(total, 256), 8192)), dim3(256),
-                               0, s, static_cast<cx<float> *>(buf.amps), total, stride);
+        rc = fill_basis_batch(R.view, nbatch, s);
+        if (rc) return rc;
         hipLaunchKernelGGL(k_dyn_fill, dim3(blocks_for((uint64_t)nbatch * R.nclbits, 256)), dim3(256), 0, s,
                            buf.clbits, (uint64_t)nbatch * R.nclbits, (uint8_t)0);
         hipLaunchKernelGGL(k_dyn_zero_int, dim3(blocks_for(nbatch, 256)), dim3(256), 0, s, buf.drawn, nbatch);

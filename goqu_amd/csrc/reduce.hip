@@ -121,11 +121,15 @@ __device__ __forceinline__ uint64_t insert_zero(uint64_t v, int pos) {
 }
 
 // x == 0: S_t = sum_i |psi_i|^2 (-1)^{|i & z_t|}; NT terms per pass (real accumulators)
+// Both Pauli kernels take states batched along gridDim.y (state y at y * shot_stride amplitudes,
+// its partials after those of states 0..y-1).
 template <typename R, int NT>
 __global__ void __launch_bounds__(kRedThreads) k_pauli_z(const cx<R> *__restrict__ st, uint64_t nbases,
                                                          const uint64_t *__restrict__ z, int nt,
-                                                         double *__restrict__ partials) {
+                                                         double *__restrict__ partials, uint64_t shot_stride) {
     __shared__ double sh[4 * NT];
+    st += blockIdx.y * shot_stride;
+    partials += (size_t)blockIdx.y * gridDim.x * NT;
     double acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = 0;
@@ -157,8 +161,10 @@ __global__ void __launch_bounds__(kRedThreads) k_pauli_z(const cx<R> *__restrict
 template <typename R, int NT>
 __global__ void __launch_bounds__(kRedThreads) k_pauli_x(const cx<R> *__restrict__ st, uint64_t nbases, uint64_t x,
                                                          int hb, const uint64_t *__restrict__ z, int nt,
-                                                         double *__restrict__ partials) {
+                                                         double *__restrict__ partials, uint64_t shot_stride) {
     __shared__ double sh[4 * 2 * NT];
+    st += blockIdx.y * shot_stride;
+    partials += (size_t)blockIdx.y * gridDim.x * 2 * NT;
     double acc[2 * NT];
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++) acc[t] = 0;
@@ -224,10 +230,10 @@ static int pauli_sums_t(State &st, uint64_t x, const uint64_t *z, int nt, double
         GOQU_HIP(hipMemcpyAsync(dz, z + t0, cnt * sizeof(uint64_t), hipMemcpyHostToDevice, s));
         if (x == 0)
             hipLaunchKernelGGL((k_pauli_z<R, NTZ>), dim3(grid), dim3(kRedThreads), 0, s,
-                               static_cast<const cx<R> *>(st.amps), nbases, dz, cnt, dpart);
+                               static_cast<const cx<R> *>(st.amps), nbases, dz, cnt, dpart, 0);
         else
             hipLaunchKernelGGL((k_pauli_x<R, NTX>), dim3(grid), dim3(kRedThreads), 0, s,
-                               static_cast<const cx<R> *>(st.amps), nbases, x, hb, dz, cnt, dpart);
+                               static_cast<const cx<R> *>(st.amps), nbases, x, hb, dz, cnt, dpart, 0);
         GOQU_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, dpart, (int)grid, nv, dout);
         GOQU_HIP(hipGetLastError());
@@ -244,6 +250,65 @@ static int pauli_sums_t(State &st, uint64_t x, const uint64_t *z, int nt, double
         }
     }
     return GOQU_OK;
+}
+
+// Batched finish: for state y, the first nv of the pstride values each of its nblocks blocks
+// wrote, summed over the blocks in a fixed order, as complex pairs out[y * out_stride +
+// 2 * (out_off + t)] (x == 0: real sums, imaginary parts 0).
+__global__ void k_finish_batch(const double *__restrict__ partials, int nblocks, int pstride, int nv, int xmode,
+                               double *__restrict__ out, int out_stride, int out_off) {
+    const int y = blockIdx.x;
+    const double *p = partials + (size_t)y * nblocks * pstride;
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+        double acc = 0;
+        for (int b = 0; b < nblocks; b++) acc += p[(size_t)b * pstride + k];
+        double *o = out + (size_t)y * out_stride + 2 * out_off;
+        if (xmode) {
+            o[k] = acc;
+        } else {
+            o[2 * k] = acc;
+            o[2 * k + 1] = 0.0;
+        }
+    }
+}
+
+template <typename R>
+static int pauli_sums_batch_t(const State &view, int nstates, uint64_t stride, uint64_t x, const uint64_t *dz, int nt,
+                              double *dout, int out_stride, int out_off, Scratch &scr, hipStream_t s) {
+    constexpr int NTZ = 32, NTX = 16;
+    const int per = x == 0 ? NTZ : NTX;
+    const int nv = x == 0 ? per : 2 * per;
+    const uint64_t N = view.size();
+    const uint64_t xhi = x & ~15ull;
+    const uint64_t nbases = xhi ? (N >> 5) : (N >> 4);
+    const int hb = xhi ? 63 - __builtin_clzll(xhi) - 4 : 0;
+    // fewer blocks per state when many states share the launch
+    const unsigned g1 = red_grid(nbases, kRedThreads * 4);
+    const unsigned grid = std::max(1u, std::min(g1, (unsigned)std::max<uint64_t>(1, 4096 / (uint64_t)nstates)));
+    int rc = scr.ensure((size_t)grid * nv * 8 * nstates);
+    if (rc) return rc;
+    double *dpart = static_cast<double *>(scr.ptr);
+    for (int t0 = 0; t0 < nt; t0 += per) {
+        const int cnt = std::min(per, nt - t0);
+        if (x == 0)
+            hipLaunchKernelGGL((k_pauli_z<R, NTZ>), dim3(grid, nstates), dim3(kRedThreads), 0, s,
+                               static_cast<const cx<R> *>(view.amps), nbases, dz + t0, cnt, dpart, stride);
+        else
+            hipLaunchKernelGGL((k_pauli_x<R, NTX>), dim3(grid, nstates), dim3(kRedThreads), 0, s,
+                               static_cast<const cx<R> *>(view.amps), nbases, x, hb, dz + t0, cnt, dpart, stride);
+        GOQU_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_finish_batch, dim3(nstates), dim3(64), 0, s, dpart, (int)grid, nv,
+                           x == 0 ? cnt : 2 * cnt, x == 0 ? 0 : 1, dout, out_stride, out_off + t0);
+        GOQU_HIP(hipGetLastError());
+    }
+    return GOQU_OK;
+}
+
+int pauli_sums_batch(const State &view, int nstates, uint64_t stride, uint64_t x, const uint64_t *dz, int nt,
+                     double *dout, int out_stride, int out_off, Scratch &scr, hipStream_t s) {
+    return view.prec == GOQU_C128
+               ? pauli_sums_batch_t<double>(view, nstates, stride, x, dz, nt, dout, out_stride, out_off, scr, s)
+               : pauli_sums_batch_t<float>(view, nstates, stride, x, dz, nt, dout, out_stride, out_off, scr, s);
 }
 
 int pauli_sums(State &st, uint64_t x, const uint64_t *z, int nt, double *out, Scratch &scr, hipStream_t s) {
@@ -357,6 +422,26 @@ template <typename R>
 __global__ void k_fill_basis(cx<R> *__restrict__ a, uint64_t n, uint64_t index) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         a[i] = cx<R>{i == index ? R(1) : R(0), R(0)};
+}
+
+// nstates consecutive states of view.size() amplitudes each, every one |0...0>
+template <typename R>
+__global__ void k_fill_zero_batch(cx<R> *__restrict__ a, uint64_t n, uint64_t stride) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = cx<R>{(i & (stride - 1)) == 0 ? R(1) : R(0), R(0)};
+}
+
+int fill_basis_batch(const State &view, int nstates, hipStream_t s) {
+    const uint64_t n = view.size() * (uint64_t)nstates;
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + 255) / 256, 1u << 14);
+    if (view.prec == GOQU_C128)
+        hipLaunchKernelGGL(k_fill_zero_batch<double>, dim3(grid), dim3(256), 0, s,
+                           static_cast<cx<double> *>(view.amps), n, view.size());
+    else
+        hipLaunchKernelGGL(k_fill_zero_batch<float>, dim3(grid), dim3(256), 0, s,
+                           static_cast<cx<float> *>(view.amps), n, view.size());
+    GOQU_HIP(hipGetLastError());
+    return GOQU_OK;
 }
 
 int fill_basis(State &st, uint64_t index, hipStream_t s) {
@@ -972,7 +1057,10 @@ int sample_state_static(State &st, int dev, uint64_t nvalid, const double *u, in
 }
 
 // ---- density matrix ---------------------------------------------------------------------------------
-__global__ void k_dm_diag(const cx<double> *__restrict__ rho, uint64_t dim, double *__restrict__ p) {
+__global__ void k_dm_diag(const cx<double> *__restrict__ rho, uint64_t dim, double *__restrict__ p,
+                          uint64_t shot_stride) {
+    rho += blockIdx.y * shot_stride;
+    p += blockIdx.y * dim;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim; i += (uint64_t)gridDim.x * blockDim.x) {
         const double v = rho[i * dim + i].x;
         p[i] = v < 0 ? 0.0 : v;  // measure.go:11-21
@@ -982,7 +1070,17 @@ __global__ void k_dm_diag(const cx<double> *__restrict__ rho, uint64_t dim, doub
 int dm_diag(const State &rho, int n, double *dprobs, hipStream_t s) {
     const uint64_t dim = 1ull << n;
     const unsigned grid = (unsigned)std::min<uint64_t>((dim + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_dm_diag, dim3(grid), dim3(256), 0, s, static_cast<const cx<double> *>(rho.amps), dim, dprobs);
+    hipLaunchKernelGGL(k_dm_diag, dim3(grid), dim3(256), 0, s, static_cast<const cx<double> *>(rho.amps), dim, dprobs,
+                       0);
+    GOQU_HIP(hipGetLastError());
+    return GOQU_OK;
+}
+
+int dm_diag_batch(const State &view, int n, int nstates, uint64_t stride, double *dprobs, hipStream_t s) {
+    const uint64_t dim = 1ull << n;
+    const unsigned grid = (unsigned)std::min<uint64_t>((dim + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_dm_diag, dim3(grid, nstates), dim3(256), 0, s, static_cast<const cx<double> *>(view.amps),
+                       dim, dprobs, stride);
     GOQU_HIP(hipGetLastError());
     return GOQU_OK;
 }
@@ -1011,9 +1109,13 @@ int dm_readout(double *dprobs, double *tmp, int n, int q, double p01, double p10
 }
 
 // sum_i rho[i, i^x] (-1)^{|i&z|}  (expect_dm.go:9-21, without coeff and i^ny)
+// density matrices batched along gridDim.y as in k_pauli_z
 __global__ void __launch_bounds__(kRedThreads) k_dm_pauli(const cx<double> *__restrict__ rho, uint64_t dim, uint64_t x,
-                                                          uint64_t z, double *__restrict__ partials) {
+                                                          uint64_t z, double *__restrict__ partials,
+                                                          uint64_t shot_stride) {
     __shared__ double sh[8];
+    rho += blockIdx.y * shot_stride;
+    partials += (size_t)blockIdx.y * gridDim.x * 2;
     double v[2] = {0, 0};
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < dim; i += (uint64_t)gridDim.x * blockDim.x) {
         const cx<double> e = rho[i * dim + (i ^ x)];
@@ -1036,9 +1138,26 @@ int dm_pauli(const State &rho, int n, uint64_t x, uint64_t z, double out[2], Scr
     if (rc) return rc;
     double *dpart = static_cast<double *>(scr.ptr);
     hipLaunchKernelGGL(k_dm_pauli, dim3(grid), dim3(kRedThreads), 0, s, static_cast<const cx<double> *>(rho.amps), dim,
-                       x, z, dpart);
+                       x, z, dpart, 0);
     GOQU_HIP(hipGetLastError());
     return finish_to_host(dpart, grid, 2, out, scr, pbytes, s);
+}
+
+int dm_pauli_batch(const State &view, int n, int nstates, uint64_t stride, uint64_t x, uint64_t z, double *dout,
+                   int out_stride, int out_off, Scratch &scr, hipStream_t s) {
+    const uint64_t dim = 1ull << n;
+    const unsigned g1 = red_grid(dim, kRedThreads * 4);
+    const unsigned grid = std::max(1u, std::min(g1, (unsigned)std::max<uint64_t>(1, 4096 / (uint64_t)nstates)));
+    int rc = scr.ensure((size_t)grid * 16 * nstates);
+    if (rc) return rc;
+    double *dpart = static_cast<double *>(scr.ptr);
+    hipLaunchKernelGGL(k_dm_pauli, dim3(grid, nstates), dim3(kRedThreads), 0, s,
+                       static_cast<const cx<double> *>(view.amps), dim, x, z, dpart, stride);
+    GOQU_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finish_batch, dim3(nstates), dim3(64), 0, s, dpart, (int)grid, 2, 2, 1, dout, out_stride,
+                       out_off);
+    GOQU_HIP(hipGetLastError());
+    return GOQU_OK;
 }
 
 // Purity (sim.go:265-276): Re sum_ij rho_ij conj(rho_ji), tiled 32x32 transposes

@@ -75,13 +75,15 @@ struct Scratch {
     ~Scratch();
 };
 
-// Shots batched along gridDim.y: shot y's state starts y * stride amplitudes after the first;
-// a shot whose act byte is 0 is skipped by every kernel launched with the batch (act == nullptr:
-// all shots run).
+// States batched along gridDim.y: state y starts y * stride amplitudes after the first; a state
+// whose act byte is 0 is skipped by every kernel launched with the batch (act == nullptr: all
+// run).  blob_stride != 0: state y runs its own program, a copy of the launched one's layout
+// y * blob_stride bytes further on (headers, instructions and pool alike).
 struct ShotBatch {
     int nshots = 1;
     uint64_t stride = 0;
     const uint8_t *act = nullptr;
+    uint64_t blob_stride = 0;
 };
 
 // A dense op on 5..9 target qubits (EOP_WIDE) staged on the device for k_wide_mfma (wide.hip):
@@ -118,6 +120,18 @@ struct PreparedOps {
 // ---- launchers (stream-ordered) -------------------------------------------------------------
 PlanConfig default_plan_config(int prec, int nq, bool paired);
 int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp);
+// Device byte layout of a program: headers | instructions | pool (as the state's real type),
+// each part 256-aligned.  Programs serialised with one common layout (sized for the largest) can
+// sit at a fixed stride and run in one batched launch (ShotBatch::blob_stride).
+struct ProgramLayout {
+    size_t hdr_off = 0, instr_off = 0, pool_off = 0, bytes = 0;
+};
+ProgramLayout program_layout(size_t nh, size_t ni, size_t np, int prec);
+// P serialised into out[0 .. L.bytes) (LDS offsets filled in); stages receives the final headers
+int serialize_program(const Program &P, int prec, const ProgramLayout &L, unsigned char *out,
+                      std::vector<StageHeader> *stages);
+// a stage that changes nothing (no instructions, stores in load order): launches skip it
+bool stage_is_identity(const StageHeader &H);
 int run_program(State &st, const DeviceProgram &dp, hipStream_t s, const ShotBatch *b = nullptr);
 // plan (with cfg, or the default configuration) + upload + run
 int run_eops(State &st, const std::vector<EOp> &eops, hipStream_t s, const PlanConfig *cfg = nullptr);
@@ -131,11 +145,18 @@ int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, s
              hipStream_t s);
 
 int fill_basis(State &st, uint64_t index, hipStream_t s);  // |index>
+// nstates states of view.size() amplitudes from view.amps on, each |0...0>
+int fill_basis_batch(const State &view, int nstates, hipStream_t s);
 int copy_state(State &dst, const State &src, hipStream_t s);
 
 // Pauli expectations: sums S_t = sum_i conj(psi[i^x]) (-1)^{|i&z_t|} psi[i] (no coefficient,
 // no i^ny), all terms sharing one x mask.  out holds 2*nt doubles.
 int pauli_sums(State &st, uint64_t x, const uint64_t *z, int nt, double *out, Scratch &scr, hipStream_t s);
+// pauli_sums over nstates states at a stride (x-group terms z = dz[0..nt), a device array), with
+// no synchronisation: state y's sum for term t goes to dout[y * out_stride + 2 * (out_off + t)]
+// (re, im); the partials live in scr, which must stay untouched until the stream has run.
+int pauli_sums_batch(const State &view, int nstates, uint64_t stride, uint64_t x, const uint64_t *dz, int nt,
+                     double *dout, int out_stride, int out_off, Scratch &scr, hipStream_t s);
 // Re/Im of <a|b> in FP64.
 int inner_product(const State &a, const State &b, double out[2], Scratch &scr, hipStream_t s);
 // sum over amplitudes with bit q == 0 of |a|^2 (q < 0: all)
@@ -166,6 +187,12 @@ int axpy_state(State &y, const State &x, double re, double im, hipStream_t s);  
 int dm_diag(const State &rho, int n, double *dprobs_out, hipStream_t s);
 int dm_readout(double *dprobs, double *tmp, int n, int q, double p01, double p10, hipStream_t s);
 int dm_pauli(const State &rho, int n, uint64_t x, uint64_t z, double out[2], Scratch &scr, hipStream_t s);
+// batched over nstates density matrices at a stride, no synchronisation: (re, im) of state y at
+// dout[y * out_stride + 2 * out_off]
+int dm_pauli_batch(const State &view, int n, int nstates, uint64_t stride, uint64_t x, uint64_t z, double *dout,
+                   int out_stride, int out_off, Scratch &scr, hipStream_t s);
+// diagonals of nstates density matrices: dprobs[y * 2^n + i]
+int dm_diag_batch(const State &view, int n, int nstates, uint64_t stride, double *dprobs, hipStream_t s);
 int dm_purity(const State &rho, int n, double *out, Scratch &scr, hipStream_t s);
 int dm_fidelity(const State &rho, int n, const void *dpsi, double *out, Scratch &scr, hipStream_t s);
 

@@ -1022,6 +1022,11 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
     }
 }
 
+template <typename T>
+__device__ __forceinline__ const T *blob_at(const T *p, uint64_t byte_off) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + byte_off);
+}
+
 // Three single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops
 // (few enough VGPRs for four waves per SIMD, and code small enough for short branches),
 // k_stage_big the stages that hold them, k_stage_dm density-matrix stages (channel
@@ -1029,23 +1034,32 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                               uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride) {
+                                               uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride,
+                                               uint64_t blob_stride) {
     if (act && !act[blockIdx.y]) return;  // a shot that this program does not reach
-    stage_body<R, false, false, false>(st + blockIdx.y * shot_stride, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+    const uint64_t bo = blockIdx.y * blob_stride;  // state y's own program (0: one shared program)
+    stage_body<R, false, false, false>(st + blockIdx.y * shot_stride, nullptr, blob_at(hdr, bo), blob_at(prog, bo),
+                                       blob_at(pool, bo), ntiles, nullptr, nullptr);
 }
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage_big(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                                   uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride) {
+                                                   uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride,
+                                                   uint64_t blob_stride) {
     if (act && !act[blockIdx.y]) return;  // a shot that this program does not reach
-    stage_body<R, false, true, false>(st + blockIdx.y * shot_stride, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+    const uint64_t bo = blockIdx.y * blob_stride;  // state y's own program (0: one shared program)
+    stage_body<R, false, true, false>(st + blockIdx.y * shot_stride, nullptr, blob_at(hdr, bo), blob_at(prog, bo),
+                                      blob_at(pool, bo), ntiles, nullptr, nullptr);
 }
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage_dm(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                   const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                                  uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride) {
+                                                  uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride,
+                                                  uint64_t blob_stride) {
     if (act && !act[blockIdx.y]) return;  // a shot that this program does not reach
-    stage_body<R, false, true, true>(st + blockIdx.y * shot_stride, nullptr, hdr, prog, pool, ntiles, nullptr, nullptr);
+    const uint64_t bo = blockIdx.y * blob_stride;  // state y's own program (0: one shared program)
+    stage_body<R, false, true, true>(st + blockIdx.y * shot_stride, nullptr, blob_at(hdr, bo), blob_at(prog, bo),
+                                     blob_at(pool, bo), ntiles, nullptr, nullptr);
 }
 
 template <typename R>
@@ -1063,7 +1077,7 @@ __global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<
 #define GOQU_STAGE_INST(EXT, K, R)                                                                \
     EXT template __global__ void K<R>(cx<R> *__restrict__, const StageHeader *__restrict__,          \
                                       const Instr *__restrict__, const R *__restrict__, uint64_t,      \
-                                      const uint8_t *__restrict__, uint64_t);
+                                      const uint8_t *__restrict__, uint64_t, uint64_t);
 #define GOQU_PAIR_INST(EXT, R)                                                                     \
     EXT template __global__ void k_stage_pair<R>(cx<R> *__restrict__, cx<R> *__restrict__,           \
                                                  const StageHeader *__restrict__, const Instr *__restrict__, \
@@ -1131,48 +1145,71 @@ PlanConfig default_plan_config(int prec, int nq, bool paired) {
     return c;
 }
 
-int upload_program(const Program &P0, int prec, hipStream_t s, DeviceProgram &dp) {
+ProgramLayout program_layout(size_t nh, size_t ni, size_t np, int prec) {
     const size_t rsz = prec == GOQU_C128 ? sizeof(double) : sizeof(float);
+    ProgramLayout L;
+    L.hdr_off = 0;
+    L.instr_off = (nh * sizeof(StageHeader) + 255) & ~size_t(255);
+    L.pool_off = (L.instr_off + ni * sizeof(Instr) + 255) & ~size_t(255);
+    // 64 spare bytes: the kernel's instruction-head fetch (64 B) stays inside the blob even for
+    // a stage without instructions at the end of the array
+    L.bytes = (L.pool_off + np * rsz + 64 + 255) & ~size_t(255);
+    return L;
+}
+
+int serialize_program(const Program &P0, int prec, const ProgramLayout &L, unsigned char *out,
+                      std::vector<StageHeader> *stages) {
     for (const StageHeader &H : P0.stages)
         if ((size_t(1) << H.m) * 8 > 65536)
             return fail(GOQU_ERR_UNSUPPORTED, "stage tile larger than the LDS offset range");
+    const size_t rsz = prec == GOQU_C128 ? sizeof(double) : sizeof(float);
+    const size_t nh = P0.stages.size(), ni = P0.instrs.size(), np = P0.pool.size();
+    if (L.instr_off < nh * sizeof(StageHeader) || L.pool_off < L.instr_off + ni * sizeof(Instr) ||
+        L.bytes < L.pool_off + np * rsz + 64)
+        return fail(GOQU_ERR_INVALID_ARG, "program larger than its layout");
     Program P;
     P.stages = P0.stages;
     P.instrs = P0.instrs;
     set_lds_offsets(P);
-    const std::vector<double> &pool = P0.pool;
-    const size_t nh = P.stages.size(), ni = P.instrs.size(), np = pool.size();
-    dp.hdr_off = 0;
-    dp.instr_off = (nh * sizeof(StageHeader) + 255) & ~size_t(255);
-    dp.pool_off = (dp.instr_off + ni * sizeof(Instr) + 255) & ~size_t(255);
-    // 64 spare bytes: the kernel's instruction-head fetch (64 B) stays inside the blob even for
-    // a stage without instructions at the end of the array
-    const size_t bytes = dp.pool_off + np * rsz + 64;
-    std::vector<unsigned char> host(bytes, 0);
-    std::memcpy(host.data(), P.stages.data(), nh * sizeof(StageHeader));
-    std::memcpy(host.data() + dp.instr_off, P.instrs.data(), ni * sizeof(Instr));
+    std::memset(out, 0, L.bytes);
+    std::memcpy(out + L.hdr_off, P.stages.data(), nh * sizeof(StageHeader));
+    std::memcpy(out + L.instr_off, P.instrs.data(), ni * sizeof(Instr));
     if (prec == GOQU_C128) {
-        std::memcpy(host.data() + dp.pool_off, pool.data(), np * sizeof(double));
+        std::memcpy(out + L.pool_off, P0.pool.data(), np * sizeof(double));
     } else {
-        float *f = reinterpret_cast<float *>(host.data() + dp.pool_off);
-        for (size_t i = 0; i < np; i++) f[i] = (float)pool[i];
+        float *f = reinterpret_cast<float *>(out + L.pool_off);
+        for (size_t i = 0; i < np; i++) f[i] = (float)P0.pool[i];
     }
-    if (dp.bytes < bytes) {
+    if (stages) *stages = P.stages;
+    return GOQU_OK;
+}
+
+int upload_program(const Program &P0, int prec, hipStream_t s, DeviceProgram &dp) {
+    const ProgramLayout L = program_layout(P0.stages.size(), P0.instrs.size(), P0.pool.size(), prec);
+    std::vector<unsigned char> host(L.bytes);
+    std::vector<StageHeader> stages;
+    int rc = serialize_program(P0, prec, L, host.data(), &stages);
+    if (rc) return rc;
+    if (dp.bytes < L.bytes) {
         if (dp.blob) GOQU_HIP(hipFree(dp.blob));
         dp.blob = nullptr;
-        GOQU_HIP(dmalloc((void **)&dp.blob, bytes));
-        dp.bytes = bytes;
+        dp.bytes = 0;
+        GOQU_HIP(dmalloc((void **)&dp.blob, L.bytes));
+        dp.bytes = L.bytes;
     }
-    GOQU_HIP(hipMemcpyAsync(dp.blob, host.data(), bytes, hipMemcpyHostToDevice, s));
+    dp.hdr_off = L.hdr_off;
+    dp.instr_off = L.instr_off;
+    dp.pool_off = L.pool_off;
+    GOQU_HIP(hipMemcpyAsync(dp.blob, host.data(), L.bytes, hipMemcpyHostToDevice, s));
     // the host buffer must outlive the async copy
     GOQU_HIP(hipStreamSynchronize(s));
-    dp.stages = P.stages;
+    dp.stages = std::move(stages);
     return GOQU_OK;
 }
 
 // A stage without instructions that stores in its load order changes nothing (a stage of
 // relabelled SWAPs alone still permutes its tile and runs).
-static bool stage_is_identity(const StageHeader &H) {
+bool stage_is_identity(const StageHeader &H) {
     return H.ninstr == 0 && std::memcmp(H.gq0, H.gqN, (size_t)H.m) == 0;
 }
 
@@ -1185,7 +1222,7 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, cons
     const bool prof = g_profile.load();
     const unsigned nshots = b ? (unsigned)b->nshots : 1u;
     const uint8_t *act = b ? b->act : nullptr;
-    const uint64_t stride = b ? b->stride : 0;
+    const uint64_t stride = b ? b->stride : 0, bstride = b ? b->blob_stride : 0;
     std::vector<hipEvent_t> ev;
     for (size_t i = 0; i < dp.stages.size(); i++) {
         const int m = dp.stages[i].m;
@@ -1207,15 +1244,15 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, cons
         if (dp.stages[i].flags & SF_DM) {
             if constexpr (sizeof(R) == 8)
                 hipLaunchKernelGGL(k_stage_dm<double>, g, dim3(threads), lds, s, static_cast<cx<double> *>(st.amps),
-                                   hdrs + i, instrs, pool, ntiles, act, stride);
+                                   hdrs + i, instrs, pool, ntiles, act, stride, bstride);
             else
                 return fail(GOQU_ERR_UNSUPPORTED, "density-matrix stages run in complex128 only");
         } else if (dp.stages[i].flags & SF_BIG)
             hipLaunchKernelGGL(k_stage_big<R>, g, dim3(threads), lds, s, static_cast<cx<R> *>(st.amps), hdrs + i,
-                               instrs, pool, ntiles, act, stride);
+                               instrs, pool, ntiles, act, stride, bstride);
         else
             hipLaunchKernelGGL(k_stage<R>, g, dim3(threads), lds, s, static_cast<cx<R> *>(st.amps), hdrs + i, instrs,
-                               pool, ntiles, act, stride);
+                               pool, ntiles, act, stride, bstride);
         GOQU_HIP(hipGetLastError());
         if (prof) GOQU_HIP(hipEventRecord(ev.back(), s));
     }

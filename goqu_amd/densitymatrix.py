@@ -78,6 +78,19 @@ def marshal(c: Circuit, noise: Optional[NZ.NoiseModel] = None) -> "N.OpBuffer":
     return buf
 
 
+def _readout_arrays(noise: Optional[NZ.NoiseModel], n: int):
+    """Per-qubit readout flip probabilities (-1: none) for goqu_dm_probs, or (None, None)."""
+    if noise is None or not noise.readout:
+        return None, None
+    p01 = np.full(n, -1.0)
+    p10 = np.full(n, -1.0)
+    for q in range(n):
+        re = noise.ReadoutFor(q)
+        if re is not None:
+            p01[q], p10[q] = re.P01, re.P10
+    return p01, p10
+
+
 class Sim:
     def __init__(self, num_qubits: int):
         if num_qubits < 1 or num_qubits > 16:
@@ -137,15 +150,7 @@ class Sim:
         N.check(N.lib().goqu_dm_set(self._h, N.dptr(a.view(np.float64)), 0, len(a)))
 
     def _readout_arrays(self):
-        if self.noise is None or not self.noise.readout:
-            return None, None
-        p01 = np.full(self.num_qubits, -1.0)
-        p10 = np.full(self.num_qubits, -1.0)
-        for q in range(self.num_qubits):
-            re = self.noise.ReadoutFor(q)
-            if re is not None:
-                p01[q], p10[q] = re.P01, re.P10
-        return p01, p10
+        return _readout_arrays(self.noise, self.num_qubits)
 
     def DiagonalProbs(self) -> np.ndarray:
         """diagonalProbs (measure.go:11-21)."""

@@ -231,6 +231,34 @@ int goqu_dm_run_dynamic(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops, con
                         int32_t num_instr, const int32_t *cond_bits, int32_t num_cond_bits, int32_t num_clbits,
                         int64_t shots, const double *uniforms, int32_t draws_per_shot, uint8_t *clbits_out);
 
+/* ---- batches of circuits on one register size (beside sweep.RunSim / RunDensitySim,
+ *      sweep/run.go:27-171, and mitigation.StatevectorExecutor / DensityMatrixExecutor,
+ *      algorithm/mitigation/executor.go:20-41, which evaluate circuits one fresh simulator at a
+ *      time) ------------------------------------------------------------------------------- */
+/* Circuit i is ops[op_offsets[i] .. op_offsets[i+1]) (op_offsets[0] = 0, num_instances + 1
+ * entries), run from |0...0> on num_qubits qubits.  Circuits whose programs launch alike run
+ * together, one launch per stage sweep over all their states.  tile_bits / low_bits choose the
+ * tile shape as goqu_sv_set_plan does ((0, 0) = the default).
+ * batch_expect: out[i * 2 * num_terms + 2 t ...] = coeff_t <psi_i|P_t|psi_i> (as
+ * goqu_sv_expect_paulis per circuit).  batch_sample: out_idx[i * shots + s] for
+ * uniforms[i * shots + s] (as goqu_sv_sample per circuit). */
+int goqu_sv_batch_expect(int32_t num_qubits, int32_t precision, int32_t tile_bits, int32_t low_bits,
+                         int32_t num_instances, const goqu_op_t *ops, const int64_t *op_offsets, int32_t num_terms,
+                         const uint64_t *xmask, const uint64_t *zmask, const int32_t *num_y, const double *coeffs,
+                         double *out);
+int goqu_sv_batch_sample(int32_t num_qubits, int32_t precision, int32_t tile_bits, int32_t low_bits,
+                         int32_t num_instances, const goqu_op_t *ops, const int64_t *op_offsets, int64_t shots,
+                         const double *uniforms, uint64_t *out_idx);
+/* Density-matrix circuits (ops as for goqu_dm_apply, noise resolved): Tr(rho_i P_t) * coeff_t
+ * (as goqu_dm_expect_paulis), or samples of the diagonal after the per-qubit readout error
+ * p01 / p10 (NULL: none; as goqu_dm_probs + goqu_dm_sample). */
+int goqu_dm_batch_expect(int32_t num_qubits, int32_t num_instances, const goqu_op_t *ops, const int64_t *op_offsets,
+                         int32_t num_terms, const uint64_t *xmask, const uint64_t *zmask, const int32_t *num_y,
+                         const double *coeffs, double *out);
+int goqu_dm_batch_sample(int32_t num_qubits, int32_t num_instances, const goqu_op_t *ops, const int64_t *op_offsets,
+                         const double *p01, const double *p10, int64_t shots, const double *uniforms,
+                         uint64_t *out_idx);
+
 /* ---- host-side planning and test hooks (no counterpart in the reference) ------------------ */
 /* Lowers and plans an op list on the host only (no device is touched): *stages receives the
  * number of state sweeps (stage-kernel launches) and *instrs the instruction count.  density != 0
