@@ -1,9 +1,12 @@
-"""Benchmark: 30-qubit complex128 gate application on one MI355X.
+"""Benchmark: 30-qubit complex128 gate application on one MI355X, plus the other workloads of the
+path (adjoint gradients, sampling, batched QAOA, density matrix, wide MFMA ops, dynamic circuits,
+circuit batches).
 
 Workload (one "step"): Evolve of the depth-40 seeded random circuit on 30 qubits —
 U3(theta, phi, lambda ~ U[0, 2pi)) on every qubit per layer, then 15 disjoint random pairs
 (a quarter of them spanning a low and a high qubit) each carrying CZ, CNOT, SWAP or CP.
-The state (16 GiB) stays resident in HBM; the timed region is the K Evolves.
+The state (16 GiB) stays resident in HBM; the timed region is the K Evolves.  The tile shape is
+the autotuner's choice for this workload (timed before the warm-up, outside the timed region).
 
 Printed JSON line:
   value     = HBM sweep throughput of the whole step: stage sweeps x 32 B x 2^30 / step time
@@ -12,9 +15,11 @@ Printed JSON line:
               its stream, against the 8 TB/s HBM peak; traffic = HBM bytes per launch from the
               committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary under profiles/
   cpu_baseline = the oracle (a C restatement of goqu's CPU statevector with its worker policy)
-              applying the first gates of the same circuit at 30 qubits on the host cores;
-              bytes per gate as for the GPU metric (32 B x 2^n, a quarter for CZ/CP, half for
-              CNOT/SWAP), divided by its wall time.
+              applying the first gates of the same circuit at 30 qubits on the host cores (the
+              reference itself cannot run 30 qubits), and the same at 28 qubits beside the GPU's
+              28-qubit time; bytes per gate as for the GPU metric (32 B x 2^n, a quarter for CZ/CP,
+              half for CNOT/SWAP), divided by its wall time.
+  each further section carries its own roofline object(s) from the same HIP-event profiler.
 """
 from __future__ import annotations
 
@@ -57,14 +62,29 @@ def gate_bytes(op, n: int) -> float:
     return full
 
 
-def cpu_baseline(circ, n: int, budget_s: float) -> dict:
-    from oracle import oracle as O
+def _cpu_info() -> dict:
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    # the host cores this process may use: its affinity set, within the OpenMP thread budget the
+    # machine gives one GPU's job when it sets one
+    omp = os.environ.get("OMP_NUM_THREADS")
+    cores = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cores": aff, "omp_num_threads": omp,
+            "cores": cores}
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    cores = min(cores, 16)
-    O.set_workers(cores)
+
+def _cpu_gate_rate(circ, n: int, budget_s: float, cores: int) -> tuple:
+    from oracle import oracle as O
     from goqu_amd import _native as N
 
+    O.set_workers(cores)
     state = O.zero_state(n)
     ops = [op for op in circ.Ops() if op.Gate is not None]
     done, nbytes = 0, 0.0
@@ -79,8 +99,69 @@ def cpu_baseline(circ, n: int, budget_s: float) -> dict:
             break
     dt = time.perf_counter() - t0
     del state
-    return {"value": round(nbytes / dt / 1e9, 3), "unit": "GB/s", "cores": cores, "kind": "port",
-            "sample": f"first {done} of {len(ops)} gates of the same {n}-qubit circuit, one sweep per gate"}
+    return nbytes / dt / 1e9, done, len(ops), dt
+
+
+def cpu_baseline(circ, n: int, budget_s: float, layers: int) -> dict:
+    from circuits import config2_circuit
+    from oracle import oracle as O
+
+    info = _cpu_info()
+    cores = info["cores"]
+    gbps, done, total, _ = _cpu_gate_rate(circ, n, budget_s, cores)
+    out = {"value": round(gbps, 3), "unit": "GB/s", "cores": cores, "kind": "port",
+           "sample": f"first {done} of {total} gates of the same {n}-qubit circuit, one sweep per gate",
+           "reference_runs": n <= 28,
+           "note": ("the reference statevector cannot run this size: statevector.New panics above 28 qubits "
+                    "(sim/statevector/sim.go:28-30); timed on the oracle's restatement of its kernels")
+           if n > 28 else "", **info,
+           "host_stream_copy_GBps": round(O.stream_copy_gbps(1 << 27, cores, 3), 1)}
+    # the largest size the reference runs: CPU beside the GPU on the same 28-qubit circuit
+    c28 = config2_circuit(28, layers, seed=2)
+    g28, done28, total28, _ = _cpu_gate_rate(c28, 28, budget_s / 2, cores)
+    out["q28"] = {"cpu_GBps": round(g28, 3), "cpu_sample": f"first {done28} of {total28} gates",
+                  "gpu_GBps": round(_gpu_rate(c28, 28), 1), "unit": "GB/s (gate-sweep bytes / time)"}
+    return out
+
+
+def _gpu_rate(circ, n: int) -> float:
+    """Gate-sweep bytes of circ / GPU Evolve time (for the CPU comparison at the same size)."""
+    from goqu_amd import _native as N
+    from goqu_amd import statevector
+
+    sim = statevector.New(n)
+    buf = N.OpBuffer()
+    for op in circ.Ops():
+        if op.Gate is not None:
+            N.add_sv_gate(buf, op.Gate, op.Qubits)
+    arr = buf.array()
+    N.check(N.lib().goqu_sv_apply(sim._h, arr, len(buf)))
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        N.check(N.lib().goqu_sv_reset(sim._h))
+        N.check(N.lib().goqu_sv_apply(sim._h, arr, len(buf)))
+    dt = (time.perf_counter() - t0) / reps
+    sim.Close()
+    return sum(gate_bytes(op, n) for op in circ.Ops() if op.Gate is not None) / dt / 1e9
+
+
+def _roof(kernel: str, bin_id: int, bound: str = "hbm"):
+    """Roofline object of one profiled kernel over everything recorded since the last reset."""
+    from goqu_amd import _native as N
+
+    ms, launches, amount = N.profile_read_kernel(bin_id)
+    if not launches or ms <= 0:
+        return None
+    if bound == "mfma":
+        achieved = amount / (ms / 1e3) / 1e12
+        peak, unit = 78.6, "TFLOP/s"  # MI355X dense FP64 MFMA
+    else:
+        achieved = amount / (ms / 1e3) / 1e9
+        peak, unit = HBM_PEAK_GBPS, "GB/s"
+    return {"kernel": kernel, "bound": bound, "achieved": round(achieved, 1), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 4), "launches": launches, "avg_launch_ms": round(ms / launches, 4),
+            "traffic": None}
 
 
 def adjoint_section(qubits: int, reps: int, terms: int, ps_params: int, seed: int = 3) -> dict:
@@ -108,12 +189,19 @@ def adjoint_section(qubits: int, reps: int, terms: int, ps_params: int, seed: in
         hs.append(NewPauliString(rng.normal(), ops, qubits))
     H = NewPauliSum(hs)
     x = rng.uniform(0, 2 * math.pi, npar)
+    from goqu_amd import _native as N
+
     sim = statevector.New(qubits)
     adj = GR.Adjoint(c, H, names, sim=sim)
     adj.value_and_grad(x)  # warm-up
+    N.profile(True)
     t0 = time.perf_counter()
     e, g = adj.value_and_grad(x)
     t_adj = time.perf_counter() - t0
+    roofs = [r for r in (_roof("k_stage_pair<double>", N.PROF_STAGE_PAIR),
+                         _roof("k_pauli_apply_tile<double>", N.PROF_HAMILTONIAN),
+                         _roof("k_stage<double>", N.PROF_STAGE)) if r]
+    N.profile(False)
     f = GR.CostFunc(c, H, names, sim=sim)
     f(x)
     k = min(ps_params, npar)
@@ -130,7 +218,8 @@ def adjoint_section(qubits: int, reps: int, terms: int, ps_params: int, seed: in
             "adjoint_s": round(t_adj, 4), "adjoint_grads_per_s": round(1.0 / t_adj, 3),
             "param_shift_s": round(t_ps, 3), "param_shift_grads_per_s": round(1.0 / t_ps, 5),
             "param_shift_sample": f"first {k} of {npar} parameters, extrapolated",
-            "speedup": round(t_ps / t_adj, 1), "max_abs_diff_vs_param_shift": float(np.max(np.abs(g[:k] - ps)))}
+            "speedup": round(t_ps / t_adj, 1), "max_abs_diff_vs_param_shift": float(np.max(np.abs(g[:k] - ps))),
+            "roofline": roofs}
 
 
 def sampling_section(n: int = 20, layers: int = 20, shots: int = 1024) -> dict:
@@ -174,16 +263,27 @@ def qaoa_batch_section(num: int = 4096, seed: int = 5) -> dict:
                 break
         p = int(rng.integers(1, 5))
         insts.append(Q.MaxCutInstance(n, edges, p, rng.uniform(0, math.pi, 2 * p)))
+    from goqu_amd import _native as N
+
     Q.BatchMaxCut(insts[:64])  # warm-up (module load, graph machinery)
+    N.profile(True)
     t0 = time.perf_counter()
     energies, grads = Q.BatchMaxCut(insts)
     dt = time.perf_counter() - t0
+    gms, _, _ = N.profile_read_kernel(N.PROF_QAOA)
+    N.profile(False)
     big = [it for it in insts if it.num_qubits > 12]
     dev_bytes = sum(192.0 * (1 << it.num_qubits) * it.layers for it in big)
+    roof = None
+    if gms > 0:
+        a = dev_bytes / (gms / 1e3) / 1e9
+        roof = {"kernel": "batched QAOA launch graph (k_qaoa_small, k_qaoa_phase, k_qaoa_finish)", "bound": "hbm",
+                "achieved": round(a, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(a / HBM_PEAK_GBPS, 4),
+                "graph_ms": round(gms, 3), "traffic": None}
     return {"workload": f"{num} MaxCut instances, n~U{{10..22}}, G(n,0.3), p~U{{1..4}}, energy + 2p gradients, c128",
             "s": round(dt, 4), "instances_per_s": round(num / dt, 1),
             "large_instance_GBps": round(dev_bytes / dt / 1e9, 1),
-            "mean_energy": float(np.mean(energies))}
+            "mean_energy": float(np.mean(energies)), "roofline": roof}
 
 
 def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
@@ -230,13 +330,14 @@ def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
     e = dm.ExpectPauliSum(H)
     t_fwd = time.perf_counter() - t0
     kms, launches, _ = N.profile_read()
+    roof = _roof("k_stage_dm<double>", N.PROF_STAGE_DM)
     N.profile(False)
     dm.Close()
     sweep = 32.0 * (1 << (2 * n))
     out = {"workload": f"{n}q density matrix, {layers} layers U3 + 7 CNOT + depolarizing/amplitude-damping noise, c128",
            "forward_ms": round(t_fwd * 1e3, 2), "stage_sweeps": launches,
            "forward_GBps": round(launches * sweep / t_fwd / 1e9, 1),
-           "k_stage_avg_ms": round(kms / max(launches, 1), 3), "energy": e}
+           "k_stage_avg_ms": round(kms / max(launches, 1), 3), "energy": e, "roofline": roof}
     g = GR.AdjointDensity(c, H, names, noise=nm, checkpoint_every=16)
     t0 = time.perf_counter()
     e2, grad = g.value_and_grad(x)
@@ -244,6 +345,100 @@ def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
     out["gradient_params"] = len(names)
     out["gradient_energy_matches_forward"] = bool(abs(e2 - e) < 1e-9)
     return out
+
+
+def wide_section(n: int = 26, targets=(5, 7, 9), seed: int = 12) -> dict:
+    """Singly controlled dense gates on 5, 7 and 9 target qubits of a 26-qubit complex128 state
+    (1 GiB): FP64 MFMA contractions (k_wide_mfma), 2^k complex multiply-adds per amplitude of the
+    controlled half."""
+    from scipy.stats import unitary_group
+
+    from goqu_amd import _native as N
+    from goqu_amd import gate as G
+    from goqu_amd import statevector
+    from goqu_amd.circuit import NewBuilder
+
+    rng = np.random.default_rng(seed)
+    out = {"workload": f"{n}q c128 state, one dense gate on each of {list(targets)} target qubits", "ops": []}
+    sim = statevector.New(n)
+    for k in targets:
+        U = unitary_group.rvs(1 << k, random_state=int(rng.integers(1 << 30)))
+        qs = [int(q) for q in rng.choice(n, k + 1, replace=False)]
+        b = NewBuilder("wide", n)
+        b.Apply(G.Controlled(G.Unitary("U", U.reshape(-1)), 1), *qs)
+        c = b.Build()
+        sim.Apply(c)  # warm-up
+        N.profile(True)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            sim.Apply(c)
+        dt = (time.perf_counter() - t0) / 3
+        roof = _roof(f"k_wide_mfma<double> k={k}", N.PROF_WIDE, bound="mfma")
+        N.profile(False)
+        out["ops"].append({"targets": k, "ms": round(dt * 1e3, 3), "roofline": roof})
+    sim.Close()
+    return out
+
+
+def dynamic_section(n: int = 12, shots: int = 20000, seed: int = 13) -> dict:
+    """Dynamic circuits with the shots batched on the device: a 12-qubit circuit with mid-circuit
+    measurements, resets, if/else feed-forward and a repeat-until-success while loop, 20,000 shots."""
+    from circuits import random_dynamic_circuit
+    from goqu_amd import statevector
+
+    c = random_dynamic_circuit(n, 4, 6, np.random.default_rng(seed))
+    sim = statevector.New(n)
+    sim.RunDynamicShots(c, 256, rng=np.random.default_rng(1))  # warm-up
+    t0 = time.perf_counter()
+    sim.RunDynamicShots(c, shots, rng=np.random.default_rng(2))
+    dt = time.perf_counter() - t0
+    sim.Close()
+    return {"workload": f"{n}q dynamic circuit ({c.NumOps()} top-level ops: measure, reset, if/else, switch, for, "
+                        f"while), {shots} shots, c128", "s": round(dt, 4), "shots_per_s": round(shots / dt, 1)}
+
+
+def batch_section(n: int = 12, reps: int = 4, points: int = 2048, terms: int = 50, seed: int = 14) -> dict:
+    """VQE-style parameter batch: EfficientSU2(12, 4) at 2,048 parameter sets, H = 50 Pauli strings,
+    one device batch (op records built per gate for all points at once); against binding and
+    evaluating one point at a time on one simulator (timed on 64 points, extrapolated)."""
+    from circuits import random_pauli_sum
+    from goqu_amd import batch as B
+    from goqu_amd import qaoa as Q
+    from goqu_amd import statevector
+
+    from goqu_amd.circuit import Bind, FreeParameters
+
+    rng = np.random.default_rng(seed)
+    c, npar = Q.EfficientSU2(n, reps)
+    H = random_pauli_sum(n, terms, rng)
+    params = rng.uniform(0, 2 * np.pi, (points, npar))
+    B.vqe_energies(c, H, params[:32])  # warm-up
+    t0 = time.perf_counter()
+    e = B.vqe_energies(c, H, params)
+    dt = time.perf_counter() - t0
+    # one simulator at a time, binding each point as sweep's runStatevectorPoint does (run.go:75-97)
+    names = FreeParameters(c)
+    sim = statevector.New(n)
+    k = 64
+    t0 = time.perf_counter()
+    seq = []
+    for row in params[:k]:
+        sim.Evolve(Bind(c, dict(zip(names, row))))
+        seq.append(sim.ExpectPauliSum(H))
+    t_seq = (time.perf_counter() - t0) / k * points
+    sim.Close()
+    return {"workload": f"EfficientSU2({n},{reps}) at {points} parameter sets, H={terms} Pauli strings, c128",
+            "s": round(dt, 4), "circuits_per_s": round(points / dt, 1),
+            "one_at_a_time_s": round(t_seq, 3), "one_at_a_time_sample": f"first {k} points, extrapolated",
+            "speedup": round(t_seq / dt, 1), "max_abs_diff": float(np.max(np.abs(e[:k] - np.array(seq))))}
+
+
+SECTIONS = ("adjoint", "sampling", "qaoa", "dm", "wide", "dynamic", "batch")
+TUNE_CANDIDATES = [(12, 3), (12, 4), (12, 5), (13, 3), (13, 4), (11, 4)]
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main() -> int:
@@ -258,8 +453,10 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adj-qubits", type=int, default=24)
     ap.add_argument("--ps-params", type=int, default=16)
-    ap.add_argument("--no-extra", action="store_true",
-                    help="skip the 20q sampling, batched QAOA and density-matrix sections")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline and the adjoint section")
+    ap.add_argument("--sections", default=",".join(SECTIONS),
+                    help=f"comma-separated subset of {','.join(SECTIONS)} (or 'none')")
+    ap.add_argument("--no-tune", action="store_true", help="use the library's default tile shape")
     ap.add_argument("--calib", action="store_true",
                     help="also run one read-only sweep (norm) and one write-only sweep (reset) of the state, "
                          "whose HBM bytes are known, to scale PMC byte counters")
@@ -267,26 +464,38 @@ def main() -> int:
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
         return 2
+    sections = set() if args.sections == "none" else set(args.sections.split(","))
+    if args.no_extra:
+        sections &= {"adjoint"}
+    if args.no_adjoint:
+        sections.discard("adjoint")
 
     from circuits import config2_circuit
     from goqu_amd import _native as N
+    from goqu_amd import autotune
     from goqu_amd import statevector
 
     n = args.qubits
     circ = config2_circuit(n, args.layers, seed=2)
+    plan, tune_times = (0, 0), {}
+    if not args.no_tune:
+        _log("autotuning the tile shape")
+        plan, tune_times = autotune.tune(circ, workload=f"config2-depth{args.layers}", candidates=TUNE_CANDIDATES,
+                                         reps=1)
     buf = N.OpBuffer()
     for op in circ.Ops():
         if op.Gate is not None:
             N.add_sv_gate(buf, op.Gate, op.Qubits)
     arr = buf.array()
     ngates = len(buf)
-    sim = statevector.New(n)
+    sim = statevector.New(n, plan=plan)
     lib = N.lib()
 
     def step():
         N.check(lib.goqu_sv_reset(sim._h))
         N.check(lib.goqu_sv_apply(sim._h, arr, ngates))
 
+    _log("headline")
     for _ in range(args.warmup):
         step()
     N.profile(True)
@@ -324,20 +533,32 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"statevector {n}q complex128 depth-{args.layers} U3 + CZ/CNOT/SWAP/CP random circuit",
                    "qubits": n, "layers": args.layers, "gates": ngates, "stage_sweeps": sweeps_per_step,
-                   "gate_equiv_GBps": round(gate_equiv, 1), "norm2": norm},
+                   "gate_equiv_GBps": round(gate_equiv, 1), "norm2": norm,
+                   "tile_shape": {"tile_bits": plan[0], "low_bits": plan[1], "autotuned": not args.no_tune,
+                                  "candidates_s": {k: round(v, 4) for k, v in tune_times.items()}}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(n, args.layers),
                      "kernel": "k_stage<double>", "avg_launch_ms": round(avg_launch_ms, 4)},
         "cpu_baseline": None,
     }
-    if not args.no_adjoint:
-        out["adjoint"] = adjoint_section(args.adj_qubits, 7, 200, args.ps_params)
-    if not args.no_extra:
-        out["sampling"] = sampling_section()
-        out["qaoa_batch"] = qaoa_batch_section()
-        out["density_matrix"] = dm_section()
+    runners = [
+        ("adjoint", lambda: adjoint_section(args.adj_qubits, 7, 200, args.ps_params)),
+        ("sampling", sampling_section),
+        ("qaoa", qaoa_batch_section),
+        ("dm", dm_section),
+        ("wide", wide_section),
+        ("dynamic", dynamic_section),
+        ("batch", batch_section),
+    ]
+    keys = {"qaoa": "qaoa_batch", "dm": "density_matrix", "batch": "circuit_batch", "dynamic": "dynamic_circuits",
+            "wide": "wide_mfma"}
+    for name, fn in runners:
+        if name in sections:
+            _log(name)
+            out[keys.get(name, name)] = fn()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(circ, n, args.cpu_budget)
+        _log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(circ, n, args.cpu_budget, args.layers)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -28,6 +28,19 @@ class GoquOp(C.Structure):
     ]
 
 
+# GoquOp as a numpy record (the layout of goqu_op_t)
+OP_DTYPE = np.dtype([("kind", "<i4"), ("gate_id", "<i4"), ("num_qubits", "<i4"), ("num_controls", "<i4"),
+                     ("qubits", "<i4", (MAX_OP_QUBITS,)), ("matrix", "<u8"), ("num_kraus", "<i4"),
+                     ("reserved", "<i4")])
+assert OP_DTYPE.itemsize == C.sizeof(GoquOp)
+
+
+def op_array(records: np.ndarray):
+    """A ctypes GoquOp array viewing a contiguous OP_DTYPE numpy array (no copy)."""
+    records = np.ascontiguousarray(records)
+    return (GoquOp * len(records)).from_buffer(records)
+
+
 class GoquDeriv(C.Structure):
     _fields_ = [("op", C.c_int32), ("param", C.c_int32), ("scale", C.c_double), ("dmatrix", C.POINTER(C.c_double))]
 
@@ -56,6 +69,7 @@ _I64 = C.POINTER(C.c_int64)
 _SIGS = {
     "goqu_last_error": ([], C.c_char_p),
     "goqu_device_count": ([_I32], C.c_int),
+    "goqu_device_name": ([C.c_char_p, C.c_int32], C.c_int),
     "goqu_sv_create": ([C.c_int32, C.c_int32, C.POINTER(_P)], C.c_int),
     "goqu_sv_destroy": ([_P], C.c_int),
     "goqu_sv_reset": ([_P], C.c_int),
@@ -102,6 +116,7 @@ _SIGS = {
     "goqu_profile_enable": ([C.c_int32], C.c_int),
     "goqu_profile_reset": ([], C.c_int),
     "goqu_profile_read": ([_D, _I64, _D], C.c_int),
+    "goqu_profile_read_kernel": ([C.c_int32, _D, _I64, _D], C.c_int),
 }
 
 
@@ -162,26 +177,40 @@ class OpBuffer:
         if matrix is not None:
             m = np.ascontiguousarray(np.asarray(matrix, dtype=np.complex128).reshape(-1)).view(np.float64)
             self.keep.append(m)
-        self.records.append((kind, gate_id, list(qubits), num_controls, m, num_kraus))
+        qs = list(qubits)
+        if len(qs) > MAX_OP_QUBITS:
+            raise ValueError(f"op acts on {len(qs)} qubits (max {MAX_OP_QUBITS})")
+        self.records.append((kind, gate_id, qs, num_controls, m, num_kraus, m.ctypes.data if m is not None else 0))
         return len(self.records) - 1
 
     def __len__(self):
         return len(self.records)
 
+    def structured(self) -> np.ndarray:
+        """The records as a numpy array of OP_DTYPE (matrix pointers into self.keep)."""
+        n = len(self.records)
+        out = np.zeros(max(1, n), dtype=OP_DTYPE)
+        if not n:
+            return out
+        R = self.records
+        out["kind"][:n] = [r[0] for r in R]
+        out["gate_id"][:n] = [r[1] for r in R]
+        out["num_qubits"][:n] = [len(r[2]) for r in R]
+        out["num_controls"][:n] = [r[3] for r in R]
+        out["num_kraus"][:n] = [r[5] for r in R]
+        out["matrix"][:n] = [r[6] for r in R]
+        q = out["qubits"]
+        for k in range(MAX_OP_QUBITS):
+            rows = [i for i, r in enumerate(R) if len(r[2]) > k]
+            if not rows:
+                break
+            q[rows, k] = [R[i][2][k] for i in rows]
+        return out
+
     def array(self):
-        arr = (GoquOp * max(1, len(self.records)))()
-        for i, (kind, gid, qs, nc, m, nk) in enumerate(self.records):
-            if len(qs) > MAX_OP_QUBITS:
-                raise ValueError(f"op acts on {len(qs)} qubits (max {MAX_OP_QUBITS})")
-            arr[i].kind = kind
-            arr[i].gate_id = gid
-            arr[i].num_qubits = len(qs)
-            arr[i].num_controls = nc
-            for j, q in enumerate(qs):
-                arr[i].qubits[j] = q
-            arr[i].matrix = dptr(m) if m is not None else None
-            arr[i].num_kraus = nk
-        return arr
+        """A ctypes GoquOp array over the records (it keeps its numpy storage alive)."""
+        st = self.structured()
+        return (GoquOp * len(st)).from_buffer(st)
 
 
 def add_sv_gate(buf: OpBuffer, g: G.Gate, qubits: Sequence[int]) -> int:
@@ -214,6 +243,16 @@ def plan_stats(nq: int, buf: OpBuffer, density: bool = False, tile_bits: int = 0
 def profile(enable: bool) -> None:
     check(lib().goqu_profile_enable(1 if enable else 0))
     check(lib().goqu_profile_reset())
+
+
+PROF_STAGE, PROF_STAGE_DM, PROF_STAGE_PAIR, PROF_HAMILTONIAN, PROF_WIDE, PROF_QAOA = range(6)
+
+
+def profile_read_kernel(kernel: int):
+    """(summed ms, launches, algorithmic bytes or flops) of one profiled kernel."""
+    ms, n, b = C.c_double(0), C.c_int64(0), C.c_double(0)
+    check(lib().goqu_profile_read_kernel(kernel, C.byref(ms), C.byref(n), C.byref(b)))
+    return ms.value, n.value, b.value
 
 
 def profile_read():

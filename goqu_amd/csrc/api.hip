@@ -8,6 +8,8 @@
 #include <new>
 #include <stdexcept>
 
+#include <cstdio>
+
 #include "runtime.h"
 
 namespace goqu {
@@ -176,6 +178,18 @@ int goqu_device_count(int32_t *out) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
     *out = e == hipSuccess ? n : 0;
+    return GOQU_OK;
+    GOQU_GUARD_END
+}
+
+int goqu_device_name(char *buf, int32_t len) {
+    GOQU_GUARD_BEGIN
+    if (!buf || len < 1) return fail(GOQU_ERR_INVALID_ARG, "null or empty buffer");
+    int dev = 0;
+    hipDeviceProp_t p;
+    GOQU_HIP(hipGetDevice(&dev));
+    GOQU_HIP(hipGetDeviceProperties(&p, dev));
+    std::snprintf(buf, (size_t)len, "%s", p.gcnArchName);
     return GOQU_OK;
     GOQU_GUARD_END
 }

@@ -496,7 +496,9 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (ce != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
     ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipEvent_t ev0 = prof_begin(s);
     if (ce == hipSuccess) ce = hipGraphLaunch(exec, s);
+    prof_end(ev0, s, PROF_QAOA, 0.0);
     if (ce == hipSuccess) ce = hipStreamSynchronize(s);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);

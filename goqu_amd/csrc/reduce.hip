@@ -721,6 +721,7 @@ int apply_hamiltonian_tiled(const State &psi, State &lam, int nt, const uint64_t
         const HGroup *dg = reinterpret_cast<const HGroup *>(dev + off_g[p]);
         const uint64_t *dz = reinterpret_cast<const uint64_t *>(dev + off_z);
         const double *dc = reinterpret_cast<const double *>(dev + off_c);
+        hipEvent_t ev0 = prof_begin(s);
         if (psi.prec == GOQU_C128)
             hipLaunchKernelGGL(k_pauli_apply_tile<double>, dim3(grid), dim3(256), 0, s,
                                static_cast<const cx<double> *>(psi.amps), static_cast<cx<double> *>(lam.amps), dq, dg,
@@ -730,6 +731,8 @@ int apply_hamiltonian_tiled(const State &psi, State &lam, int nt, const uint64_t
                                static_cast<const cx<float> *>(psi.amps), static_cast<cx<float> *>(lam.amps), dq, dg, ng,
                                dz, dc, p > 0 ? 1 : 0);
         GOQU_HIP(hipGetLastError());
+        // psi read, lambda written (and read back after the first pass)
+        prof_end(ev0, s, PROF_HAMILTONIAN, (p > 0 ? 3.0 : 2.0) * psi.size() * psi.elem);
     }
     GOQU_HIP(hipStreamSynchronize(s));  // the host staging buffer is released on return
     done = true;

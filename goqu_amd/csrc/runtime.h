@@ -117,6 +117,22 @@ struct PreparedOps {
     ~PreparedOps() { release(); }
 };
 
+// ---- kernel profiling (profile.hip; goqu_profile_*) -------------------------------------------
+enum ProfBin {
+    PROF_STAGE = 0,        // k_stage / k_stage_big (statevector stages, single or batched)
+    PROF_STAGE_DM = 1,     // k_stage_dm
+    PROF_STAGE_PAIR = 2,   // k_stage_pair (adjoint reverse sweeps)
+    PROF_HAMILTONIAN = 3,  // k_pauli_apply_tile (lambda = H psi)
+    PROF_WIDE = 4,         // k_wide_mfma (bytes: FP64 flops of the contraction)
+    PROF_QAOA = 5,         // one batched-QAOA launch graph
+    PROF_BINS = 6
+};
+bool profiling() noexcept;
+// nullptr when profiling is off; otherwise an event recorded on s
+hipEvent_t prof_begin(hipStream_t s);
+// records the end on s: when the totals are read, the span's time, one launch and `bytes` go to bin
+void prof_end(hipEvent_t start, hipStream_t s, int bin, double bytes);
+
 // ---- launchers (stream-ordered) -------------------------------------------------------------
 PlanConfig default_plan_config(int prec, int nq, bool paired);
 int upload_program(const Program &P, int prec, hipStream_t s, DeviceProgram &dp);

@@ -1108,19 +1108,6 @@ GOQU_PAIR_INST(extern, double)
 GOQU_PAIR_INST(extern, float)
 GOQU_STAGE_INST(extern, k_stage_dm, double)
 
-// ---- profiling ---------------------------------------------------------------------------------
-static std::atomic<bool> g_profile{false};
-static std::mutex g_prof_mu;
-static double g_prof_ms = 0, g_prof_bytes = 0;
-static int64_t g_prof_launches = 0;
-
-static void profile_add(double ms, int64_t launches, double bytes) {
-    std::lock_guard<std::mutex> lk(g_prof_mu);
-    g_prof_ms += ms;
-    g_prof_launches += launches;
-    g_prof_bytes += bytes;
-}
-
 // ---------------------------------------------------------------------------------------------
 // Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS) holding the 4 lowest qubits.  On the
 // 30-qubit depth-40 circuit: 61 sweeps of 7.85 ms (485 ms per Evolve) against 58 of 8.37 ms (492 ms)
@@ -1219,11 +1206,10 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, cons
     const StageHeader *hdrs = reinterpret_cast<const StageHeader *>(blob + dp.hdr_off);
     const Instr *instrs = reinterpret_cast<const Instr *>(blob + dp.instr_off);
     const R *pool = reinterpret_cast<const R *>(blob + dp.pool_off);
-    const bool prof = g_profile.load();
     const unsigned nshots = b ? (unsigned)b->nshots : 1u;
     const uint8_t *act = b ? b->act : nullptr;
     const uint64_t stride = b ? b->stride : 0, bstride = b ? b->blob_stride : 0;
-    std::vector<hipEvent_t> ev;
+    const double sweep_bytes = 2.0 * st.size() * sizeof(cx<R>) * nshots;  // read + write of every state
     for (size_t i = 0; i < dp.stages.size(); i++) {
         const int m = dp.stages[i].m;
         if (stage_is_identity(dp.stages[i])) continue;
@@ -1234,12 +1220,7 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, cons
         // workgroups sized to the device, whose waves stay in step and load in bursts
         const uint64_t cap = 1ull << 20;
         const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
-        if (prof) {
-            ev.resize(ev.size() + 2);
-            GOQU_HIP(hipEventCreate(&ev[ev.size() - 2]));
-            GOQU_HIP(hipEventCreate(&ev[ev.size() - 1]));
-            GOQU_HIP(hipEventRecord(ev[ev.size() - 2], s));
-        }
+        hipEvent_t ev0 = prof_begin(s);
         const dim3 g(grid, nshots);
         if (dp.stages[i].flags & SF_DM) {
             if constexpr (sizeof(R) == 8)
@@ -1254,20 +1235,7 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, cons
             hipLaunchKernelGGL(k_stage<R>, g, dim3(threads), lds, s, static_cast<cx<R> *>(st.amps), hdrs + i, instrs,
                                pool, ntiles, act, stride, bstride);
         GOQU_HIP(hipGetLastError());
-        if (prof) GOQU_HIP(hipEventRecord(ev.back(), s));
-    }
-    if (prof && !ev.empty()) {
-        GOQU_HIP(hipStreamSynchronize(s));
-        double ms_total = 0;
-        for (size_t i = 0; i < ev.size(); i += 2) {
-            float ms = 0;
-            GOQU_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-            ms_total += ms;
-            (void)hipEventDestroy(ev[i]);
-            (void)hipEventDestroy(ev[i + 1]);
-        }
-        profile_add(ms_total, (int64_t)(ev.size() / 2),
-                    (double)(ev.size() / 2) * 2.0 * st.size() * sizeof(cx<R>) * nshots);
+        prof_end(ev0, s, (dp.stages[i].flags & SF_DM) ? PROF_STAGE_DM : PROF_STAGE, sweep_bytes);
     }
     return GOQU_OK;
 }
@@ -1321,9 +1289,11 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
         if (stage_is_identity(H)) continue;
         const unsigned threads = 1u << (m - kRegBits);
         const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
+        hipEvent_t ev0 = prof_begin(s);
         hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),
                            static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
         if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "paired stage launch failed");
+        prof_end(ev0, s, PROF_STAGE_PAIR, 4.0 * psi.size() * sizeof(cx<R>));  // read + write of psi and lambda
         if (!rc && H.ngrad) {
             hipLaunchKernelGGL(k_sum_rows, dim3(H.ngrad), dim3(256), 0, s, gpart, (int64_t)grid, gsum + H.grad_base);
             if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "gradient reduction launch failed");
@@ -1440,29 +1410,4 @@ int launch_prepared(State &st, const PreparedOps &p, hipStream_t s, const ShotBa
 
 }  // namespace goqu
 
-extern "C" {
-int goqu_profile_enable(int32_t on) {
-    GOQU_GUARD_BEGIN
-    goqu::g_profile.store(on != 0);
-    return GOQU_OK;
-    GOQU_GUARD_END
-}
-int goqu_profile_reset(void) {
-    GOQU_GUARD_BEGIN
-    std::lock_guard<std::mutex> lk(goqu::g_prof_mu);
-    goqu::g_prof_ms = goqu::g_prof_bytes = 0;
-    goqu::g_prof_launches = 0;
-    return GOQU_OK;
-    GOQU_GUARD_END
-}
-int goqu_profile_read(double *kernel_ms, int64_t *launches, double *bytes) {
-    GOQU_GUARD_BEGIN
-    std::lock_guard<std::mutex> lk(goqu::g_prof_mu);
-    if (kernel_ms) *kernel_ms = goqu::g_prof_ms;
-    if (launches) *launches = goqu::g_prof_launches;
-    if (bytes) *bytes = goqu::g_prof_bytes;
-    return GOQU_OK;
-    GOQU_GUARD_END
-}
-}
 #endif  // GOQU_STAGE_KERNEL

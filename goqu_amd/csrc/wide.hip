@@ -154,6 +154,7 @@ int launch_wide(void *amps, int prec, const WideOp &w, hipStream_t s, const Shot
     const size_t lds = (size_t)D * 16 * 2 * sizeof(double);
     const uint64_t *rb = reinterpret_cast<const uint64_t *>(dev);
     const uint8_t *fq = reinterpret_cast<const uint8_t *>(dev + w.off_f);
+    hipEvent_t ev0 = prof_begin(s);
     if (prec == GOQU_C128)
         hipLaunchKernelGGL(k_wide_mfma<double>, dim3(grid, shots), dim3(256), lds, s, static_cast<cx<double> *>(amps),
                            w.k, rb, fq, w.nfree, w.cmask, w.ncols, dur, dur + mat, dur + 2 * mat, act, stride);
@@ -161,6 +162,9 @@ int launch_wide(void *amps, int prec, const WideOp &w, hipStream_t s, const Shot
         hipLaunchKernelGGL(k_wide_mfma<float>, dim3(grid, shots), dim3(256), lds, s, static_cast<cx<float> *>(amps),
                            w.k, rb, fq, w.nfree, w.cmask, w.ncols, dur, dur + mat, dur + 2 * mat, act, stride);
     GOQU_HIP(hipGetLastError());
+    // FP64 flops of the contraction: 2^k x 2^k complex times 2^k x ncols, 8 real flops per complex
+    // multiply-add, for every shot of the batch
+    prof_end(ev0, s, PROF_WIDE, 8.0 * D * D * (double)w.ncols * shots);
     return GOQU_OK;
 }
 

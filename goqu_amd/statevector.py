@@ -41,7 +41,7 @@ def _uniforms(shots: int, uniforms=None, rng=None) -> np.ndarray:
 class Sim:
     """sim.Simulator implementation (sim/sim.go:9-22) backed by a device state vector."""
 
-    def __init__(self, num_qubits: int, precision: int = N.C128):
+    def __init__(self, num_qubits: int, precision: int = N.C128, plan=None):
         if num_qubits < 1 or num_qubits > 34:
             raise ValueError(f"statevector: numQubits {num_qubits} out of range [1, 34]")
         self.num_qubits = num_qubits
@@ -49,6 +49,16 @@ class Sim:
         h = C.c_void_p()
         N.check(N.lib().goqu_sv_create(num_qubits, precision, C.byref(h)))
         self._h = h
+        if plan == "tuned":  # the autotuner's persisted shape for this size, if any
+            from . import autotune
+
+            plan = autotune.plan_for(num_qubits, precision)
+        if plan is not None:
+            self.SetPlan(*plan)
+
+    def SetPlan(self, tile_bits: int, low_bits: int) -> None:
+        """Tile shape of this simulator's programs (goqu_sv_set_plan; (0, 0) = the default)."""
+        N.check(N.lib().goqu_sv_set_plan(self._h, int(tile_bits), int(low_bits)))
 
     # -- lifecycle --
     def Close(self) -> None:
@@ -221,6 +231,7 @@ class Sim:
         return out.value
 
 
-def New(num_qubits: int, precision: int = N.C128) -> Sim:
-    """statevector.New (sim.go:27-35)."""
-    return Sim(num_qubits, precision)
+def New(num_qubits: int, precision: int = N.C128, plan=None) -> Sim:
+    """statevector.New (sim.go:27-35); plan = (tile_bits, low_bits), or "tuned" for the
+    autotuner's choice."""
+    return Sim(num_qubits, precision, plan)

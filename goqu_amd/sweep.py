@@ -220,8 +220,13 @@ def _run(ctx, c: Circuit, shots: int, sw: Sweep, density: bool, nm, rng, uniform
                     s.Close()
             elif live and shots > 0:
                 u = uniforms[live] if uniforms is not None else rng.random((len(live), shots))
-                circs = [bound[i] for i in live]
-                idx = B.dm_sample(circs, shots, nm, uniforms=u) if density else B.sample(circs, shots, uniforms=u)
+                if density:
+                    idx = B.dm_sample([bound[i] for i in live], shots, nm, uniforms=u)
+                else:  # the points' op records straight from the parameter columns
+                    names = FreeParameters(c)
+                    params = np.array([[results[i].Bindings[k] for k in names] for i in live],
+                                      dtype=np.float64).reshape(len(live), len(names))
+                    idx = B.param_sample(c, params, shots, names, uniforms=u)
                 for k, i in enumerate(live):
                     results[i].Counts = _counts(idx[k], nq)
         except Exception as e:  # noqa: BLE001

@@ -80,6 +80,9 @@ typedef struct goqu_op {
 /* ---- library ------------------------------------------------------------------------------ */
 const char *goqu_last_error(void);
 int goqu_device_count(int32_t *out);
+/* Architecture name of the current device (e.g. "gfx950:sramecc+:xnack-"), NUL-terminated in
+ * buf[0 .. len); the autotuner's cache key. */
+int goqu_device_name(char *buf, int32_t len);
 
 /* ---- statevector (replaces sim/statevector.Sim, sim/statevector/sim.go:21-383) ------------ */
 typedef struct goqu_sv goqu_sv_t;
@@ -275,13 +278,24 @@ int goqu_plan_stats(int32_t num_qubits, int32_t density, const goqu_op_t *ops, i
 int goqu_debug_inject_fault(int32_t kind, int32_t countdown);
 
 /* ---- profiling (no counterpart in the reference; for benchmarks) -------------------------- */
-/* When enabled, every stage-kernel launch is bracketed by HIP events on its own stream; the
- * totals accumulate process-wide until goqu_profile_reset. */
+/* When enabled, every launch of the profiled kernels is bracketed by HIP events on its own stream;
+ * the totals accumulate process-wide until goqu_profile_reset. */
 int goqu_profile_enable(int32_t on);
 int goqu_profile_reset(void);
 /* kernel_ms: summed event time of the stage kernels; launches: their count; bytes: their
  * algorithmic HBM bytes (read + write of the whole state per launch). */
 int goqu_profile_read(double *kernel_ms, int64_t *launches, double *bytes);
+/* The same per kernel: 0 k_stage / k_stage_big, 1 k_stage_dm, 2 k_stage_pair (bytes: read + write
+ * of psi and lambda), 3 k_pauli_apply_tile (psi read, lambda written, and read after a group's first
+ * pass), 4 k_wide_mfma (bytes holds its FP64 flops), 5 one batched-QAOA launch graph (no bytes).
+ * Reading resolves the recorded events, which synchronises the device. */
+#define GOQU_PROF_STAGE 0
+#define GOQU_PROF_STAGE_DM 1
+#define GOQU_PROF_STAGE_PAIR 2
+#define GOQU_PROF_HAMILTONIAN 3
+#define GOQU_PROF_WIDE 4
+#define GOQU_PROF_QAOA 5
+int goqu_profile_read_kernel(int32_t kernel, double *kernel_ms, int64_t *launches, double *bytes);
 
 #ifdef __cplusplus
 }

@@ -761,3 +761,35 @@ double orc_dm_fidelity(const double *rho_d, int nq, const double *pure_d) {
         for (int64_t j = 0; j < dim; j++) f = cadd(f, cmul(cmul(cconj(ps[i]), rho[i * dim + j]), ps[j]));
     return fabs(f.re);
 }
+
+/* ======================================================================================
+ * Host memory bandwidth, to label the CPU baseline: a STREAM-style copy b[i] = a[i] of n
+ * doubles by `threads` OpenMP threads (first touch in parallel), best of `reps`, in GB/s of
+ * bytes read + written.  Not part of the reference.
+ * ====================================================================================== */
+double orc_stream_copy(int64_t n, int threads, int reps) {
+    double *a = (double *)malloc((size_t)n * sizeof(double));
+    double *b = (double *)malloc((size_t)n * sizeof(double));
+    if (!a || !b) {
+        free(a);
+        free(b);
+        return 0.0;
+    }
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        a[i] = (double)i;
+        b[i] = 0.0;
+    }
+    double best = 1e30;
+    for (int r = 0; r < reps; r++) {
+        double t0 = omp_get_wtime();
+#pragma omp parallel for num_threads(threads) schedule(static)
+        for (int64_t i = 0; i < n; i++) b[i] = a[i];
+        double dt = omp_get_wtime() - t0;
+        if (dt < best) best = dt;
+    }
+    double sink = b[n / 2];
+    free(a);
+    free(b);
+    return sink >= 0 ? 2.0 * (double)n * sizeof(double) / best / 1e9 : 0.0;
+}

@@ -42,6 +42,8 @@ def lib():
         L.orc_sv_expectation_value.restype = C.c_double
         L.orc_sv_measure_qubit.argtypes = [_D, C.c_int, C.c_int, C.c_double]
         L.orc_sv_measure_qubit.restype = C.c_int
+        L.orc_stream_copy.argtypes = [C.c_int64, C.c_int, C.c_int]
+        L.orc_stream_copy.restype = C.c_double
         L.orc_dm_measure_qubit.argtypes = [_D, C.c_int, C.c_int, C.c_double]
         L.orc_dm_measure_qubit.restype = C.c_int
         L.orc_dm_apply.argtypes = [_D, C.c_int, C.POINTER(N.GoquOp), C.c_int]
@@ -200,3 +202,8 @@ def dm_purity(rho: np.ndarray, n: int) -> float:
 def dm_fidelity(rho: np.ndarray, n: int, pure: np.ndarray) -> float:
     p = np.ascontiguousarray(pure, dtype=np.complex128)
     return lib().orc_dm_fidelity(_dp(rho), n, _dp(p))
+
+
+def stream_copy_gbps(n_doubles: int = 1 << 27, threads: int = 1, reps: int = 3) -> float:
+    """Host copy bandwidth (bytes read + written per second, GB/s) with `threads` threads."""
+    return lib().orc_stream_copy(n_doubles, threads, reps)

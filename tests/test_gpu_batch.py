@@ -206,3 +206,24 @@ def test_mitigation_executors_and_local_backend():
     assert set(counts) <= {"00", "11"} and sum(counts.values()) == 400
     assert seen[0].NumQubits == 2 and seen[0].Shots == 400 and seen[1] is None
     assert be.Status(ctx, job.ID).State == BK.StateCompleted
+
+
+def test_autotuner_picks_a_measured_shape_and_results_agree(tmp_path, monkeypatch):
+    """tune() times the candidates on the device, persists the fastest, and a simulator built with
+    plan="tuned" gives the default shape's amplitudes to rounding."""
+    from goqu_amd import autotune as AT
+    from goqu_amd import statevector
+
+    monkeypatch.setenv("GOQU_TUNE_CACHE", str(tmp_path / "tune.json"))
+    c = random_circuit(16, 6, np.random.default_rng(800), three_q=False)
+    best, times = AT.tune(c, candidates=[(10, 3), (11, 4), (12, 4)], reps=1)
+    assert best in [(10, 3), (11, 4), (12, 4)] and len(times) == 3
+    assert min(times.values()) == times[f"{best[0]},{best[1]}"]
+    assert AT.plan_for(16) == best
+    a = statevector.New(16, plan="tuned")
+    b = statevector.New(16)
+    a.Evolve(c)
+    b.Evolve(c)
+    assert np.max(np.abs(a.StateVector() - b.StateVector())) < 1e-12
+    a.Close()
+    b.Close()

@@ -91,3 +91,51 @@ def test_gate_count_counts_control_flow_bodies():
     b = NewBuilder("cf", 2).WithClbits(1).H(0).Measure(0, 0)
     b.IfElseBlock(0, 1, lambda x: x.X(1).Z(1), lambda x: x.Y(1))
     assert BK._gate_count(b.Build()) == 3 + 3
+
+
+def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
+    """The tuner's JSON cache (GOQU_TUNE_CACHE): a stored choice reads back; a corrupt file
+    reads as empty; shapes outside the kernel's limits are never candidates."""
+    from goqu_amd import autotune as AT
+
+    monkeypatch.setenv("GOQU_TUNE_CACHE", str(tmp_path / "t.json"))
+    assert AT._load() == {}
+    AT._store({"k": {"tile_bits": 11, "low_bits": 3}})
+    assert AT._load()["k"]["tile_bits"] == 11
+    (tmp_path / "t.json").write_text("{not json")
+    assert AT._load() == {}
+    assert all(8 <= t <= 13 and 2 <= l <= min(6, t - 4) for t, l in AT.DEFAULT_CANDIDATES)
+    assert not AT._valid((13, 4), 10) and AT._valid((10, 4), 10) and not AT._valid((12, 9), 30)
+
+
+def test_param_records_equal_bound_circuit_records():
+    """The vectorised parameter-batch marshalling gives, row by row, the op records (kinds, tiers,
+    qubits) and matrices of binding each row and marshalling the bound circuit."""
+    import ctypes as C
+
+    import numpy as np
+
+    from goqu_amd import batch as B
+    from goqu_amd import qaoa as Q
+    from goqu_amd.circuit import Add, FreeParameters, Literal, Mul
+
+    c, npar = Q.EfficientSU2(5, 2)
+    th = New("extra")
+    b = NewBuilder("more", 5)
+    for op in c.Ops():
+        b.ops.append(op)
+    b.SymCP(Mul(Literal(2.0), th), 0, 3).SymRXX(Add(th, Literal(0.5)), 1, 2).SymU3(th, th, th, 4)
+    b.SymRot(th, th, th, 2).SymFSim(th, th, 0, 1).H(2).SymRZZ(th, 3, 4).SymRYY(th, 0, 4).SymPhase(th, 1)
+    c2 = b.Build()
+    names = FreeParameters(c2)
+    params = np.random.default_rng(1).uniform(0, 6.28, (4, len(names)))
+    R = B.ParamRecords(c2, params, names)
+    buf, _ = B._marshal_sv(B.bind_all(c2, params, names))
+    st = buf.structured()
+    assert len(st) == len(R.records)
+    for a, r in zip(st, R.records):
+        assert a["kind"] == r["kind"] and a["gate_id"] == r["gate_id"] and (a["qubits"] == r["qubits"]).all()
+        k = (1 << a["num_qubits"]) ** 2
+        ma = np.ctypeslib.as_array((C.c_double * (2 * k)).from_address(int(a["matrix"])))
+        mr = np.ctypeslib.as_array((C.c_double * (2 * k)).from_address(int(r["matrix"])))
+        assert np.max(np.abs(ma - mr)) < 1e-15
