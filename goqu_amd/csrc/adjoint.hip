@@ -210,6 +210,10 @@ static int alloc_like(State &s, const State &ref) {
     return GOQU_OK;
 }
 
+// Reverse sweep by segments of `every` ops: rho at the start of every segment is stored on the
+// forward pass; going backwards, each segment that holds derivatives is replayed once from its
+// checkpoint, keeping rho_{k-1} of each derivative op k in a snapshot, and the sandwiches of an op's
+// derivatives (U3: three) share one pass over h and the snapshot.
 int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_deriv_t *derivs, int nd, int np,
                int nt, const uint64_t *x, const uint64_t *z, const int32_t *ny, const double *coeffs, int every,
                double *energy, double *grads, Scratch &scr, hipStream_t s) {
@@ -233,21 +237,33 @@ int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_der
         int rc = lower_dm_ops(&ops[k], 1, n, low[k], err);
         if (rc) return fail(rc, "op " + std::to_string(k) + ": " + err);
     }
-    // forward with checkpoints; widen the spacing until they fit in free device memory (h, the
-    // work copy and two more states stay free)
+    // segment length: checkpoints plus one segment's snapshots must fit in free device memory
+    // (h, the replay state and two more states stay free)
+    auto snaps_of = [&](int ev) {
+        int mx = 0;
+        for (int c0 = 0; c0 < nops; c0 += ev) {
+            int cnt = 0;
+            for (int k = c0; k < std::min(nops, c0 + ev); k++) cnt += by_op[k].empty() ? 0 : 1;
+            mx = std::max(mx, cnt);
+        }
+        return mx;
+    };
     {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(GOQU_ERR_DEVICE, "hipMemGetInfo failed");
         const size_t sbytes = rho.size() * rho.elem;
-        const size_t avail = fr > 4 * sbytes ? fr - 4 * sbytes : 0;
-        const int64_t max_ck = std::max<int64_t>(1, (int64_t)(avail / sbytes));
-        every = (int)std::max<int64_t>(every, (nops + max_ck - 1) / max_ck);
+        const int64_t max_states = fr > 4 * sbytes ? (int64_t)((fr - 4 * sbytes) / sbytes) : 0;
+        while (every < std::max(nops, 1) && (nops + every - 1) / every + snaps_of(every) > max_states) every++;
+        if ((nops + every - 1) / every + snaps_of(every) > max_states)
+            return fail(GOQU_ERR_OUT_OF_MEMORY, "adjoint: checkpoints do not fit in device memory");
     }
     const int nck = (nops + every - 1) / every;
-    std::vector<State> ck(std::max(nck, 0));
+    std::vector<State> ck(std::max(nck, 0)), snap(snaps_of(every));
     State h, work;
     auto cleanup = [&]() {
         for (State &c : ck)
+            if (c.amps) (void)hipFree(c.amps);
+        for (State &c : snap)
             if (c.amps) (void)hipFree(c.amps);
         if (h.amps) (void)hipFree(h.amps);
         if (work.amps) (void)hipFree(work.amps);
@@ -260,6 +276,7 @@ int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_der
         for (int k = c * every; k < std::min(nops, (c + 1) * every); k++) seg.insert(seg.end(), low[k].begin(), low[k].end());
         if (!rc) rc = run_eops(rho, seg, s);
     }
+    for (size_t j = 0; j < snap.size() && !rc; j++) rc = alloc_like(snap[j], rho);
     if (!rc) rc = alloc_like(h, rho);
     if (!rc) rc = alloc_like(work, rho);
     if (!rc) rc = fill_basis(h, ~0ull, s);
@@ -285,49 +302,72 @@ int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_der
     for (int p = 0; p < np; p++) grads[p] = 0;
 
     std::vector<EOp> pend;
-    for (int k = nops - 1; k >= 0 && !rc; k--) {
-        if (!by_op[k].empty()) {
-            rc = run_eops(h, pend, s);
-            pend.clear();
-            const int c = k / every;
-            if (!rc) rc = copy_state(work, ck[c], s);
-            std::vector<EOp> seg;
-            for (int j = c * every; j < k; j++) seg.insert(seg.end(), low[j].begin(), low[j].end());
-            if (!rc) rc = run_eops(work, seg, s);
-            const goqu_op_t &op = ops[k];
-            const int kg = op.num_qubits, d = 1 << kg, D = d * d;
-            for (int i : by_op[k]) {
-                if (rc) break;
-                // dop = dU (x) conj(U) on (row qubits, column qubits)
-                EOp dop;
-                dop.kind = EOP_DENSE;
-                dop.k = 2 * kg;
-                for (int j = 0; j < kg; j++) {
-                    dop.q[j] = op.qubits[j] + n;
-                    dop.q[kg + j] = op.qubits[j];
-                }
-                dop.m.assign(2 * (size_t)D * D, 0.0);
-                const double *dU = derivs[i].dmatrix, *U = op.matrix;
-                for (int r1 = 0; r1 < d; r1++)
-                    for (int r0 = 0; r0 < d; r0++)
-                        for (int c1 = 0; c1 < d; c1++)
-                            for (int c0 = 0; c0 < d; c0++) {
-                                const double ar = dU[2 * (r1 * d + r0)], ai = dU[2 * (r1 * d + r0) + 1];
-                                const double br = U[2 * (c1 * d + c0)], bi = -U[2 * (c1 * d + c0) + 1];
-                                const size_t idx = (size_t)(r1 * d + c1) * D + (r0 * d + c0);
-                                dop.m[2 * idx] = ar * br - ai * bi;
-                                dop.m[2 * idx + 1] = ar * bi + ai * br;
-                            }
-                double sres[2];
-                rc = sandwich(h, work, dop, sres, scr, s);
-                if (!rc) grads[derivs[i].param] += derivs[i].scale * 2.0 * sres[0];
+    std::vector<double> sres(2 * kMaxSandwich);
+    for (int c = nck - 1; c >= 0 && !rc; c--) {
+        const int k0 = c * every, k1 = std::min(nops, (c + 1) * every);
+        // replay the segment from its checkpoint up to its last derivative op, keeping rho_{k-1}
+        // of every derivative op k
+        std::vector<int> dks;
+        for (int k = k0; k < k1; k++)
+            if (!by_op[k].empty()) dks.push_back(k);
+        if (!dks.empty()) {
+            rc = copy_state(work, ck[c], s);
+            int at = k0;
+            for (size_t j = 0; j < dks.size() && !rc; j++) {
+                std::vector<EOp> seg;
+                for (int k = at; k < dks[j]; k++) seg.insert(seg.end(), low[k].begin(), low[k].end());
+                rc = run_eops(work, seg, s);
+                if (!rc) rc = copy_state(snap[j], work, s);
+                at = dks[j];
             }
         }
-        for (int j = (int)low[k].size() - 1; j >= 0 && !rc; j--) {
-            EOp a;
-            rc = invert_eop(low[k][j], a, err);  // conjugate transpose = adjoint superoperator
-            if (rc) rc = fail(rc, err);
-            pend.push_back(a);
+        int sj = (int)dks.size() - 1;
+        for (int k = k1 - 1; k >= k0 && !rc; k--) {
+            if (!by_op[k].empty()) {
+                rc = run_eops(h, pend, s);
+                pend.clear();
+                const goqu_op_t &op = ops[k];
+                const int kg = op.num_qubits, d = 1 << kg, D = d * d;
+                const std::vector<int> &ent = by_op[k];
+                for (size_t e0 = 0; e0 < ent.size() && !rc; e0 += kMaxSandwich) {
+                    std::vector<EOp> dops;
+                    for (size_t e = e0; e < std::min(ent.size(), e0 + kMaxSandwich); e++) {
+                        // dop = dU (x) conj(U) on (row qubits, column qubits)
+                        EOp dop;
+                        dop.kind = EOP_DENSE;
+                        dop.k = 2 * kg;
+                        for (int j = 0; j < kg; j++) {
+                            dop.q[j] = op.qubits[j] + n;
+                            dop.q[kg + j] = op.qubits[j];
+                        }
+                        dop.m.assign(2 * (size_t)D * D, 0.0);
+                        const double *dU = derivs[ent[e]].dmatrix, *U = op.matrix;
+                        for (int r1 = 0; r1 < d; r1++)
+                            for (int r0 = 0; r0 < d; r0++)
+                                for (int c1 = 0; c1 < d; c1++)
+                                    for (int c0 = 0; c0 < d; c0++) {
+                                        const double ar = dU[2 * (r1 * d + r0)], ai = dU[2 * (r1 * d + r0) + 1];
+                                        const double br = U[2 * (c1 * d + c0)], bi = -U[2 * (c1 * d + c0) + 1];
+                                        const size_t idx = (size_t)(r1 * d + c1) * D + (r0 * d + c0);
+                                        dop.m[2 * idx] = ar * br - ai * bi;
+                                        dop.m[2 * idx + 1] = ar * bi + ai * br;
+                                    }
+                        dops.push_back(std::move(dop));
+                    }
+                    rc = sandwich_multi(h, snap[sj], dops, sres.data(), scr, s);
+                    for (size_t e = 0; e < dops.size() && !rc; e++) {
+                        const goqu_deriv_t &dv = derivs[ent[e0 + e]];
+                        grads[dv.param] += dv.scale * 2.0 * sres[2 * e];
+                    }
+                }
+                sj--;
+            }
+            for (int j = (int)low[k].size() - 1; j >= 0 && !rc; j--) {
+                EOp a;
+                rc = invert_eop(low[k][j], a, err);  // conjugate transpose = adjoint superoperator
+                if (rc) rc = fail(rc, err);
+                pend.push_back(a);
+            }
         }
     }
     cleanup();

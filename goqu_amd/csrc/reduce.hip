@@ -56,13 +56,21 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh /* >= 4*NV
     __syncthreads();
 }
 
-// partials[b*nv + k] -> out[k], fixed order
-__global__ void k_finish(const double *__restrict__ partials, int nblocks, int nv, double *__restrict__ out) {
-    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
-        double s = 0;
-        for (int b = 0; b < nblocks; b++) s += partials[(size_t)b * nv + k];
-        out[k] = s;
+// partials[b*nv + k] -> out[k]: one workgroup per value k, each thread a strided share of the
+// blocks, then a fixed-shape tree (deterministic)
+__global__ void __launch_bounds__(256) k_finish(const double *__restrict__ partials, int nblocks, int nv,
+                                                double *__restrict__ out) {
+    __shared__ double sh[256];
+    const int k = blockIdx.x;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += partials[(size_t)b * nv + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[k] = sh[0];
 }
 
 static unsigned red_grid(uint64_t work, unsigned per_block) {
@@ -235,7 +243,7 @@ static int pauli_sums_t(State &st, uint64_t x, const uint64_t *z, int nt, double
             hipLaunchKernelGGL((k_pauli_x<R, NTX>), dim3(grid), dim3(kRedThreads), 0, s,
                                static_cast<const cx<R> *>(st.amps), nbases, x, hb, dz, cnt, dpart, 0);
         GOQU_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, dpart, (int)grid, nv, dout);
+        hipLaunchKernelGGL(k_finish, dim3(nv), dim3(256), 0, s, dpart, (int)grid, nv, dout);
         GOQU_HIP(hipGetLastError());
         GOQU_HIP(hipMemcpyAsync(host.data(), dout, nv * sizeof(double), hipMemcpyDeviceToHost, s));
         GOQU_HIP(hipStreamSynchronize(s));
@@ -351,7 +359,7 @@ __global__ void __launch_bounds__(kRedThreads) k_prob0(const cx<R> *__restrict__
 static int finish_to_host(const double *dpart, int grid, int nv, double *host, Scratch &scr, size_t out_off,
                           hipStream_t s) {
     double *dout = reinterpret_cast<double *>(static_cast<char *>(scr.ptr) + out_off);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, s, dpart, grid, nv, dout);
+    hipLaunchKernelGGL(k_finish, dim3(nv), dim3(256), 0, s, dpart, grid, nv, dout);
     GOQU_HIP(hipGetLastError());
     GOQU_HIP(hipMemcpyAsync(host, dout, nv * sizeof(double), hipMemcpyDeviceToHost, s));
     GOQU_HIP(hipStreamSynchronize(s));
@@ -821,6 +829,108 @@ static int sandwich_t(const State &lam, const State &psi, const EOp &e, double o
 int sandwich(const State &lam, const State &psi, const EOp &e, double out[2], Scratch &scr, hipStream_t s) {
     return psi.prec == GOQU_C128 ? sandwich_t<double>(lam, psi, e, out, scr, s)
                                  : sandwich_t<float>(lam, psi, e, out, scr, s);
+}
+
+// Several <lam| M_j |psi> at once, the M_j on the same qubits: one read of lam and psi for all.
+template <typename R, int K>
+__global__ void __launch_bounds__(kRedThreads) k_sandwich_multi(const cx<R> *__restrict__ lam,
+                                                                const cx<R> *__restrict__ psi, uint64_t ngroups,
+                                                                const int *__restrict__ qs,
+                                                                const int *__restrict__ sorted, uint64_t cmask,
+                                                                const double *__restrict__ M, int nm,
+                                                                double *__restrict__ partials) {
+    constexpr int D = 1 << K;
+    __shared__ double sh[4 * 2 * kMaxSandwich];
+    double v[2 * kMaxSandwich];
+#pragma unroll
+    for (int j = 0; j < 2 * kMaxSandwich; j++) v[j] = 0;
+    for (uint64_t gidx = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; gidx < ngroups;
+         gidx += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t base = gidx;
+#pragma unroll
+        for (int j = 0; j < K; j++) base = insert_zero(base, sorted[j]);
+        if ((base & cmask) != cmask) continue;
+        cx<R> p[D], l[D];
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            uint64_t i = base;
+#pragma unroll
+            for (int j = 0; j < K; j++)
+                if (r >> (K - 1 - j) & 1) i |= 1ull << qs[j];
+            p[r] = psi[i];
+            l[r] = lam[i];
+        }
+#pragma unroll
+        for (int m = 0; m < kMaxSandwich; m++) {
+            if (m >= nm) break;
+            const double *Mm = M + (size_t)m * 2 * D * D;
+#pragma unroll
+            for (int r = 0; r < D; r++) {
+                double sr = 0, si = 0;
+#pragma unroll
+                for (int c = 0; c < D; c++) {
+                    const double mr = Mm[2 * (r * D + c)], mi = Mm[2 * (r * D + c) + 1];
+                    sr += mr * p[c].x - mi * p[c].y;
+                    si += mr * p[c].y + mi * p[c].x;
+                }
+                v[2 * m] += (double)l[r].x * sr + (double)l[r].y * si;
+                v[2 * m + 1] += (double)l[r].x * si - (double)l[r].y * sr;
+            }
+        }
+    }
+    block_sum<2 * kMaxSandwich>(v, sh);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < 2 * nm; j++) partials[(size_t)blockIdx.x * 2 * kMaxSandwich + j] = v[j];
+}
+
+template <typename R>
+static int sandwich_multi_t(const State &lam, const State &psi, const std::vector<EOp> &ms, double *out,
+                            Scratch &scr, hipStream_t s) {
+    const EOp &e = ms[0];
+    const int K = e.k, D = 1 << K, nm = (int)ms.size();
+    const uint64_t ng = psi.size() >> K;
+    const unsigned grid = red_grid(ng, kRedThreads * 4);
+    const size_t hdr = 512;
+    const size_t mbytes = (size_t)nm * 2 * D * D * 8;
+    const size_t pbytes = (size_t)grid * 2 * kMaxSandwich * 8;
+    int rc = scr.ensure(hdr + mbytes + pbytes + 2 * kMaxSandwich * 8 + 64);
+    if (rc) return rc;
+    char *base = static_cast<char *>(scr.ptr);
+    std::vector<unsigned char> hbuf(hdr + mbytes, 0);
+    int *qs = reinterpret_cast<int *>(hbuf.data()), *sorted = qs + 8;
+    for (int j = 0; j < K; j++) qs[j] = sorted[j] = e.q[j];
+    std::sort(sorted, sorted + K);
+    for (int m = 0; m < nm; m++)
+        std::memcpy(hbuf.data() + hdr + (size_t)m * 2 * D * D * 8, ms[m].m.data(), (size_t)2 * D * D * 8);
+    GOQU_HIP(hipMemcpyAsync(base, hbuf.data(), hbuf.size(), hipMemcpyHostToDevice, s));
+    const int *dqs = reinterpret_cast<const int *>(base), *dsorted = dqs + 8;
+    const double *dM = reinterpret_cast<const double *>(base + hdr);
+    double *dpart = reinterpret_cast<double *>(base + hdr + mbytes);
+    const cx<R> *L = static_cast<const cx<R> *>(lam.amps), *P = static_cast<const cx<R> *>(psi.amps);
+    switch (K) {
+    case 1: hipLaunchKernelGGL((k_sandwich_multi<R, 1>), dim3(grid), dim3(kRedThreads), 0, s, L, P, ng, dqs, dsorted, e.cmask, dM, nm, dpart); break;
+    case 2: hipLaunchKernelGGL((k_sandwich_multi<R, 2>), dim3(grid), dim3(kRedThreads), 0, s, L, P, ng, dqs, dsorted, e.cmask, dM, nm, dpart); break;
+    case 3: hipLaunchKernelGGL((k_sandwich_multi<R, 3>), dim3(grid), dim3(kRedThreads), 0, s, L, P, ng, dqs, dsorted, e.cmask, dM, nm, dpart); break;
+    case 4: hipLaunchKernelGGL((k_sandwich_multi<R, 4>), dim3(grid), dim3(kRedThreads), 0, s, L, P, ng, dqs, dsorted, e.cmask, dM, nm, dpart); break;
+    default: return fail(GOQU_ERR_UNSUPPORTED, "derivative on more than 4 target qubits");
+    }
+    GOQU_HIP(hipGetLastError());
+    std::vector<double> host(2 * kMaxSandwich);
+    rc = finish_to_host(dpart, grid, 2 * kMaxSandwich, host.data(), scr, hdr + mbytes + pbytes, s);
+    if (rc) return rc;
+    for (int j = 0; j < 2 * nm; j++) out[j] = host[j];
+    return GOQU_OK;
+}
+
+int sandwich_multi(const State &lam, const State &psi, const std::vector<EOp> &ms, double *out, Scratch &scr,
+                   hipStream_t s) {
+    if (ms.empty()) return GOQU_OK;
+    if ((int)ms.size() > kMaxSandwich) return fail(GOQU_ERR_INVALID_ARG, "too many matrices for one sandwich");
+    for (const EOp &e : ms)
+        if (e.k != ms[0].k || e.cmask != ms[0].cmask || std::memcmp(e.q, ms[0].q, sizeof(e.q)) != 0)
+            return fail(GOQU_ERR_INVALID_ARG, "sandwich matrices on different qubits");
+    return psi.prec == GOQU_C128 ? sandwich_multi_t<double>(lam, psi, ms, out, scr, s)
+                                 : sandwich_multi_t<float>(lam, psi, ms, out, scr, s);
 }
 
 // ---- sampling: FP64 inverse CDF ---------------------------------------------------------------------

@@ -196,6 +196,10 @@ int apply_hamiltonian_tiled(const State &psi, State &lam, int nt, const uint64_t
                             const double *cfold, Scratch &scr, hipStream_t s, bool &done);
 // <lam| (dM on targets, under cmask) |psi>, dM dense k-qubit (k<=4) in gate order
 int sandwich(const State &lam, const State &psi, const EOp &dop, double out[2], Scratch &scr, hipStream_t s);
+// the same for up to kMaxSandwich matrices on the same qubits in one pass: out[2j], out[2j+1]
+constexpr int kMaxSandwich = 4;
+int sandwich_multi(const State &lam, const State &psi, const std::vector<EOp> &ms, double *out, Scratch &scr,
+                   hipStream_t s);
 int scale_state(State &st, double re, double im, hipStream_t s);
 int axpy_state(State &y, const State &x, double re, double im, hipStream_t s);  // y += c x
 
