@@ -16,10 +16,15 @@
 // Instances with n <= 12 run entirely in one workgroup's LDS (psi and lambda together).  Larger
 // instances are swept in tiles of 4096 amplitudes: phase A holds qubits 0..11, phase B the
 // qubits 0..L-1 and 12..n-1; the phases of all instances advance together, one launch per
-// phase, and the whole sequence is captured in a single hipGraph.
+// phase, and the whole sequence is captured in a single hipGraph.  A tile lives in registers, 16
+// amplitudes per thread: the mixer runs on register bits, and two or three LDS relayouts bring
+// the other tile bits into registers.  cut(z) is computed once per instance (first cost layer)
+// and kept as one byte per amplitude; the energy and lambda = H psi are formed when the reverse
+// sweep first loads psi.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
+#include <type_traits>
 
 #include "runtime.h"
 
@@ -36,6 +41,7 @@ struct QInst {
     uint32_t up[kQMaxQubits];           // neighbours j > i
     double gamma[kQMaxLayers], beta[kQMaxLayers];
     uint64_t psi, lam;                  // element offsets into the amplitude pool (large)
+    uint64_t cut;                       // byte offset of cut(z), z = 0..2^n-1, in the cut pool (large)
     int64_t part;                       // offset into the partials (large)
     int32_t ntiles, small;
 };
@@ -199,14 +205,112 @@ __device__ __forceinline__ uint64_t tile_global(const QInst &I, int phase, uint6
     return lo | (tile << Lb) | (hi << kQTileBits);
 }
 
+// Register tiles: thread t (lane, wave) holds 16 amplitudes of the tile; which 4 local bits its
+// register index r covers depends on the layout:
+//   0: r -> local bits 8-11, lane -> 0-5, wave -> 6-7   (loads and stores: lanes on the lowest bits)
+//   1: r -> local bits 0-3,  lane bits 4-5 -> 4-5, wave -> 6-7, lane bits 0-3 -> 8-11
+//   2: r -> local bits 4-7,  lane bits 4-5 -> 0-1, wave -> 2-3, lane bits 0-3 -> 8-11
+// The mixer acts on the register bits of each layout in turn; layouts change through LDS.  A local
+// index is the thread's part OR the register's part (disjoint bits), and the LDS swizzle is
+// XOR-linear, so an LDS address is (thread part) XOR (compile-time register part).
+template <int LAY>
+__device__ __forceinline__ uint32_t qthread(uint32_t lane, uint32_t wave) {
+    if constexpr (LAY == 0) return lane | (wave << 6);
+    else if constexpr (LAY == 1) return ((lane >> 4) << 4) | (wave << 6) | ((lane & 15) << 8);
+    else return (lane >> 4) | (wave << 2) | ((lane & 15) << 8);
+}
+template <int LAY>
+__device__ __forceinline__ constexpr uint32_t qreg(uint32_t r) {
+    return LAY == 0 ? (r << 8) : LAY == 1 ? r : (r << 4);
+}
+// LDS element of local index l: low bits XOR bits 8-11 (spreads the layouts' strided accesses)
+__device__ __forceinline__ constexpr uint32_t qswz(uint32_t l) { return l ^ ((l >> 8) & 15u); }
+
+// RX on register bit J of a 16-amplitude tile: [a0,a1] <- [c a0 - i s a1, -i s a0 + c a1]
+template <int J>
+__device__ __forceinline__ void rx_reg(c2d (&A)[16], double c, double s) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        if (r & (1 << J)) continue;
+        const c2d a0 = A[r], a1 = A[r | (1 << J)];
+        A[r] = {c * a0.x + s * a1.y, c * a0.y - s * a1.x};
+        A[r | (1 << J)] = {c * a1.x + s * a0.y, c * a1.y - s * a0.x};
+    }
+}
+// Im <lambda| X_J |psi> over the register pairs of bit J
+template <int J>
+__device__ __forceinline__ double xov_reg(const c2d (&P)[16], const c2d (&L)[16]) {
+    double w = 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        if (r & (1 << J)) continue;
+        const c2d p0 = P[r], p1 = P[r | (1 << J)], l0 = L[r], l1 = L[r | (1 << J)];
+        w += l0.x * p1.y - l0.y * p1.x + l1.x * p0.y - l1.y * p0.x;
+    }
+    return w;
+}
+
+// moves a register tile from layout FROM to layout TO through LDS (real parts, then imaginary)
+template <int FROM, int TO>
+__device__ __forceinline__ void relayout(c2d (&A)[16], double *buf, uint32_t lane, uint32_t wave) {
+    const uint32_t wt = qswz(qthread<FROM>(lane, wave)), rt = qswz(qthread<TO>(lane, wave));
+#pragma unroll
+    for (int part = 0; part < 2; part++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) buf[wt ^ qswz(qreg<FROM>(r))] = part ? A[r].y : A[r].x;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const double v = buf[rt ^ qswz(qreg<TO>(r))];
+            if (part)
+                A[r].y = v;
+            else
+                A[r].x = v;
+        }
+        __syncthreads();
+    }
+}
+
+// the mixer on the register bits of layout LAY that carry local bits >= b0 (forward), or the
+// overlaps and the uncompute of both states (reverse)
+template <bool REV, int LAY>
+__device__ __forceinline__ void mix_group(c2d (&P)[16], c2d (&L)[16], int b0, double cb, double sb, double &wy) {
+    constexpr int base = LAY == 0 ? 8 : LAY == 1 ? 0 : 4;
+    auto bit = [&](auto jc) __attribute__((always_inline)) {
+        constexpr int J = decltype(jc)::value;
+        if (base + J < b0) return;
+        if constexpr (REV) {
+            wy += xov_reg<J>(P, L);
+            rx_reg<J>(P, cb, -sb);
+            rx_reg<J>(L, cb, -sb);
+        } else {
+            rx_reg<J>(P, cb, sb);
+        }
+    };
+    bit(std::integral_constant<int, 0>{});
+    bit(std::integral_constant<int, 1>{});
+    bit(std::integral_constant<int, 2>{});
+    bit(std::integral_constant<int, 3>{});
+}
+
+template <bool REV, int FROM, int TO>
+__device__ __forceinline__ void relayout_states(c2d (&P)[16], c2d (&L)[16], double *buf, uint32_t lane,
+                                                uint32_t wave) {
+    relayout<FROM, TO>(P, buf, lane, wave);
+    if constexpr (REV) relayout<FROM, TO>(L, buf, lane, wave);
+}
+
+// REV: the launch holds only reverse phases (psi and lambda), otherwise only forward phases (psi):
+// the forward kernel keeps half the registers and runs more waves.
+template <bool REV>
 __global__ void __launch_bounds__(kQThreads) k_qaoa_phase(const QInst *__restrict__ insts,
                                                           const QWork *__restrict__ work, int nwork,
-                                                          c2d *__restrict__ pool, double *__restrict__ parts) {
-    extern __shared__ unsigned char smem[];
-    c2d *P = reinterpret_cast<c2d *>(smem);
-    c2d *Lm = P + (1u << kQTileBits);
-    c2d *tab = Lm + (1u << kQTileBits);
+                                                          c2d *__restrict__ pool, uint8_t *__restrict__ cuts,
+                                                          double *__restrict__ parts) {
+    __shared__ double buf[1 << kQTileBits];
+    __shared__ c2d tab[kQMaxEdges + 1];
     __shared__ double sh[16];
+    __shared__ uint32_t sup[kQMaxQubits];
     // which work item: last with first_block <= blockIdx.x
     int lo = 0, hi = nwork;
     while (hi - lo > 1) {
@@ -218,85 +322,106 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_phase(const QInst *__restric
     const QInst &I = insts[W.inst];
     const uint64_t tile = blockIdx.x - W.first_block;
     const int n = I.n, ne = I.nedges, k = W.layer, ph = W.phase;
-    const uint32_t TN = 1u << kQTileBits;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     c2d *psi = pool + I.psi, *lam = pool + I.lam;
+    uint8_t *cut = cuts + I.cut;
     double *part = parts + I.part + (int64_t)tile * I.slot_stride;
-    __shared__ uint32_t sup[kQMaxQubits];
     stage_up(sup, I);
-    const bool rev = ph == PH_REV_A || ph == PH_REV_B;
+    constexpr bool rev = REV;
     const bool isA = ph == PH_FWD_A || ph == PH_REV_A;
-    // load
+    const int Lb = 2 * kQTileBits - n;
+    const int b0 = isA ? 0 : Lb;  // local bits b0..11 carry this phase's mixer qubits
+    c2d P[16], L[16];
+    // global index of register r in layout 0: the tile mapping deposits bits, so it splits into
+    // the thread's part and the register index's part
+    const uint64_t g0 = tile_global(I, ph, tile, qthread<0>(lane, wave));
+    const uint64_t d0 = tile_global(I, ph, 0, 1u << 8), d1 = tile_global(I, ph, 0, 2u << 8);
+    const uint64_t d2 = tile_global(I, ph, 0, 4u << 8), d3 = tile_global(I, ph, 0, 8u << 8);
+    auto gidx = [&](int r) __attribute__((always_inline)) {
+        return g0 | ((r & 1) ? d0 : 0) | ((r & 2) ? d1 : 0) | ((r & 4) ? d2 : 0) | ((r & 8) ? d3 : 0);
+    };
+    // load (layer 0 starts from H^n |0>: every amplitude (1/sqrt2)^n)
     if (ph == PH_FWD_A && k == 0) {
         double a = 1.0;
         for (int q = 0; q < n; q++) a *= 0.70710678118654752440;
-        for (uint32_t l = threadIdx.x; l < TN; l += blockDim.x) P[l] = {a, 0.0};
+#pragma unroll
+        for (int r = 0; r < 16; r++) P[r] = {a, 0.0};
     } else {
-        for (uint32_t l = threadIdx.x; l < TN; l += blockDim.x) {
-            const uint64_t g = tile_global(I, ph, tile, l);
-            P[l] = psi[g];
-            if (rev) Lm[l] = lam[g];
-        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) P[r] = psi[gidx(r)];
     }
-    __syncthreads();
-    double sb, cb;
-    sincos(I.beta[k], &sb, &cb);
-    const int Lb = 2 * kQTileBits - n;
-    // local bits that carry mixer qubits in this phase
-    const int b0 = isA ? 0 : Lb, b1 = kQTileBits;
-    if (!rev) {
-        if (isA) {
-            build_tab(tab, ne, I.gamma[k]);
-            for (uint32_t l = threadIdx.x; l < TN; l += blockDim.x)
-                P[l] = cmul(P[l], tab[cut_of(sup, n, (uint32_t)tile_global(I, ph, tile, l))]);
-            __syncthreads();
-        }
-        for (int b = b0; b < b1; b++) rx_pass(P, kQTileBits, b, cb, sb);
-        const bool last = !isA && k == I.p - 1;
-        if (last) {  // energy partial and lambda = H psi
+    if constexpr (REV) {
+        if (ph == PH_REV_B && k == I.p - 1) {  // lambda = H psi (H = -cut) and the energy
             double e = 0;
-            for (uint32_t l = threadIdx.x; l < TN; l += blockDim.x) {
-                const uint64_t g = tile_global(I, ph, tile, l);
-                const double h = -(double)cut_of(sup, n, (uint32_t)g);
-                const c2d v = P[l];
-                e += h * (v.x * v.x + v.y * v.y);
-                lam[g] = {h * v.x, h * v.y};
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const double h = -(double)cut[gidx(r)];
+                e += h * (P[r].x * P[r].x + P[r].y * P[r].y);
+                L[r] = {h * P[r].x, h * P[r].y};
             }
             e = block_sum1(e, sh);
             if (threadIdx.x == 0) part[0] = e;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; r++) L[r] = lam[gidx(r)];
         }
-        for (uint32_t l = threadIdx.x; l < TN; l += blockDim.x) psi[tile_global(I, ph, tile, l)] = P[l];
-        return;
     }
-    // reverse: mixer derivative on this phase's qubits, then uncompute them
-    double wy = 0;
-    for (int b = b0; b < b1; b++) wy += x_overlap(P, Lm, kQTileBits, b).y;
-    wy = block_sum1(wy, sh);
-    if (threadIdx.x == 0) part[1 + 3 * k + (isA ? 1 : 2)] = wy;
-    for (int b = b0; b < b1; b++) {
-        rx_pass(P, kQTileBits, b, cb, -sb);
-        rx_pass(Lm, kQTileBits, b, cb, -sb);
-    }
-    if (isA) {  // cost derivative and uncompute (all mixer qubits of the layer are undone now)
+    double sb, cb;
+    sincos(I.beta[k], &sb, &cb);
+    if (!rev && isA) {  // cost layer: exp(-i gamma/2 S(z)); layer 0 computes and keeps cut(z)
         build_tab(tab, ne, I.gamma[k]);
-        double gy = 0;
-        for (uint32_t l = threadIdx.x; l < TN; l += blockDim.x) {
-            const int c = cut_of(sup, n, (uint32_t)tile_global(I, ph, tile, l));
-            const double S = (double)(ne - 2 * c);
-            const c2d pz = P[l], lz = Lm[l];
-            gy += S * (lz.x * pz.y - lz.y * pz.x);
-            const c2d t = {tab[c].x, -tab[c].y};
-            P[l] = cmul(pz, t);
-            Lm[l] = cmul(lz, t);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int c;
+            if (k == 0) {
+                c = cut_of(sup, n, (uint32_t)gidx(r));
+                cut[gidx(r)] = (uint8_t)c;
+            } else {
+                c = cut[gidx(r)];
+            }
+            P[r] = cmul(P[r], tab[c]);
         }
-        __syncthreads();
-        gy = block_sum1(gy, sh);
-        if (threadIdx.x == 0) part[1 + 3 * k] = gy;
     }
-    for (uint32_t l = threadIdx.x; l < TN; l += blockDim.x) {
-        const uint64_t g = tile_global(I, ph, tile, l);
-        psi[g] = P[l];
-        lam[g] = Lm[l];
+    // mixer bits, register group by register group: layout 0 holds local bits 8-11, layout 1
+    // bits 0-3, layout 2 bits 4-7.  b0 <= 3: all three groups; b0 in 4..7: groups of layouts 0
+    // and 2; otherwise layout 0 alone.  Each path ends back in layout 0 (coalesced stores).
+    double wy = 0;
+    mix_group<REV, 0>(P, L, b0, cb, sb, wy);
+    if (b0 <= 3) {
+        relayout_states<REV, 0, 1>(P, L, buf, lane, wave);
+        mix_group<REV, 1>(P, L, b0, cb, sb, wy);
+        relayout_states<REV, 1, 2>(P, L, buf, lane, wave);
+        mix_group<REV, 2>(P, L, b0, cb, sb, wy);
+        relayout_states<REV, 2, 0>(P, L, buf, lane, wave);
+    } else if (b0 <= 7) {
+        relayout_states<REV, 0, 2>(P, L, buf, lane, wave);
+        mix_group<REV, 2>(P, L, b0, cb, sb, wy);
+        relayout_states<REV, 2, 0>(P, L, buf, lane, wave);
     }
+    if constexpr (REV) {
+        wy = block_sum1(wy, sh);
+        if (threadIdx.x == 0) part[1 + 3 * k + (isA ? 1 : 2)] = wy;
+        if (isA) {  // cost derivative and uncompute (all mixer qubits of the layer are undone now)
+            build_tab(tab, ne, I.gamma[k]);
+            double gy = 0;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int c = cut[gidx(r)];
+                const double S = (double)(ne - 2 * c);
+                const c2d pz = P[r], lz = L[r];
+                gy += S * (lz.x * pz.y - lz.y * pz.x);  // Im(conj(l) S p)
+                const c2d t = {tab[c].x, -tab[c].y};
+                P[r] = cmul(pz, t);
+                L[r] = cmul(lz, t);
+            }
+            gy = block_sum1(gy, sh);
+            if (threadIdx.x == 0) part[1 + 3 * k] = gy;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) lam[gidx(r)] = L[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) psi[gidx(r)] = P[r];
 }
 
 // energies / gradients of the large instances from their per-tile partials (fixed order)
@@ -348,7 +473,7 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     // host-side instance table
     std::vector<QInst> insts(num_instances);
     std::vector<int32_t> small_ids, large_ids;
-    uint64_t pool_elems = 0;
+    uint64_t pool_elems = 0, cut_bytes = 0;
     int64_t parts = 0;
     for (int i = 0; i < num_instances; i++) {
         QInst &I = insts[i];
@@ -386,6 +511,8 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
             pool_elems += 1ull << I.n;
             I.lam = pool_elems;
             pool_elems += 1ull << I.n;
+            I.cut = cut_bytes;
+            cut_bytes += 1ull << I.n;
             I.part = parts;
             parts += (int64_t)I.ntiles * I.slot_stride;
             large_ids.push_back(i);
@@ -396,14 +523,16 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     // phase schedule of the large instances: forward A,B per layer, then reverse B,A per layer
     int max_phases = 0;
     for (int id : large_ids) max_phases = std::max(max_phases, 4 * insts[id].p);
-    std::vector<std::vector<QWork>> launches(max_phases);
+    // launches[2 s]: the forward work of step s, launches[2 s + 1]: its reverse work
+    std::vector<std::vector<QWork>> launches(2 * max_phases);
     for (int s = 0; s < max_phases; s++) {
-        int blocks = 0;
+        int blocks[2] = {0, 0};
         for (int id : large_ids) {
             const int p = insts[id].p;
             if (s >= 4 * p) continue;
             QWork w;
             w.inst = id;
+            int rv = 0;
             if (s < 2 * p) {
                 w.layer = s / 2;
                 w.phase = (s % 2) ? PH_FWD_B : PH_FWD_A;
@@ -411,18 +540,19 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
                 const int r = s - 2 * p;
                 w.layer = p - 1 - r / 2;
                 w.phase = (r % 2) ? PH_REV_A : PH_REV_B;
+                rv = 1;
             }
-            w.first_block = blocks;
-            blocks += insts[id].ntiles;
-            launches[s].push_back(w);
+            w.first_block = blocks[rv];
+            blocks[rv] += insts[id].ntiles;
+            launches[2 * s + rv].push_back(w);
         }
     }
     // device buffers
-    DevBuf d_inst, d_small, d_large, d_goff, d_e, d_g, d_pool, d_parts, d_work;
+    DevBuf d_inst, d_small, d_large, d_goff, d_e, d_g, d_pool, d_parts, d_work, d_cut;
     const size_t ng = (size_t)(goff[num_instances]);
     size_t wtotal = 0;
-    std::vector<size_t> woff(max_phases);
-    for (int s = 0; s < max_phases; s++) {
+    std::vector<size_t> woff(launches.size());
+    for (size_t s = 0; s < launches.size(); s++) {
         woff[s] = wtotal;
         wtotal += launches[s].size();
     }
@@ -441,6 +571,7 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
                                                     " bytes does not fit; split the batch");
         }
         GOQU_HIP(dmalloc((void **)&d_parts.p, std::max<int64_t>(1, parts) * 8));
+        GOQU_HIP(dmalloc((void **)&d_cut.p, cut_bytes));
     }
     hipStream_t s;
     GOQU_HIP(hipStreamCreate(&s));
@@ -460,7 +591,6 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
         GOQU_HIP(hipMemcpyAsync(d_work.p, wflat.data(), wflat.size() * sizeof(QWork), hipMemcpyHostToDevice, s));
     GOQU_HIP(hipStreamSynchronize(s));
 
-    const size_t lds_large = (2u << kQTileBits) * sizeof(c2d) + (kQMaxEdges + 1) * sizeof(c2d);
     size_t lds_small = 0;
     for (int id : small_ids)
         lds_small = std::max<size_t>(lds_small, (size_t(2) << insts[id].n) * sizeof(c2d) + (kQMaxEdges + 1) * sizeof(c2d));
@@ -468,8 +598,6 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     if (lds_small)
         GOQU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_qaoa_small),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_small));
-    GOQU_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_qaoa_phase),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_large));
     // capture every launch in one graph
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -479,13 +607,20 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
                            static_cast<const QInst *>(d_inst.p), static_cast<const int32_t *>(d_small.p),
                            static_cast<double *>(d_e.p), static_cast<const int64_t *>(d_goff.p),
                            static_cast<double *>(d_g.p));
-    for (int ph = 0; ph < max_phases; ph++) {
+    for (size_t ph = 0; ph < launches.size(); ph++) {
         const auto &v = launches[ph];
         if (v.empty()) continue;
         const unsigned blocks = (unsigned)(v.back().first_block + insts[v.back().inst].ntiles);
-        hipLaunchKernelGGL(k_qaoa_phase, dim3(blocks), dim3(kQThreads), lds_large, s,
-                           static_cast<const QInst *>(d_inst.p), static_cast<const QWork *>(d_work.p) + woff[ph],
-                           (int)v.size(), static_cast<c2d *>(d_pool.p), static_cast<double *>(d_parts.p));
+        if (ph % 2)
+            hipLaunchKernelGGL(k_qaoa_phase<true>, dim3(blocks), dim3(kQThreads), 0, s,
+                               static_cast<const QInst *>(d_inst.p), static_cast<const QWork *>(d_work.p) + woff[ph],
+                               (int)v.size(), static_cast<c2d *>(d_pool.p), static_cast<uint8_t *>(d_cut.p),
+                               static_cast<double *>(d_parts.p));
+        else
+            hipLaunchKernelGGL(k_qaoa_phase<false>, dim3(blocks), dim3(kQThreads), 0, s,
+                               static_cast<const QInst *>(d_inst.p), static_cast<const QWork *>(d_work.p) + woff[ph],
+                               (int)v.size(), static_cast<c2d *>(d_pool.p), static_cast<uint8_t *>(d_cut.p),
+                               static_cast<double *>(d_parts.p));
     }
     if (!large_ids.empty())
         hipLaunchKernelGGL(k_qaoa_finish, dim3((unsigned)((large_ids.size() + 127) / 128)), dim3(128), 0, s,
