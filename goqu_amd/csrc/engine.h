@@ -90,6 +90,8 @@ enum InstrOp : int32_t {
     I_XSWAP = 9,  // register slot s[0] trades places with lane bit k (0..5) by cross-lane moves
                   // (permlane swaps for lane bits 4, 5; row-shift DPP for 0..3); lay / kr give
                   // the resulting layout and its LDS offsets, as for I_REMAP
+    I_END = 10,   // closes every stage's instruction list, right after its ninstr instructions (not
+                  // counted in ninstr): the kernel runs a stage's ops up to it without a count test
 };
 
 // Instr::flags, bits 0..7

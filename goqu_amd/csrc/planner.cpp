@@ -1392,6 +1392,13 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             for (int i = H.instr_off; i < (int)P.instrs.size(); i++) remap = remap || P.instrs[i].op == I_REMAP;
             if (!remap && std::memcmp(H.gq0, H.gqN, (size_t)m) != 0) H.flags |= SF_SYNC_STORE;
         }
+        {
+            Instr E;
+            std::memset(&E, 0, sizeof(E));
+            E.op = I_END;
+            E.flags = F_OUTER | (K_OUTER << kCodeShift);
+            P.instrs.push_back(E);
+        }
         P.stages.push_back(H);
     }
     assign_codes(P);

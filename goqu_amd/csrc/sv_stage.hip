@@ -728,44 +728,59 @@ __device__ __forceinline__ double op_grad(const cx<R> (&a)[16], const cx<R> (&b)
 // wave.  Unlike __syncthreads it leaves outstanding global memory operations in flight.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// LDS byte address lb ^ k of a remap element.  asm volatile keeps each pass's addresses next to
+// the pass (recomputed, not held in 32 VGPRs across both passes of a complex128 remap); the LDS
+// pointer is formed from the integer directly (the stage kernels' dynamic LDS starts at 0).
+__device__ __forceinline__ uint32_t lds_xaddr(uint32_t lb, uint32_t k) {
+    uint32_t a;
+    asm volatile("v_xor_b32 %0, %1, %2" : "=v"(a) : "s"(k), "v"(lb));
+    return a;
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(3))) T *lds_ptr(uint32_t a) {
+    return reinterpret_cast<__attribute__((address_space(3))) T *>(static_cast<size_t>(a));
+}
+
 // One remap through LDS: the 16 amplitudes leave by the old offsets (lb ^ K[r]) and arrive by
 // the new ones.  complex64: one pass of 8-byte amplitudes.  complex128: the real parts, then the
 // imaginary parts (each pass writes only registers the other pass does not read).
 template <typename R>
 __device__ __forceinline__ void lds_remap(unsigned char *smem, cx<R> (&v)[16], uint32_t lb, const KOff &K,
                                           uint32_t lbn, const KOff &Kn) {
+    (void)smem;
     if constexpr (sizeof(R) == 4) {
         lds_barrier();
+        typedef R v2 __attribute__((ext_vector_type(2)));
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
-            *reinterpret_cast<cx<R> *>(smem + (lb ^ koff<r>(K))) = v[r];
+            v2 x;
+            x.x = v[r].x;
+            x.y = v[r].y;
+            *lds_ptr<v2>(lds_xaddr(lb, koff<r>(K))) = x;
         });
         lds_barrier();
         static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
             constexpr int r = decltype(rc)::value;
-            v[r] = *reinterpret_cast<const cx<R> *>(smem + (lbn ^ koff<r>(Kn)));
+            const v2 x = *lds_ptr<v2>(lds_xaddr(lbn, koff<r>(Kn)));
+            v[r].x = x.x;
+            v[r].y = x.y;
         });
     } else {
-        lds_barrier();
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            *reinterpret_cast<R *>(smem + (lb ^ koff<r>(K))) = v[r].x;
-        });
-        lds_barrier();
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            v[r].x = *reinterpret_cast<const R *>(smem + (lbn ^ koff<r>(Kn)));
-        });
-        lds_barrier();
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            *reinterpret_cast<R *>(smem + (lb ^ koff<r>(K))) = v[r].y;
-        });
-        lds_barrier();
-        static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
-            constexpr int r = decltype(rc)::value;
-            v[r].y = *reinterpret_cast<const R *>(smem + (lbn ^ koff<r>(Kn)));
-        });
+        auto pass = [&](auto comp) __attribute__((always_inline)) {
+            constexpr int c = decltype(comp)::value;
+            lds_barrier();
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value;
+                *lds_ptr<R>(lds_xaddr(lb, koff<r>(K))) = c ? v[r].y : v[r].x;
+            });
+            lds_barrier();
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value;
+                (c ? v[r].y : v[r].x) = *lds_ptr<R>(lds_xaddr(lbn, koff<r>(Kn)));
+            });
+        };
+        pass(std::integral_constant<int, 0>{});
+        pass(std::integral_constant<int, 1>{});
     }
 }
 
@@ -778,7 +793,6 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                                            double *gacc) {
     extern __shared__ unsigned char smem_raw[];
     const int m = hdr->m;
-    const int ninstr = hdr->ninstr;
     const uint32_t tid = threadIdx.x;
     const Instr *__restrict__ P = prog + hdr->instr_off;
     const uint32_t *Tw = reinterpret_cast<const uint32_t *>(hdr->T);
@@ -829,18 +843,28 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
         KOff K;
         load_koff(K, hdr->k0);
 
-        int pc = 0;
+        // The head of the current instruction and its code path.  Every stage's list ends with
+        // an I_END (code K_OUTER), so the inner loop below has its only exit at its head and no
+        // count test.  The instruction's byte offset goes through readfirstlane so that it stays
+        // provably uniform (instruction fields then come through scalar loads).
+        uint32_t ioff = 0;
+        auto at = [&]() __attribute__((always_inline)) {
+            return reinterpret_cast<const Instr *>(reinterpret_cast<const unsigned char *>(P) + ioff);
+        };
+        IHead cur = load_head(at(), 0);
+        int code = (cur.flags >> kCodeShift) & 255;
+        auto next = [&]() __attribute__((always_inline)) {
+            ioff = __builtin_amdgcn_readfirstlane(ioff + (uint32_t)sizeof(Instr));
+            cur = load_head(at(), 0);
+            code = (cur.flags >> kCodeShift) & 255;
+        };
         while (true) {
             // The ops up to the next remap.  The remap lives outside this loop: with it inside,
             // the compiler splits the amplitudes into two register sets and copies all of them
-            // on every op.  pc goes through readfirstlane so that it stays provably uniform
-            // (instruction fields then come through scalar loads).
-            for (; pc < ninstr; pc++) {
-                pc = __builtin_amdgcn_readfirstlane(pc);
-                const IHead cur = load_head(P, pc);  // one batch of scalar loads
-                const int code = (cur.flags >> kCodeShift) & 255;
-                if (code == K_OUTER) break;
-                if ((cur.flags & F_TILE) && (base & cur.omask) != cur.omask) continue;  // uniform per workgroup
+            // on every op.
+            for (; code != K_OUTER; next()) {
+                // controls on qubits off the tile (omask = 0 unless F_TILE): uniform per workgroup
+                if ((base & cur.omask) != cur.omask) continue;
                 const bool lok = (tid & cur.tmask) == cur.tmask;
                 const uint32_t pat = cur.pat;
                 const R *__restrict__ M = pool + 2 * cur.mat;
@@ -954,11 +978,14 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     }
                 }
             }
-            if (pc >= ninstr) break;
-            pc = __builtin_amdgcn_readfirstlane(pc);
-            const Instr &I = P[pc++];
-            if (I.op == I_RESET) {
-                switch (I.s[0]) {
+            if (cur.op == I_END) break;
+            // an outer instruction: its head is in `h`, the rest (layout, LDS offsets) at *I
+            const Instr *__restrict__ I = at();
+            const IHead h = cur;
+            next();
+            const int s0 = (int)(h.sw & 255u);
+            if (h.op == I_RESET) {
+                switch (s0) {
                 case 0: op_reset<0>(a); break;
                 case 1: op_reset<1>(a); break;
                 case 2: op_reset<2>(a); break;
@@ -966,32 +993,32 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                 }
                 continue;
             }
-            if (I.op == I_DENSE) {  // three or four slots (stages flagged SF_BIG only)
+            if (h.op == I_DENSE) {  // three or four slots (stages flagged SF_BIG only)
                 if constexpr (!BIG) continue;
-                if ((I.flags & F_TILE) && (base & I.omask) != I.omask) continue;
-                const bool lok = (tid & I.tmask) == I.tmask;
-                const R *__restrict__ M = pool + 2 * I.mat;
-                switch (I.smask) {
-                case 7: op_dense_big<7, PAIR>(a, b, M, I.pat, lok); break;
-                case 11: op_dense_big<11, PAIR>(a, b, M, I.pat, lok); break;
-                case 13: op_dense_big<13, PAIR>(a, b, M, I.pat, lok); break;
-                case 14: op_dense_big<14, PAIR>(a, b, M, I.pat, lok); break;
-                case 15: op_dense_big<15, PAIR>(a, b, M, I.pat, lok); break;
+                if ((base & h.omask) != h.omask) continue;
+                const bool lok = (tid & h.tmask) == h.tmask;
+                const R *__restrict__ M = pool + 2 * h.mat;
+                switch (h.smask) {
+                case 7: op_dense_big<7, PAIR>(a, b, M, h.pat, lok); break;
+                case 11: op_dense_big<11, PAIR>(a, b, M, h.pat, lok); break;
+                case 13: op_dense_big<13, PAIR>(a, b, M, h.pat, lok); break;
+                case 14: op_dense_big<14, PAIR>(a, b, M, h.pat, lok); break;
+                case 15: op_dense_big<15, PAIR>(a, b, M, h.pat, lok); break;
                 default: break;
                 }
                 continue;
             }
-            if (I.op == I_XSWAP) {  // cross-lane exchange; the LDS offsets follow the new layout
-                xswap_dispatch(a, I.s[0], I.k, (uint32_t)lane);
-                if constexpr (PAIR) xswap_dispatch(b, I.s[0], I.k, (uint32_t)lane);
-                lb = lds_lane<R>(I.lay, m, tid);
-                load_koff(K, I.kr);
+            if (h.op == I_XSWAP) {  // cross-lane exchange; the LDS offsets follow the new layout
+                xswap_dispatch(a, s0, h.k, (uint32_t)lane);
+                if constexpr (PAIR) xswap_dispatch(b, s0, h.k, (uint32_t)lane);
+                lb = lds_lane<R>(I->lay, m, tid);
+                load_koff(K, I->kr);
                 continue;
             }
             // remap: the tile goes through LDS into the next layout
-            const uint32_t lbn = lds_lane<R>(I.lay, m, tid);
+            const uint32_t lbn = lds_lane<R>(I->lay, m, tid);
             KOff Kn;
-            load_koff(Kn, I.kr);
+            load_koff(Kn, I->kr);
             lds_remap<R>(smem_raw, a, lb, K, lbn, Kn);
             if constexpr (PAIR) lds_remap<R>(smem_raw, b, lb, K, lbn, Kn);
             lb = lbn;
@@ -1028,11 +1055,12 @@ __device__ __forceinline__ const T *blob_at(const T *p, uint64_t byte_off) {
 }
 
 // Three single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops
-// (few enough VGPRs for four waves per SIMD, and code small enough for short branches),
+// (at most 96 VGPRs: five waves per SIMD, which five 2^12-amplitude tiles of LDS also allow;
+// the code is small enough for short branches),
 // k_stage_big the stages that hold them, k_stage_dm density-matrix stages (channel
 // superoperators as well).
 template <typename R>
-__global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride,
                                                uint64_t blob_stride) {
@@ -1067,7 +1095,10 @@ __global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<
                                                     const StageHeader *__restrict__ hdr,
                                                     const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                     uint64_t ntiles, double *__restrict__ gpart) {
-    __shared__ double gacc[kMaxGradPerStage * 8];
+    // the gradient partials follow the tile's remap area in dynamic LDS (the remaps address LDS
+    // from 0, so the kernel holds no static LDS)
+    extern __shared__ unsigned char smem_raw[];
+    double *gacc = reinterpret_cast<double *>(smem_raw + (size_t(8) << hdr->m));
     stage_body<R, true, true, false>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
 }
 
@@ -1146,9 +1177,14 @@ ProgramLayout program_layout(size_t nh, size_t ni, size_t np, int prec) {
 
 int serialize_program(const Program &P0, int prec, const ProgramLayout &L, unsigned char *out,
                       std::vector<StageHeader> *stages) {
-    for (const StageHeader &H : P0.stages)
+    for (const StageHeader &H : P0.stages) {
         if ((size_t(1) << H.m) * 8 > 65536)
             return fail(GOQU_ERR_UNSUPPORTED, "stage tile larger than the LDS offset range");
+        // the stage kernels run a stage's ops up to its closing I_END
+        const size_t e = (size_t)H.instr_off + (size_t)H.ninstr;
+        if (H.instr_off < 0 || H.ninstr < 0 || e >= P0.instrs.size() || P0.instrs[e].op != I_END)
+            return fail(GOQU_ERR_INVALID_ARG, "stage instruction list not closed by I_END");
+    }
     const size_t rsz = prec == GOQU_C128 ? sizeof(double) : sizeof(float);
     const size_t nh = P0.stages.size(), ni = P0.instrs.size(), np = P0.pool.size();
     if (L.instr_off < nh * sizeof(StageHeader) || L.pool_off < L.instr_off + ni * sizeof(Instr) ||
@@ -1288,7 +1324,8 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
         const StageHeader &H = P.stages[i];
         if (stage_is_identity(H)) continue;
         const unsigned threads = 1u << (m - kRegBits);
-        const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
+        // 8-byte LDS elements, then the per-wave gradient partials
+        const size_t lds = (size_t(1) << m) * 8 + kMaxGradPerStage * 8 * sizeof(double);
         hipEvent_t ev0 = prof_begin(s);
         hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),
                            static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
