@@ -1405,6 +1405,25 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
     return P;
 }
 
+// The register slot s when a one-table phase op (16 entries) is a single uniform phase on the
+// registers with bit s set (every other entry exactly 1), else -1.
+int diag_single_slot(const Instr &I, const std::vector<double> &pool) {
+    for (int s = 0; s < kRegBits; s++) {
+        uint32_t want = 0;
+        for (int r = 0; r < kRegAmps; r++)
+            if (r >> s & 1) want |= 1u << r;
+        if ((I.pat & 0xFFFFu) != want) continue;
+        const double *t = pool.data() + 2 * (size_t)I.mat;
+        const double pr = t[2 * (1 << s)], pi = t[2 * (1 << s) + 1];
+        for (int r = 0; r < kRegAmps; r++) {
+            const bool on = r >> s & 1;
+            if (t[2 * r] != (on ? pr : 1.0) || t[2 * r + 1] != (on ? pi : 0.0)) return -1;
+        }
+        return s;
+    }
+    return -1;
+}
+
 void assign_codes(Program &P) {
     auto pair_index = [](int a, int b) {  // slot pairs 01 02 03 12 13 23
         static const int tab[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
@@ -1422,8 +1441,15 @@ void assign_codes(Program &P) {
                 case I_DIAG: {
                     bool tile_bits = !tc;
                     for (int j = 0; j < I.k; j++) tile_bits = tile_bits && I.ls[j] >= 16;
-                    if (I.k == 0 && !tc) code = (I.pat | 1u) == 0xFFFFu ? K_DIAG_FULL : K_DIAG_MASK;
-                    else code = tile_bits ? K_DIAG_TILE : K_DIAG_LANE;
+                    const int slot = I.k == 0 ? diag_single_slot(I, P.pool) : -1;
+                    if (I.k == 0 && !tc)
+                        code = slot >= 0                                  ? K_DIAG_SLOT + slot
+                               : __builtin_popcount(I.pat & 0xFFFEu) >= 11 ? K_DIAG_FULL
+                                                                          : K_DIAG_MASK;
+                    else if (I.k == 0 && slot >= 0)
+                        code = K_DIAG_TSLOT + slot;
+                    else
+                        code = tile_bits ? K_DIAG_TILE : K_DIAG_LANE;
                     break;
                 }
                 case I_ANTI1: code = K_ANTI1 + 4 * I.s[0] + 2 * pure + tc; break;

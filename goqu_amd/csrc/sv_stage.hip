@@ -585,16 +585,58 @@ __device__ __forceinline__ void op_diag(cx<R> (&a)[16], cx<R> (&b)[16], const IH
         cmul_inplace<U>(a[r].x, a[r].y, ph.x, ph.y);
         if constexpr (PAIR) cmul_inplace<U>(b[r].x, b[r].y, ph.x, ph.y);
     };
-    if (code != K_DIAG_LANE) {  // a uniform table: four entries per scalar load
-        const bool full = code == K_DIAG_FULL;  // registers 1..15 all nontrivial: no tests
+    if (code >= K_DIAG_SLOT) {  // one phase on the eight registers of one slot
+        cx<R> ph[1];
+        load_cx_batch<R, 1>(tab + (1 << (code & 3)), ph);
+        const bool tc = code >= K_DIAG_TSLOT;  // lanes whose thread-bit controls fail keep 1
+        cx<R> pl;
+        pl.x = (!tc || lok) ? ph[0].x : R(1);
+        pl.y = (!tc || lok) ? ph[0].y : R(0);
+        auto slot = [&](auto sc, auto tcc) __attribute__((always_inline)) {
+            constexpr int S = decltype(sc)::value;
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value;
+                if constexpr (r >> S & 1) {
+                    if constexpr (decltype(tcc)::value) mul(rc, std::false_type{}, pl);
+                    else mul(rc, std::true_type{}, ph[0]);
+                }
+            });
+        };
+        using U = std::false_type;  // uniform phase
+        using L = std::true_type;   // per-lane phase (thread-bit controls)
+        switch (code - K_DIAG_SLOT) {
+        case 0: slot(std::integral_constant<int, 0>{}, U{}); break;
+        case 1: slot(std::integral_constant<int, 1>{}, U{}); break;
+        case 2: slot(std::integral_constant<int, 2>{}, U{}); break;
+        case 3: slot(std::integral_constant<int, 3>{}, U{}); break;
+        case 4: slot(std::integral_constant<int, 0>{}, L{}); break;
+        case 5: slot(std::integral_constant<int, 1>{}, L{}); break;
+        case 6: slot(std::integral_constant<int, 2>{}, L{}); break;
+        default: slot(std::integral_constant<int, 3>{}, L{}); break;
+        }
+        return;
+    }
+    if (code == K_DIAG_FULL) {  // registers 1..15 without tests, four entries per scalar load
         static_for<0, 4>([&](auto cc) __attribute__((always_inline)) {
             constexpr int c0 = 4 * decltype(cc)::value;
-            if (full || ((nontriv >> c0) & 15u)) {
+            cx<R> ph[4];
+            load_cx_batch<R, 4>(tab + c0, ph);
+            static_for<0, 4>([&](auto jc) __attribute__((always_inline)) {
+                constexpr int r = c0 + decltype(jc)::value;
+                if (r > 0 || (nontriv & 1u)) mul(std::integral_constant<int, r>{}, std::true_type{}, ph[decltype(jc)::value]);
+            });
+        });
+        return;
+    }
+    if (code != K_DIAG_LANE) {  // a uniform table on the registers of `pat`: four entries per scalar load
+        static_for<0, 4>([&](auto cc) __attribute__((always_inline)) {
+            constexpr int c0 = 4 * decltype(cc)::value;
+            if ((nontriv >> c0) & 15u) {
                 cx<R> ph[4];
                 load_cx_batch<R, 4>(tab + c0, ph);
                 static_for<0, 4>([&](auto jc) __attribute__((always_inline)) {
                     constexpr int r = c0 + decltype(jc)::value;
-                    if ((full && r > 0) || ((nontriv >> r) & 1u))
+                    if ((nontriv >> r) & 1u)
                         mul(std::integral_constant<int, r>{}, std::true_type{}, ph[decltype(jc)::value]);
                 });
             }

@@ -82,7 +82,7 @@ enum Cov {
     C_DENSE, C_DENSE_TCTRL, C_DENSE_PAT, C_DENSE_TILE, C_DENSE_BIG, C_DIAG_UNIFORM, C_DIAG_KL1, C_DIAG_KL2,
     C_DIAG_TCTRL, C_DIAG_TILE, C_DIAG_REGS, C_ANTI_PURE, C_ANTI_PHASED, C_ANTI_TCTRL, C_ANTI_PAT, C_SWAP_PURE,
     C_SWAP_PHASED, C_SWAP_TCTRL, C_SWAP_PAT, C_REMAP, C_GRAD, C_GRAD_CTRL, C_RESET, C_PARITY2, C_PARITY4,
-    C_PARITY_REAL, C_SHEAR2, C_SHEAR3, C_XSWAP_PERM, C_XSWAP_DPP, C_NCOV
+    C_PARITY_REAL, C_SHEAR2, C_SHEAR3, C_XSWAP_PERM, C_XSWAP_DPP, C_DIAG_FULL, C_DIAG_SLOT, C_DIAG_TSLOT, C_NCOV
 };
 int64_t g_cov[C_NCOV];
 void count(const Instr &I) {
@@ -94,12 +94,17 @@ void count(const Instr &I) {
         if (pt) g_cov[C_DENSE_PAT]++;
         if (tl) g_cov[C_DENSE_TILE]++;
         break;
-    case I_DIAG:
+    case I_DIAG: {
+        const int code = instr_code(I.flags);
+        if (code == K_DIAG_FULL) g_cov[C_DIAG_FULL]++;
+        if (code >= K_DIAG_SLOT && code < K_DIAG_TSLOT) g_cov[C_DIAG_SLOT]++;
+        if (code >= K_DIAG_TSLOT && code < K_ANTI1) g_cov[C_DIAG_TSLOT]++;
         g_cov[I.k == 0 ? (tc ? C_DIAG_TCTRL : C_DIAG_UNIFORM) : I.k == 1 ? C_DIAG_KL1 : C_DIAG_KL2]++;
         if (tc) g_cov[C_DIAG_TCTRL]++;
         if (tl) g_cov[C_DIAG_TILE]++;
         if (I.smask) g_cov[C_DIAG_REGS]++;
         break;
+    }
     case I_ANTI1:
         g_cov[(I.flags & F_PURE) ? C_ANTI_PURE : C_ANTI_PHASED]++;
         if (tc) g_cov[C_ANTI_TCTRL]++;
@@ -125,7 +130,7 @@ void count(const Instr &I) {
 
 // The kernel dispatches on the code in flags bits 8..15 alone: decode it the way the kernel's
 // switch does and check that the code path it selects is the one the instruction's fields describe.
-bool check_code(const Instr &I, std::string &err) {
+bool check_code(const Instr &I, const std::vector<double> &pool, std::string &err) {
     const int code = instr_code(I.flags);
     static const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
     static const int grad_masks[10] = {1, 2, 3, 4, 5, 6, 8, 9, 10, 12};
@@ -137,10 +142,16 @@ bool check_code(const Instr &I, std::string &err) {
     else if (code < K_ANTI1) {
         bool tile_bits = I.k > 0 && !tc;
         for (int j = 0; j < I.k; j++) tile_bits = tile_bits && I.ls[j] >= 16;
-        ok = I.op == I_DIAG && (code == K_DIAG_LANE   ? (I.k > 0 || tc) && !tile_bits
-                                : code == K_DIAG_TILE ? tile_bits
-                                                      : (I.k == 0 && !tc)) &&
-             (code != K_DIAG_FULL || (I.pat | 1u) == 0xFFFFu);
+        // single-slot phases: one uniform entry on the registers of the slot, the rest exactly 1;
+        // full tables: every register 1..15 is multiplied (trivial entries are exactly 1)
+        const int slot = I.op == I_DIAG && I.k == 0 ? diag_single_slot(I, pool) : -1;
+        if (code >= K_DIAG_TSLOT) ok = I.op == I_DIAG && I.k == 0 && tc && slot == code - K_DIAG_TSLOT;
+        else if (code >= K_DIAG_SLOT) ok = I.op == I_DIAG && I.k == 0 && !tc && slot == code - K_DIAG_SLOT;
+        else
+            ok = I.op == I_DIAG && (code == K_DIAG_LANE   ? (I.k > 0 || tc) && !tile_bits
+                                    : code == K_DIAG_TILE ? tile_bits
+                                                          : (I.k == 0 && !tc)) &&
+                 (code != K_DIAG_FULL || __builtin_popcount(I.pat & 0xFFFEu) >= 11);
     }
     else if (code < K_SWAP2) {
         const int c = code - K_ANTI1;
@@ -188,7 +199,7 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
         const int nthr = 1 << (m - 4);
         for (int pc = 0; pc < H.ninstr; pc++) {
             count(P.instrs[H.instr_off + pc]);
-            if (!check_code(P.instrs[H.instr_off + pc], err)) return 1;
+            if (!check_code(P.instrs[H.instr_off + pc], P.pool, err)) return 1;
             if (P.instrs[H.instr_off + pc].op == I_PARITY && !(H.flags & SF_DM)) {
                 err = "a stage with channel ops is not flagged for the density-matrix kernel";
                 return 1;
