@@ -92,6 +92,9 @@ enum InstrOp : int32_t {
                   // the resulting layout and its LDS offsets, as for I_REMAP
     I_END = 10,   // closes every stage's instruction list, right after its ninstr instructions (not
                   // counted in ninstr): the kernel runs a stage's ops up to it without a count test
+    I_SHEARS = 11, // I_SHEAR on several register slots at once (consecutive shears on distinct
+                   // slots commute): bits 2s..2s+1 of k = the shear count on slot s (0, 2 or 3),
+                   // the coefficients c0, c1, c2 of slot s at pool[mat] + 3 s (twelve reals)
 };
 
 // Instr::flags, bits 0..7
@@ -106,21 +109,22 @@ constexpr int32_t F_REAL = 32;  // I_PARITY, I_DENSE: every matrix entry is real
 // stage kernel dispatches every op with one switch on a small integer.  Ranges:
 enum KCode : int32_t {
     K_SHEAR = 0,     // 8: 2 * slot + (three shears)
-    K_DIAG_FULL = 8, // uniform phase table applied to registers 1..15 without tests (and to
-                     // register 0 if bit 0 of `pat` is set): at least 11 of them nontrivial
-    K_DIAG_MASK = 9, // uniform phase table, the registers of `pat`
-    K_DIAG_TILE = 10,  // phase table selected by bits of the tile base (uniform per workgroup)
-    K_DIAG_LANE = 11,  // phase table indexed by thread-id bits, or under thread-bit controls
-    K_DIAG_SLOT = 12,  // 4: slot; one uniform phase (table entry 1 << slot) on the 8 registers
-                       // with that slot's bit set, every other entry 1
-    K_DIAG_TSLOT = 16, // 4: slot; as K_DIAG_SLOT, on the lanes whose thread-bit controls hold
-    K_ANTI1 = 20,    // 16: 4 * slot + 2 * pure + thread controls
-    K_SWAP2 = 36,    // 24: 4 * slot pair (01 02 03 12 13 23) + 2 * pure + thread controls
-    K_DENSE1 = 60,   // 12: 3 * slot + (0 controlled, 1 real, 2 complex)
-    K_DENSE2 = 72,   // 12: 2 * slot pair + (0 controlled, 1 complex)
-    K_PARITY2 = 84,  // 12: 2 * slot pair + (0 real, 1 complex)
-    K_PARITY4 = 96,  // 6: 2 * (partner of slot 0 - 1) + (0 real, 1 complex)
-    K_GRAD = 102,    // 10: index of the target slot mask in {1,2,3,4,5,6,8,9,10,12}
+    K_SHEARS = 8,    // I_SHEARS
+    K_DIAG_FULL = 12, // uniform phase table applied to registers 1..15 without tests (and to
+                      // register 0 if bit 0 of `pat` is set): at least 11 of them nontrivial
+    K_DIAG_MASK = 13, // uniform phase table, the registers of `pat`
+    K_DIAG_TILE = 14, // phase table selected by bits of the tile base (uniform per workgroup)
+    K_DIAG_LANE = 15, // phase table indexed by thread-id bits, or under thread-bit controls
+    K_DIAG_SLOT = 16, // 4: slot; one uniform phase (table entry 1 << slot) on the 8 registers
+                      // with that slot's bit set, every other entry 1
+    K_DIAG_TSLOT = 20, // 4: slot; as K_DIAG_SLOT, on the lanes whose thread-bit controls hold
+    K_ANTI1 = 24,    // 16: 4 * slot + 2 * pure + thread controls
+    K_SWAP2 = 40,    // 24: 4 * slot pair (01 02 03 12 13 23) + 2 * pure + thread controls
+    K_DENSE1 = 64,   // 12: 3 * slot + (0 controlled, 1 real, 2 complex)
+    K_DENSE2 = 76,   // 12: 2 * slot pair + (0 controlled, 1 complex)
+    K_PARITY2 = 88,  // 12: 2 * slot pair + (0 real, 1 complex)
+    K_PARITY4 = 100, // 6: 2 * (partner of slot 0 - 1) + (0 real, 1 complex)
+    K_GRAD = 106,    // 10: index of the target slot mask in {1,2,3,4,5,6,8,9,10,12}
     K_OUTER = 255,   // F_OUTER instructions (the kernel's outer loop)
 };
 constexpr int kCodeShift = 8;

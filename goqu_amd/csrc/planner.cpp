@@ -577,6 +577,45 @@ static Layout make_layout(int m, const int reg[4], const int *qmap, const int pr
 
 }  // namespace
 
+// Runs of consecutive I_SHEAR instructions on distinct register slots (from index `from` on)
+// become one I_SHEARS: shears on different slots act on disjoint register pairs, so they
+// commute and the merged op performs the same fused multiply-adds under one dispatch.
+static void merge_shears(Program &P, int from) {
+    std::vector<Instr> out;
+    out.reserve(P.instrs.size() - (size_t)from);
+    size_t i = (size_t)from;
+    while (i < P.instrs.size()) {
+        size_t j = i;
+        uint32_t slots = 0;
+        while (j < P.instrs.size() && P.instrs[j].op == I_SHEAR && !(slots >> P.instrs[j].s[0] & 1)) {
+            slots |= 1u << P.instrs[j].s[0];
+            j++;
+        }
+        if (j - i < 2) {
+            out.push_back(P.instrs[i]);
+            i++;
+            continue;
+        }
+        Instr M;
+        std::memset(&M, 0, sizeof(M));
+        M.op = I_SHEARS;
+        M.pat = 0xFFFFu;
+        M.smask = (int32_t)slots;
+        M.mat = (int32_t)(P.pool.size() / 2);
+        double coef[12] = {0};
+        for (size_t t = i; t < j; t++) {
+            const Instr &S = P.instrs[t];
+            M.k |= S.k << (2 * S.s[0]);
+            for (int c = 0; c < 3; c++) coef[3 * S.s[0] + c] = shear_coef(S, c);
+        }
+        P.pool.insert(P.pool.end(), coef, coef + 12);
+        out.push_back(M);
+        i = j;
+    }
+    P.instrs.resize((size_t)from);
+    P.instrs.insert(P.instrs.end(), out.begin(), out.end());
+}
+
 Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &cfg) {
     Program P;
     P.nq = nq;
@@ -1386,6 +1425,7 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
             H.layN[i] = (uint8_t)cur.lay[i];
             H.gqN[i] = (uint8_t)qmap[cur.lay[i]];
         }
+        merge_shears(P, H.instr_off);
         H.ninstr = (int32_t)P.instrs.size() - H.instr_off;
         {
             bool remap = false;
@@ -1438,6 +1478,7 @@ void assign_codes(Program &P) {
             if (!(I.flags & F_OUTER)) {
                 switch (I.op) {
                 case I_SHEAR: code = K_SHEAR + 2 * I.s[0] + (I.k == 3); break;
+                case I_SHEARS: code = K_SHEARS; break;
                 case I_DIAG: {
                     bool tile_bits = !tc;
                     for (int j = 0; j < I.k; j++) tile_bits = tile_bits && I.ls[j] >= 16;

@@ -302,6 +302,31 @@ __device__ __forceinline__ void op_dense(cx<R> (&a)[16], cx<R> (&b)[16], const R
 // A real rotation on slot S as NSH in-place shears (lifting steps): a0 += c0 a1, a1 += c1 a0,
 // then (NSH = 3) a0 += c2 a1.  One FMA per amplitude component and shear, no temporaries and no
 // write-backs; the planner picks coefficients whose products stay bounded.
+// shears P0 .. P1-1 of a rotation on slot S (even: a0 += c a1, odd: a1 += c a0)
+template <int S, int P0, int P1, bool PAIR, typename R>
+__device__ __forceinline__ void shear_steps(cx<R> (&a)[16], cx<R> (&b)[16], const R *c) {
+    auto apply = [&](cx<R>(&v)[16]) __attribute__((always_inline)) {
+        static_for<P0, P1>([&](auto pc) __attribute__((always_inline)) {
+            constexpr int p = decltype(pc)::value;
+            static_for<0, 16>([&](auto rc) __attribute__((always_inline)) {
+                constexpr int r = decltype(rc)::value;
+                if constexpr (!(r >> S & 1)) {
+                    constexpr int r1 = r | (1 << S);
+                    if constexpr (p & 1) {
+                        fma_inplace(v[r1].x, v[r].x, c[p]);
+                        fma_inplace(v[r1].y, v[r].y, c[p]);
+                    } else {
+                        fma_inplace(v[r].x, v[r1].x, c[p]);
+                        fma_inplace(v[r].y, v[r1].y, c[p]);
+                    }
+                }
+            });
+        });
+    };
+    apply(a);
+    if constexpr (PAIR) apply(b);
+}
+
 template <int S, int NSH, bool PAIR, typename R>
 __device__ __forceinline__ void op_shear(cx<R> (&a)[16], cx<R> (&b)[16], const IHead &h) {
     const R c[3] = {R(shear_coef(h, 0)), R(shear_coef(h, 1)), R(shear_coef(h, 2))};
@@ -325,6 +350,48 @@ __device__ __forceinline__ void op_shear(cx<R> (&a)[16], cx<R> (&b)[16], const I
     };
     apply(a);
     if constexpr (PAIR) apply(b);
+}
+
+// Shears on several register slots (I_SHEARS): slot s takes (cnt >> 2s) & 3 shears (0, 2 or 3)
+// with coefficients C[3s..3s+2], the same fused multiply-adds as one I_SHEAR per slot.
+template <bool PAIR, typename R>
+__device__ __forceinline__ void op_shears(cx<R> (&a)[16], cx<R> (&b)[16], const R *__restrict__ C, uint32_t cnt) {
+    R c[12];
+    if constexpr (sizeof(R) == 8) {  // 96 bytes: two batches of scalar loads
+        typedef uint32_t v16 __attribute__((ext_vector_type(16)));
+        typedef uint32_t v8 __attribute__((ext_vector_type(8)));
+        const v16 w0 = *reinterpret_cast<const v16 *>(C);
+        const v8 w1 = *reinterpret_cast<const v8 *>(C + 8);
+        static_for<0, 8>([&](auto ic) __attribute__((always_inline)) {
+            constexpr int i = decltype(ic)::value;
+            c[i] = __builtin_bit_cast(R, (uint64_t)w0[2 * i] | ((uint64_t)w0[2 * i + 1] << 32));
+        });
+        static_for<0, 4>([&](auto ic) __attribute__((always_inline)) {
+            constexpr int i = decltype(ic)::value;
+            c[8 + i] = __builtin_bit_cast(R, (uint64_t)w1[2 * i] | ((uint64_t)w1[2 * i + 1] << 32));
+        });
+    } else {
+        typedef uint32_t v8 __attribute__((ext_vector_type(8)));
+        typedef uint32_t v4 __attribute__((ext_vector_type(4)));
+        const v8 w0 = *reinterpret_cast<const v8 *>(C);
+        const v4 w1 = *reinterpret_cast<const v4 *>(C + 8);
+        static_for<0, 8>([&](auto ic) __attribute__((always_inline)) {
+            const uint32_t x = w0[decltype(ic)::value];
+            c[decltype(ic)::value] = __builtin_bit_cast(R, x);
+        });
+        static_for<0, 4>([&](auto ic) __attribute__((always_inline)) {
+            const uint32_t x = w1[decltype(ic)::value];
+            c[8 + decltype(ic)::value] = __builtin_bit_cast(R, x);
+        });
+    }
+    static_for<0, 4>([&](auto sc) __attribute__((always_inline)) {
+        constexpr int S = decltype(sc)::value;
+        const uint32_t n = (cnt >> (2 * S)) & 3u;
+        if (n) {
+            shear_steps<S, 0, 2, PAIR>(a, b, c + 3 * S);
+            if (n == 3) shear_steps<S, 2, 3, PAIR>(a, b, c + 3 * S);
+        }
+    });
 }
 
 // Dense op on three or four register slots: computed for every lane, kept where the
@@ -920,7 +987,8 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                     case 4: op_shear<2, 2, PAIR>(a, b, cur); break;
                     case 5: op_shear<2, 3, PAIR>(a, b, cur); break;
                     case 6: op_shear<3, 2, PAIR>(a, b, cur); break;
-                    default: op_shear<3, 3, PAIR>(a, b, cur); break;
+                    case 7: op_shear<3, 3, PAIR>(a, b, cur); break;
+                    default: op_shears<PAIR>(a, b, M, (uint32_t)cur.k); break;  // K_SHEARS
                     }
                 } else if (code < K_ANTI1) {
                     op_diag<PAIR>(a, b, cur, pool, tid, base, lok, code);
@@ -1097,12 +1165,12 @@ __device__ __forceinline__ const T *blob_at(const T *p, uint64_t byte_off) {
 }
 
 // Three single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops
-// (at most 96 VGPRs: five waves per SIMD, which five 2^12-amplitude tiles of LDS also allow;
-// the code is small enough for short branches),
+// (few enough VGPRs for four waves per SIMD, and code small enough for short branches; a fifth
+// wave, forced by spilling, measured slower),
 // k_stage_big the stages that hold them, k_stage_dm density-matrix stages (channel
 // superoperators as well).
 template <typename R>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5, 8))) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+__global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride,
                                                uint64_t blob_stride) {

@@ -82,7 +82,8 @@ enum Cov {
     C_DENSE, C_DENSE_TCTRL, C_DENSE_PAT, C_DENSE_TILE, C_DENSE_BIG, C_DIAG_UNIFORM, C_DIAG_KL1, C_DIAG_KL2,
     C_DIAG_TCTRL, C_DIAG_TILE, C_DIAG_REGS, C_ANTI_PURE, C_ANTI_PHASED, C_ANTI_TCTRL, C_ANTI_PAT, C_SWAP_PURE,
     C_SWAP_PHASED, C_SWAP_TCTRL, C_SWAP_PAT, C_REMAP, C_GRAD, C_GRAD_CTRL, C_RESET, C_PARITY2, C_PARITY4,
-    C_PARITY_REAL, C_SHEAR2, C_SHEAR3, C_XSWAP_PERM, C_XSWAP_DPP, C_DIAG_FULL, C_DIAG_SLOT, C_DIAG_TSLOT, C_NCOV
+    C_PARITY_REAL, C_SHEAR2, C_SHEAR3, C_XSWAP_PERM, C_XSWAP_DPP, C_DIAG_FULL, C_DIAG_SLOT, C_DIAG_TSLOT, C_SHEARS,
+    C_NCOV
 };
 int64_t g_cov[C_NCOV];
 void count(const Instr &I) {
@@ -119,6 +120,7 @@ void count(const Instr &I) {
     case I_GRAD: g_cov[(tc || pt || tl) ? C_GRAD_CTRL : C_GRAD]++; break;
     case I_RESET: g_cov[C_RESET]++; break;
     case I_SHEAR: g_cov[I.k == 2 ? C_SHEAR2 : C_SHEAR3]++; break;
+    case I_SHEARS: g_cov[C_SHEARS]++; break;
     case I_XSWAP: g_cov[I.k >= 4 ? C_XSWAP_PERM : C_XSWAP_DPP]++; break;
     case I_PARITY:
         g_cov[I.k == 2 ? C_PARITY2 : C_PARITY4]++;
@@ -138,7 +140,13 @@ bool check_code(const Instr &I, const std::vector<double> &pool, std::string &er
     const bool ctrl = I.flags & (F_TCTRL | F_PAT);
     bool ok = false;
     if (I.flags & F_OUTER) ok = code == K_OUTER;
-    else if (code < K_DIAG_FULL) ok = I.op == I_SHEAR && I.s[0] == code / 2 && I.k == 2 + code % 2;
+    else if (code == K_SHEARS) {
+        ok = I.op == I_SHEARS && (I.k & ~0xFF) == 0 && I.smask > 0 && I.smask < 16;
+        for (int sl = 0; sl < 4 && ok; sl++) {
+            const int n = (I.k >> (2 * sl)) & 3;
+            ok = (n == 1 ? false : (n != 0) == ((I.smask >> sl & 1) != 0));
+        }
+    } else if (code < K_SHEARS) ok = I.op == I_SHEAR && I.s[0] == code / 2 && I.k == 2 + code % 2;
     else if (code < K_ANTI1) {
         bool tile_bits = I.k > 0 && !tc;
         for (int j = 0; j < I.k; j++) tile_bits = tile_bits && I.ls[j] >= 16;
@@ -478,6 +486,22 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                                 }
                                 if ((I.flags & F_TCTRL) && !lok) ph = 1;
                                 A[r] *= ph;
+                            }
+                            break;
+                        }
+                        case I_SHEARS: {
+                            // slot by slot, each the fused multiply-adds of one I_SHEAR
+                            const double *Cf = P.pool.data() + 2 * (size_t)I.mat;
+                            for (int S = 0; S < 4; S++) {
+                                const int n = (I.k >> (2 * S)) & 3;
+                                for (int p = 0; p < n; p++)
+                                    for (int r = 0; r < 16; r++) {
+                                        if (r >> S & 1) continue;
+                                        cd &dst = (p & 1) ? A[r | (1 << S)] : A[r];
+                                        const cd src = (p & 1) ? A[r] : A[r | (1 << S)];
+                                        const double c = Cf[3 * S + p];
+                                        dst = cd(std::fma(c, src.real(), dst.real()), std::fma(c, src.imag(), dst.imag()));
+                                    }
                             }
                             break;
                         }

@@ -74,7 +74,7 @@ COVERAGE = ["dense", "dense_tctrl", "dense_pat", "dense_tile", "dense_big", "dia
             "diag_kl2", "diag_tctrl", "diag_tile", "diag_regs", "anti_pure", "anti_phased", "anti_tctrl",
             "anti_pat", "swap_pure", "swap_phased", "swap_tctrl", "swap_pat", "remap", "grad", "grad_ctrl",
             "reset", "parity2", "parity4", "parity_real", "shear2", "shear3", "xswap_perm",
-            "xswap_dpp", "diag_full", "diag_slot", "diag_tslot"]
+            "xswap_dpp", "diag_full", "diag_slot", "diag_tslot", "shears"]
 
 
 def coverage(reset: bool = True) -> dict:
