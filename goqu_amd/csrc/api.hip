@@ -282,21 +282,87 @@ int goqu_sv_reset(goqu_sv_t *h) {
     GOQU_GUARD_END
 }
 
+// The content of a statevector op list as cache-key bytes: per op its kind, gate id, qubits and
+// matrix entries.  False for records the key does not cover (the op list then runs uncached and
+// the lowering reports any error).
+static bool sv_ops_key(const goqu_op_t *ops, int nops, std::vector<unsigned char> &key) {
+    key.clear();
+    auto put = [&](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        key.insert(key.end(), c, c + n);
+    };
+    for (int i = 0; i < nops; i++) {
+        const goqu_op_t &o = ops[i];
+        if (o.num_qubits < 0 || o.num_qubits > GOQU_MAX_OP_QUBITS) return false;
+        int mk = 0;  // the matrix acts on mk qubits
+        if (o.kind == GOQU_OP_GATE) mk = o.num_qubits;
+        else if (o.kind == GOQU_OP_CONTROLLED) mk = o.num_qubits - o.num_controls;
+        else if (o.kind != GOQU_OP_RESET && o.kind != GOQU_OP_SKIP) return false;
+        if ((o.kind == GOQU_OP_GATE || o.kind == GOQU_OP_CONTROLLED) && (mk < 1 || mk > kMaxWideTargets)) return false;
+        put(&o.kind, 4);
+        put(&o.gate_id, 4);
+        put(&o.num_qubits, 4);
+        put(&o.num_controls, 4);
+        put(o.qubits, sizeof(int32_t) * (size_t)o.num_qubits);
+        const unsigned char has = o.matrix != nullptr && mk > 0;
+        put(&has, 1);
+        if (has) put(o.matrix, sizeof(double) * (size_t(2) << (2 * mk)));
+    }
+    return true;
+}
+
 int goqu_sv_apply(goqu_sv_t *h, const goqu_op_t *ops, int32_t num_ops) {
     GOQU_GUARD_BEGIN
     if (!h || (num_ops > 0 && !ops)) return fail(GOQU_ERR_INVALID_ARG, "null argument");
     host_fault_point();
+    ProgramCache &C = h->cache;
+    std::vector<unsigned char> key;
+    const bool keyed = num_ops > 0 && sv_ops_key(ops, num_ops, key);
+    uint64_t hash = 1469598103934665603ull;  // FNV-1a
+    if (keyed)
+        for (unsigned char c : key) hash = (hash ^ c) * 1099511628211ull;
+    auto launch = [&](const PreparedOps &p) -> int {
+        int rc = launch_prepared(h->st, p, h->stream, nullptr);
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (rc) return rc;
+        if (e != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("stage kernels: ") + hipGetErrorString(e));
+        return GOQU_OK;
+    };
+    if (keyed)
+        for (ProgramCache::Entry &E : C.entries)
+            if (E.hash == hash && E.key == key) {
+                E.used = ++C.clock;
+                return launch(E.prep);
+            }
     std::vector<EOp> eops;
     std::string err;
     int rc = lower_sv_ops(ops, num_ops, h->nq_user, eops, err);
     if (rc) return fail(rc, err);
-    return run_eops(h->st, eops, h->stream, h->plan_set ? &h->plan : nullptr);
+    if (!keyed) return run_eops(h->st, eops, h->stream, h->plan_set ? &h->plan : nullptr);
+    PreparedOps prep;
+    const PlanConfig cfg = h->plan_set ? h->plan : default_plan_config(h->st.prec, h->st.nq, false);
+    rc = prepare_eops(h->st, eops, cfg, h->stream, prep);
+    if (rc) return rc;
+    if (C.entries.size() >= ProgramCache::kMaxEntries) {  // the least recently used goes
+        size_t lru = 0;
+        for (size_t i = 1; i < C.entries.size(); i++)
+            if (C.entries[i].used < C.entries[lru].used) lru = i;
+        C.entries.erase(C.entries.begin() + (std::ptrdiff_t)lru);
+    }
+    C.entries.emplace_back();
+    ProgramCache::Entry &E = C.entries.back();
+    E.key = std::move(key);
+    E.hash = hash;
+    E.used = ++C.clock;
+    E.prep = std::move(prep);
+    return launch(E.prep);
     GOQU_GUARD_END
 }
 
 int goqu_sv_set_plan(goqu_sv_t *h, int32_t tile_bits, int32_t low_bits) {
     GOQU_GUARD_BEGIN
     if (!h) return fail_cstr(GOQU_ERR_INVALID_ARG, "null handle");
+    h->cache.clear();  // cached programs follow the tile shape they were planned with
     if (tile_bits == 0 && low_bits == 0) {
         h->plan_set = false;
         return GOQU_OK;

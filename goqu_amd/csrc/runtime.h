@@ -114,7 +114,33 @@ struct PreparedOps {
     PreparedOps(const PreparedOps &) = delete;
     PreparedOps &operator=(const PreparedOps &) = delete;
     PreparedOps(PreparedOps &&o) noexcept : pieces(std::move(o.pieces)), stages(o.stages) { o.pieces.clear(); }
+    PreparedOps &operator=(PreparedOps &&o) noexcept {
+        if (this != &o) {
+            release();
+            pieces = std::move(o.pieces);
+            stages = o.stages;
+            o.pieces.clear();
+            o.stages = 0;
+        }
+        return *this;
+    }
     ~PreparedOps() { release(); }
+};
+
+// Planned, uploaded programs of a handle's recent Apply calls, keyed by the content of their op
+// records (kinds, qubits, matrices): repeating an op list (Evolve of one circuit, sampling runs,
+// repeated cost evaluations) launches the stored programs without lowering, planning or upload.
+struct ProgramCache {
+    struct Entry {
+        std::vector<unsigned char> key;
+        uint64_t hash = 0;
+        uint64_t used = 0;
+        PreparedOps prep;
+    };
+    static constexpr size_t kMaxEntries = 4;
+    std::vector<Entry> entries;
+    uint64_t clock = 0;
+    void clear() { entries.clear(); }
 };
 
 // ---- kernel profiling (profile.hip; goqu_profile_*) -------------------------------------------
@@ -228,6 +254,7 @@ struct goqu_sv {
     goqu::State adj_lam, adj_work;  // adjoint work states, allocated on first use
     hipStream_t stream = nullptr;
     goqu::Scratch scr;
+    goqu::ProgramCache cache;  // programs of recent Apply calls (cleared by goqu_sv_set_plan)
 };
 
 struct goqu_dm {

@@ -91,18 +91,11 @@ class Sim:
         """Apply (sim.go:92-157)."""
         if c.NumQubits() != self.num_qubits:
             raise ValueError(f"circuit has {c.NumQubits()} qubits, simulator has {self.num_qubits}")
-        buf = N.OpBuffer()
-        for op in c.Ops():
-            g = op.Gate
-            if g is None or op.ControlFlow is not None:
-                continue
-            if isinstance(g, G.StatePrepGate):
-                self._flush(buf)
-                buf = N.OpBuffer()
-                self._state_prep(g, op.Qubits)
-                continue
-            N.add_sv_gate(buf, g, op.Qubits)
-        self._flush(buf)
+        for kind, a, b in _segments(c):
+            if kind == "ops":
+                N.check(N.lib().goqu_sv_apply(self._h, a[0], b))
+            else:
+                self._state_prep(a, b)
 
     def _flush(self, buf: "N.OpBuffer") -> None:
         if len(buf):
@@ -229,6 +222,35 @@ class Sim:
         out = C.c_double(0)
         N.check(N.lib().goqu_sv_norm2(self._h, C.byref(out)))
         return out.value
+
+
+def _segments(c: Circuit) -> list:
+    """The circuit's op records for goqu_sv_apply, marshalled once per circuit (a built circuit
+    does not change): ("ops", (record array, the buffer owning its matrices), count) runs split at
+    StatePrep gates, which appear as ("prep", gate, qubits)."""
+    segs = getattr(c, "_sv_segments", None)
+    if segs is not None:
+        return segs
+    segs = []
+    buf = N.OpBuffer()
+
+    def close(buf):
+        if len(buf):
+            segs.append(("ops", (buf.array(), buf), len(buf)))
+
+    for op in c.Ops():
+        g = op.Gate
+        if g is None or op.ControlFlow is not None:
+            continue
+        if isinstance(g, G.StatePrepGate):
+            close(buf)
+            buf = N.OpBuffer()
+            segs.append(("prep", g, list(op.Qubits)))
+            continue
+        N.add_sv_gate(buf, g, op.Qubits)
+    close(buf)
+    c._sv_segments = segs
+    return segs
 
 
 def New(num_qubits: int, precision: int = N.C128, plan=None) -> Sim:

@@ -288,3 +288,39 @@ def test_wide_controlled_gates_mfma(n, nt, nc):
     print(f"wide n={n} targets={nt} controls={nc}", flush=True)
     _check(c)
     _check(c, TOL64, N.C64)
+
+
+def test_program_cache_repeats_and_misses():
+    """Apply keeps the planned programs of recent op lists per handle: repeating an op list
+    reuses them (twice without a reset is U.U), an op list that differs in one matrix entry or a
+    new tile shape plans afresh, and more distinct lists than the cache holds evict the oldest."""
+    n = 12
+    rng = np.random.default_rng(91)
+    c = random_circuit(n, 5, rng)
+    sim = statevector.New(n)
+    for _ in range(3):  # Evolve of one circuit, three times
+        sim.Evolve(c)
+        assert np.max(np.abs(sim.StateVector() - O.sv_evolve(c))) <= TOL128
+    sim.Apply(c)  # no reset: the cached program runs on U|0>
+    want2 = O.sv_evolve(c, O.sv_evolve(c))
+    assert np.max(np.abs(sim.StateVector() - want2)) <= TOL128
+    # one rotation angle changed by 1e-9: a different op list
+    b = NewBuilder("c1", n)
+    b2 = NewBuilder("c2", n)
+    for q in range(n):
+        b.RY(0.3 + 0.1 * q, q)
+        b2.RY(0.3 + 0.1 * q + (1e-9 if q == 5 else 0.0), q)
+    for q in range(n - 1):
+        b.CNOT(q, q + 1)
+        b2.CNOT(q, q + 1)
+    c1, c2 = b.Build(), b2.Build()
+    for cc in (c1, c2, c1, c2):
+        sim.Evolve(cc)
+        assert np.max(np.abs(sim.StateVector() - O.sv_evolve(cc))) <= TOL128
+    # a new tile shape replans; more lists than the cache holds
+    sim.SetPlan(8, 3)
+    circuits = [random_circuit(n, 2, np.random.default_rng(200 + i)) for i in range(6)]
+    for cc in circuits + circuits[::-1]:
+        sim.Evolve(cc)
+        assert np.max(np.abs(sim.StateVector() - O.sv_evolve(cc))) <= TOL128
+    sim.Close()
