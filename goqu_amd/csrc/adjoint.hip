@@ -210,14 +210,111 @@ static int alloc_like(State &s, const State &ref) {
     return GOQU_OK;
 }
 
+// h = vec(H): sum_t coeff_t i^{ny_t} P_t as a 2n-qubit vector (row bits high)
+static int build_h(State &h, int n, int nt, const uint64_t *x, const uint64_t *z, const int32_t *ny,
+                   const double *coeffs, hipStream_t s) {
+    int rc = fill_basis(h, ~0ull, s);
+    const uint64_t dim = 1ull << n;
+    for (int t = 0; t < nt && !rc; t++) {
+        double yr, yi;
+        ipow(ny[t], yr, yi);
+        const double cr = coeffs[2 * t] * yr - coeffs[2 * t + 1] * yi;
+        const double ci = coeffs[2 * t] * yi + coeffs[2 * t + 1] * yr;
+        const unsigned grid = (unsigned)std::min<uint64_t>((dim + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_add_pauli_matrix, dim3(grid), dim3(256), 0, s, static_cast<cx<double> *>(h.amps), dim,
+                           x[t], z[t], cr, ci);
+        if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "pauli matrix kernel launch failed");
+        // terms with the same x touch the same entries: keep launches ordered
+    }
+    return rc;
+}
+
+// The fused reverse sweep (when every op allows it, build_dm_reverse): the forward pass stores
+// rho at the end of every segment; going backwards, rho is restored from the segment's end
+// checkpoint (so uncomputing errors never cross a segment) and the segment's reverse program
+// runs on (rho, h) by paired stage sweeps, with the derivative terms reduced inside the sweeps.
+// fused = false: not applicable, nothing was run.
+static int adjoint_dm_fused(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_deriv_t *derivs, int nd,
+                            int np, int nt, const uint64_t *x, const uint64_t *z, const int32_t *ny,
+                            const double *coeffs, int every, const std::vector<std::vector<EOp>> &low,
+                            double *energy, double *grads, Scratch &scr, hipStream_t s, bool &fused) {
+    fused = false;
+    (void)every;  // segments follow the error growth of uncomputing, not a fixed op count
+    if (nops == 0) return GOQU_OK;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(GOQU_ERR_DEVICE, "hipMemGetInfo failed");
+    const size_t sbytes = rho.size() * rho.elem;
+    const int64_t max_ck = fr > 4 * sbytes ? (int64_t)((fr - 4 * sbytes) / sbytes) : 0;
+    // Segment boundaries: a segment ends before the bound on the error growth of uncomputing its
+    // ops (product of the inverse channels' norms) would exceed kMaxGrowth; rho is restored
+    // exactly at every boundary.
+    constexpr double kMaxGrowth = 4.0;
+    std::vector<int> bounds{0};
+    double grow = 1.0;
+    for (int k = 0; k < nops; k++) {
+        const double g = dm_uncompute_growth(ops[k], n);
+        if (!(g < kMaxGrowth)) return GOQU_OK;  // a reset, a singular or badly conditioned channel
+        if (grow * g > kMaxGrowth && k > bounds.back()) {
+            bounds.push_back(k);
+            grow = 1.0;
+        }
+        grow *= g;
+    }
+    bounds.push_back(nops);
+    const int nck = (int)bounds.size() - 1;
+    if (nck - 1 > max_ck) return GOQU_OK;
+    std::vector<std::vector<EOp>> rev(nck);
+    std::string err;
+    for (int c = 0; c < nck; c++) {
+        bool ok = false;
+        int rc = build_dm_reverse(ops, bounds[c], bounds[c + 1], n, derivs, nd, rev[c], ok, err);
+        if (rc) return fail(rc, err);
+        if (!ok) return GOQU_OK;
+    }
+    fused = true;
+    std::vector<State> ck(nck - 1);
+    State h;
+    auto cleanup = [&]() {
+        for (State &c : ck)
+            if (c.amps) (void)hipFree(c.amps);
+        if (h.amps) (void)hipFree(h.amps);
+    };
+    int rc = fill_basis(rho, 0, s);
+    for (int c = 0; c < nck && !rc; c++) {
+        std::vector<EOp> seg;
+        for (int k = bounds[c]; k < bounds[c + 1]; k++) seg.insert(seg.end(), low[k].begin(), low[k].end());
+        rc = run_eops(rho, seg, s);
+        if (!rc && c < nck - 1) rc = alloc_like(ck[c], rho);
+        if (!rc && c < nck - 1) rc = copy_state(ck[c], rho, s);
+    }
+    if (!rc) rc = alloc_like(h, rho);
+    if (!rc) rc = build_h(h, n, nt, x, z, ny, coeffs, s);
+    double e2[2] = {0, 0};
+    if (!rc) rc = inner_product(h, rho, e2, scr, s);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    *energy = e2[0];
+    for (int p = 0; p < np; p++) grads[p] = 0;
+    std::vector<double> vals;
+    for (int c = nck - 1; c >= 0 && !rc; c--) {
+        if (c < nck - 1) rc = copy_state(rho, ck[c], s);  // rho at the end of segment c, exactly
+        if (!rc) rc = run_pair(rho, h, rev[c], nd, vals, s);
+        for (int i = 0; i < nd && !rc; i++) grads[derivs[i].param] += derivs[i].scale * 2.0 * vals[i];
+    }
+    cleanup();
+    return rc;
+}
+
 // Reverse sweep by segments of `every` ops: rho at the start of every segment is stored on the
 // forward pass; going backwards, each segment that holds derivatives is replayed once from its
 // checkpoint, keeping rho_{k-1} of each derivative op k in a snapshot, and the sandwiches of an op's
-// derivatives (U3: three) share one pass over h and the snapshot.
+// derivatives (U3: three) share one pass over h and the snapshot.  Used when the fused sweep does
+// not apply (resets, channels that are not parity-block or are badly conditioned).
 int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_deriv_t *derivs, int nd, int np,
                int nt, const uint64_t *x, const uint64_t *z, const int32_t *ny, const double *coeffs, int every,
                double *energy, double *grads, Scratch &scr, hipStream_t s) {
-    if (every < 1) every = 1;
     const uint64_t valid = (1ull << n) - 1;
     for (int t = 0; t < nt; t++)
         if ((x[t] & ~valid) || (z[t] & ~valid)) return fail(GOQU_ERR_INVALID_ARG, "Pauli mask beyond the qubit count");
@@ -237,6 +334,13 @@ int adjoint_dm(State &rho, int n, const goqu_op_t *ops, int nops, const goqu_der
         int rc = lower_dm_ops(&ops[k], 1, n, low[k], err);
         if (rc) return fail(rc, "op " + std::to_string(k) + ": " + err);
     }
+    {
+        bool fused = false;
+        int rc = adjoint_dm_fused(rho, n, ops, nops, derivs, nd, np, nt, x, z, ny, coeffs, every, low, energy, grads,
+                                  scr, s, fused);
+        if (rc || fused) return rc;
+    }
+    if (every < 1) every = 1;
     // segment length: checkpoints plus one segment's snapshots must fit in free device memory
     // (h, the replay state and two more states stay free)
     auto snaps_of = [&](int ev) {

@@ -7,6 +7,8 @@
 // with D = dU_k U_k^dag, and then psi_{k-1} = U_k^dag psi_k, lam_{k-1} = U_k^dag lam_k.  The
 // whole reverse sweep is one program of EOP_GRAD ops and inverse ops applied to (psi, lam)
 // together (reference: gradient.go's parameter-shift rule computes the same derivatives).
+#include <algorithm>
+#include <cmath>
 #include <complex>
 #include <string>
 #include <vector>
@@ -126,6 +128,166 @@ int build_sv_reverse(const goqu_op_t *ops, int nops, int nq_user, const goqu_der
             rev.push_back(std::move(ie));
         }
     }
+    return GOQU_OK;
+}
+
+bool inverse_superop(const EOp &e, EOp &inv, double bound) {
+    if (e.kind != EOP_PARITY && e.kind != EOP_DENSE) return false;
+    const int d = 1 << e.k;
+    std::vector<cd> A((size_t)d * d), B((size_t)d * d, cd(0, 0));
+    for (int i = 0; i < d * d; i++) A[i] = cd(e.m[2 * i], e.m[2 * i + 1]);
+    for (int i = 0; i < d; i++) B[(size_t)i * d + i] = 1.0;
+    double scale = 0;
+    for (const cd &x : A) scale = std::max(scale, std::abs(x));
+    if (!(scale > 0)) return false;
+    for (int c = 0; c < d; c++) {  // Gauss-Jordan elimination with partial pivoting
+        int p = c;
+        for (int r = c + 1; r < d; r++)
+            if (std::abs(A[(size_t)r * d + c]) > std::abs(A[(size_t)p * d + c])) p = r;
+        if (std::abs(A[(size_t)p * d + c]) <= 1e-12 * scale) return false;
+        if (p != c)
+            for (int j = 0; j < d; j++) {
+                std::swap(A[(size_t)p * d + j], A[(size_t)c * d + j]);
+                std::swap(B[(size_t)p * d + j], B[(size_t)c * d + j]);
+            }
+        const cd piv = A[(size_t)c * d + c];
+        for (int j = 0; j < d; j++) {
+            A[(size_t)c * d + j] /= piv;
+            B[(size_t)c * d + j] /= piv;
+        }
+        for (int r = 0; r < d; r++) {
+            if (r == c) continue;
+            const cd f = A[(size_t)r * d + c];
+            if (f == cd(0, 0)) continue;
+            for (int j = 0; j < d; j++) {
+                A[(size_t)r * d + j] -= f * A[(size_t)c * d + j];
+                B[(size_t)r * d + j] -= f * B[(size_t)c * d + j];
+            }
+        }
+    }
+    inv = e;
+    for (int i = 0; i < d * d; i++) {
+        if (!(std::abs(B[i]) <= bound)) return false;
+        inv.m[2 * i] = B[i].real();
+        inv.m[2 * i + 1] = B[i].imag();
+    }
+    return true;
+}
+
+double dm_uncompute_growth(const goqu_op_t &op, int n) {
+    if (op.kind == GOQU_OP_RESET) return INFINITY;
+    if (op.kind != GOQU_OP_CHANNEL || op.num_kraus < 2) return 1.0;  // unitary: exact inverse
+    std::vector<EOp> low;
+    std::string err;
+    if (lower_dm_ops(&op, 1, n, low, err)) return INFINITY;
+    double g = 1.0;
+    for (EOp e : low) {
+        if (e.kind == EOP_DIAG) {
+            double mx = 0;
+            for (size_t i = 0; i < e.m.size(); i += 2) {
+                const double a = std::abs(cd(e.m[i], e.m[i + 1]));
+                if (!(a > 0)) return INFINITY;
+                mx = std::max(mx, 1.0 / a);
+            }
+            g *= mx;
+            continue;
+        }
+        EOp inv;
+        if (!inverse_superop(e, inv, 1e3)) return INFINITY;
+        const int d = 1 << e.k;
+        double mx = 0;  // infinity norm: the largest absolute row sum
+        for (int r = 0; r < d; r++) {
+            double rs = 0;
+            for (int c = 0; c < d; c++) rs += std::abs(cd(inv.m[2 * (r * d + c)], inv.m[2 * (r * d + c) + 1]));
+            mx = std::max(mx, rs);
+        }
+        g *= mx;
+    }
+    return g;
+}
+
+int build_dm_reverse(const goqu_op_t *ops, int k0, int k1, int n, const goqu_deriv_t *derivs, int nd,
+                     std::vector<EOp> &rev, bool &fused, std::string &err) {
+    rev.clear();
+    fused = false;
+    std::vector<std::vector<int>> by_op(k1 > k0 ? k1 - k0 : 0);
+    for (int i = 0; i < nd; i++)
+        if (derivs[i].op >= k0 && derivs[i].op < k1) by_op[derivs[i].op - k0].push_back(i);
+    for (int k = k1 - 1; k >= k0; k--) {
+        const goqu_op_t &op = ops[k];
+        for (int i : by_op[k - k0]) {
+            // D = dU U^dag acts on the row qubits (q + n) of rho's vector
+            goqu_op_t rop = op;
+            for (int j = 0; j < op.num_qubits; j++) rop.qubits[j] = op.qubits[j] + n;
+            EOp g;
+            if (op.kind != GOQU_OP_GATE || !grad_eop(rop, derivs[i].dmatrix, i, g)) {
+                err = "op " + std::to_string(k) + ": derivative of a gate that is not a 1-2 qubit matrix";
+                return GOQU_OK;
+            }
+            rev.push_back(std::move(g));
+        }
+        if (op.kind == GOQU_OP_RESET) {  // not invertible
+            err = "op " + std::to_string(k) + ": reset";
+            return GOQU_OK;
+        }
+        // a single Kraus operator runs as K (x) conj(K): its inverse is the conjugate transpose only
+        // when K is unitary, which is not checked here
+        if (op.kind == GOQU_OP_CHANNEL && op.num_kraus == 1) {
+            err = "op " + std::to_string(k) + ": single-Kraus channel";
+            return GOQU_OK;
+        }
+        std::vector<EOp> low;
+        int rc = lower_dm_ops(&op, 1, n, low, err);
+        if (rc) {
+            err = "op " + std::to_string(k) + ": " + err;
+            return rc;
+        }
+        const bool channel = op.kind == GOQU_OP_CHANNEL && op.num_kraus > 1;
+        for (int j = (int)low.size() - 1; j >= 0; j--) {
+            EOp e = low[j];
+            if (channel && e.kind == EOP_DIAG && e.cmask == 0 && (e.k == 2 || e.k == 4)) {
+                // a diagonal superoperator (phase damping ...) as a parity-block one, so that it can
+                // act on one state of the pair
+                const int d = 1 << e.k;
+                std::vector<double> full(2 * (size_t)d * d, 0.0);
+                for (int i = 0; i < d; i++) {
+                    full[2 * ((size_t)i * d + i)] = e.m[2 * i];
+                    full[2 * ((size_t)i * d + i) + 1] = e.m[2 * i + 1];
+                }
+                e.m = std::move(full);
+                e.kind = EOP_PARITY;
+            }
+            if (e.kind == EOP_WIDE) {
+                err = "op " + std::to_string(k) + ": gate on more than four targets";
+                return GOQU_OK;
+            }
+            if (channel) {
+                // rho moves through S^-1, h through S^dag; only parity-block channels (the paired
+                // kernel applies channels one state at a time through its parity-block code)
+                if (e.kind != EOP_PARITY) {
+                    err = "op " + std::to_string(k) + ": channel superoperator is not parity-block";
+                    return GOQU_OK;
+                }
+                EOp ri, hi;
+                if (!inverse_superop(e, ri, 1e3)) {
+                    err = "op " + std::to_string(k) + ": channel superoperator singular or badly conditioned";
+                    return GOQU_OK;
+                }
+                rc = invert_eop(e, hi, err);
+                if (rc) return rc;
+                ri.side = 1;
+                hi.side = 2;
+                rev.push_back(std::move(ri));
+                rev.push_back(std::move(hi));
+                continue;
+            }
+            EOp ie;
+            rc = invert_eop(e, ie, err);
+            if (rc) return rc;
+            rev.push_back(std::move(ie));
+        }
+    }
+    fused = true;
     return GOQU_OK;
 }
 

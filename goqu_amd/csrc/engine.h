@@ -51,6 +51,10 @@ struct EOp {
     std::vector<int> qw;    // EOP_WIDE: target qubits, qw[0] = MSB of the matrix index
     bool pure_swap = false; // ANTI1 / SWAP2 with unit phases: move only
     int32_t slot = -1;      // EOP_GRAD: global gradient slot
+    // paired programs: 0 = the op acts on both states, 1 = on the first only, 2 = on the second
+    // only (EOP_PARITY ops alone: a density-matrix reverse sweep uncomputes rho through the
+    // inverse of a channel while h moves through its adjoint)
+    int32_t side = 0;
     // which of the q[] are targets that must sit in registers (DENSE, ANTI1, SWAP2, RESET)
     bool needs_regs() const { return kind != EOP_DIAG; }
     bool is_grad() const { return kind == EOP_GRAD; }
@@ -104,6 +108,8 @@ constexpr int32_t F_PAT = 4;    // controls on register slots (pat)
 constexpr int32_t F_TILE = 8;   // controls on qubits outside the tile (omask), uniform per tile
 constexpr int32_t F_OUTER = 16; // run by the kernel outside its inner op loop (REMAP, XSWAP, RESET, dense on 3-4 slots)
 constexpr int32_t F_REAL = 32;  // I_PARITY, I_DENSE: every matrix entry is real (only real parts are read)
+constexpr int32_t F_SIDE_A = 64;  // I_PARITY in a paired program: acts on the first state only
+constexpr int32_t F_SIDE_B = 128; // I_PARITY in a paired program: acts on the second state only
 
 // Instr::flags, bits 8..15: the kernel code path of the instruction (assign_codes), so that the
 // stage kernel dispatches every op with one switch on a small integer.  Ranges:
@@ -260,5 +266,20 @@ bool grad_eop(const goqu_op_t &op, const double *dm, int slot, EOp &out);
 // fused = false (and rev empty) when some derivative does not fit an EOP_GRAD op.
 int build_sv_reverse(const goqu_op_t *ops, int nops, int nq_user, const goqu_deriv_t *derivs, int nd,
                      std::vector<EOp> &rev, bool &fused, std::string &err);
+// The inverse of a channel superoperator (EOP_PARITY or EOP_DENSE) by Gaussian elimination with
+// partial pivoting; false when it is singular or its largest entry exceeds `bound` (uncomputing
+// rho through it would amplify rounding errors too much).
+bool inverse_superop(const EOp &e, EOp &inv, double bound);
+// The reverse sweep of a density-matrix adjoint over ops [k0, k1) as one program on (rho, h), rho
+// the first state and h = vec(H) adjoint-propagated the second: for k = k1-1..k0, an EOP_GRAD per
+// derivative of op k (D = dU U^dag on its row qubits: 2 Re <h_k| D rho_k> is the derivative
+// term), then the inverse of op k: unitary pieces on both states, a channel superoperator S as
+// S^-1 on rho (side 1) and S^dag on h (side 2).  fused = false when some op cannot be run this way
+// (a reset, a channel that is not parity-block or badly conditioned, a wide or >2-target gate).
+int build_dm_reverse(const goqu_op_t *ops, int k0, int k1, int n, const goqu_deriv_t *derivs, int nd,
+                     std::vector<EOp> &rev, bool &fused, std::string &err);
+// A bound on how much uncomputing rho through the inverse of op k can grow errors (the infinity
+// norm of its inverse superoperator; 1 for gates, infinity for a reset or a singular channel).
+double dm_uncompute_growth(const goqu_op_t &op, int n);
 
 }  // namespace goqu

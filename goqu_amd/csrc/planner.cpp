@@ -1052,6 +1052,8 @@ Program plan_program(const std::vector<EOp> &eops_in, int nq, const PlanConfig &
                         P.pool.push_back(x.imag());
                     }
                     if (real) I.flags |= F_REAL;
+                    if (e.side == 1) I.flags |= F_SIDE_A;
+                    if (e.side == 2) I.flags |= F_SIDE_B;
                     break;
                 }
                 case EOP_GRAD:

@@ -1045,9 +1045,18 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
 #undef DENSE2_CASE
                 } else if (code < K_GRAD) {
                     if constexpr (DM) {  // density-matrix channel superoperators
-#define PAR2_CASE(P, LO, HI)                                                          \
-    case K_PARITY2 + 2 * P + 0: op_parity2<LO, HI, true>(a, M); break;                  \
-    case K_PARITY2 + 2 * P + 1: op_parity2<LO, HI, false>(a, M); break;
+                        // paired programs: F_SIDE_A / F_SIDE_B restrict a channel to one state
+                        auto par = [&](auto fn) __attribute__((always_inline)) {
+                            if (!(cur.flags & F_SIDE_B)) fn(a);
+                            if constexpr (PAIR)
+                                if (!(cur.flags & F_SIDE_A)) fn(b);
+                        };
+#define PAR2_CASE(P, LO, HI)                                                                             \
+    case K_PARITY2 + 2 * P + 0: par([&](cx<R>(&v)[16]) __attribute__((always_inline)) { op_parity2<LO, HI, true>(v, M); }); break; \
+    case K_PARITY2 + 2 * P + 1: par([&](cx<R>(&v)[16]) __attribute__((always_inline)) { op_parity2<LO, HI, false>(v, M); }); break;
+#define PAR4_CASE(C, S1, S2, S3)                                                                         \
+    case K_PARITY4 + 2 * C + 0: par([&](cx<R>(&v)[16]) __attribute__((always_inline)) { op_parity4<S1, S2, S3, true>(v, M); }); break; \
+    case K_PARITY4 + 2 * C + 1: par([&](cx<R>(&v)[16]) __attribute__((always_inline)) { op_parity4<S1, S2, S3, false>(v, M); }); break;
                         switch (code) {
                             PAR2_CASE(0, 0, 1)
                             PAR2_CASE(1, 0, 2)
@@ -1055,14 +1064,12 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
                             PAR2_CASE(3, 1, 2)
                             PAR2_CASE(4, 1, 3)
                             PAR2_CASE(5, 2, 3)
-                        case K_PARITY4 + 0: op_parity4<1, 2, 3, true>(a, M); break;
-                        case K_PARITY4 + 1: op_parity4<1, 2, 3, false>(a, M); break;
-                        case K_PARITY4 + 2: op_parity4<2, 1, 3, true>(a, M); break;
-                        case K_PARITY4 + 3: op_parity4<2, 1, 3, false>(a, M); break;
-                        case K_PARITY4 + 4: op_parity4<3, 1, 2, true>(a, M); break;
-                        case K_PARITY4 + 5: op_parity4<3, 1, 2, false>(a, M); break;
+                            PAR4_CASE(0, 1, 2, 3)
+                            PAR4_CASE(1, 2, 1, 3)
+                            PAR4_CASE(2, 3, 1, 2)
                         default: break;
                         }
+#undef PAR4_CASE
 #undef PAR2_CASE
                     }
                 } else {
@@ -1211,18 +1218,31 @@ __global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<
     double *gacc = reinterpret_cast<double *>(smem_raw + (size_t(8) << hdr->m));
     stage_body<R, true, true, false>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
 }
+// Paired density-matrix stages (the reverse sweep of the density-matrix adjoint): rho and
+// h = vec(H) together, channel superoperators on one state or the other (F_SIDE_A / F_SIDE_B).
+template <typename R>
+__global__ void __launch_bounds__(256) k_stage_pair_dm(cx<R> *__restrict__ psi, cx<R> *__restrict__ lam,
+                                                    const StageHeader *__restrict__ hdr,
+                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
+                                                    uint64_t ntiles, double *__restrict__ gpart) {
+    // the gradient partials follow the tile's remap area in dynamic LDS (the remaps address LDS
+    // from 0, so the kernel holds no static LDS)
+    extern __shared__ unsigned char smem_raw[];
+    double *gacc = reinterpret_cast<double *>(smem_raw + (size_t(8) << hdr->m));
+    stage_body<R, true, true, true>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
+}
 
-// The seven stage kernels take minutes each to compile.  build.py compiles this file once per
-// kernel with GOQU_STAGE_KERNEL = 1..7 (that kernel's code only), in parallel, and once without
+// The eight stage kernels take minutes each to compile.  build.py compiles this file once per
+// kernel with GOQU_STAGE_KERNEL = 1..8 (that kernel's code only), in parallel, and once without
 // it: the host code below, which refers to the kernels as extern instantiations.
 #define GOQU_STAGE_INST(EXT, K, R)                                                                \
     EXT template __global__ void K<R>(cx<R> *__restrict__, const StageHeader *__restrict__,          \
                                       const Instr *__restrict__, const R *__restrict__, uint64_t,      \
                                       const uint8_t *__restrict__, uint64_t, uint64_t);
-#define GOQU_PAIR_INST(EXT, R)                                                                     \
-    EXT template __global__ void k_stage_pair<R>(cx<R> *__restrict__, cx<R> *__restrict__,           \
-                                                 const StageHeader *__restrict__, const Instr *__restrict__, \
-                                                 const R *__restrict__, uint64_t, double *__restrict__);
+#define GOQU_PAIR_INST(EXT, K, R)                                                                  \
+    EXT template __global__ void K<R>(cx<R> *__restrict__, cx<R> *__restrict__,                     \
+                                      const StageHeader *__restrict__, const Instr *__restrict__,     \
+                                      const R *__restrict__, uint64_t, double *__restrict__);
 #ifdef GOQU_STAGE_KERNEL
 #if GOQU_STAGE_KERNEL == 1
 GOQU_STAGE_INST(, k_stage, double)
@@ -1233,11 +1253,13 @@ GOQU_STAGE_INST(, k_stage_big, double)
 #elif GOQU_STAGE_KERNEL == 4
 GOQU_STAGE_INST(, k_stage_big, float)
 #elif GOQU_STAGE_KERNEL == 5
-GOQU_PAIR_INST(, double)
+GOQU_PAIR_INST(, k_stage_pair, double)
 #elif GOQU_STAGE_KERNEL == 6
-GOQU_PAIR_INST(, float)
+GOQU_PAIR_INST(, k_stage_pair, float)
 #elif GOQU_STAGE_KERNEL == 7
 GOQU_STAGE_INST(, k_stage_dm, double)
+#elif GOQU_STAGE_KERNEL == 8
+GOQU_PAIR_INST(, k_stage_pair_dm, double)
 #endif
 }  // namespace goqu
 #else  // the host-code object
@@ -1245,9 +1267,10 @@ GOQU_STAGE_INST(extern, k_stage, double)
 GOQU_STAGE_INST(extern, k_stage, float)
 GOQU_STAGE_INST(extern, k_stage_big, double)
 GOQU_STAGE_INST(extern, k_stage_big, float)
-GOQU_PAIR_INST(extern, double)
-GOQU_PAIR_INST(extern, float)
+GOQU_PAIR_INST(extern, k_stage_pair, double)
+GOQU_PAIR_INST(extern, k_stage_pair, float)
 GOQU_STAGE_INST(extern, k_stage_dm, double)
+GOQU_PAIR_INST(extern, k_stage_pair_dm, double)
 
 // ---------------------------------------------------------------------------------------------
 // Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS) holding the 4 lowest qubits.  On the
@@ -1437,8 +1460,17 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
         // 8-byte LDS elements, then the per-wave gradient partials
         const size_t lds = (size_t(1) << m) * 8 + kMaxGradPerStage * 8 * sizeof(double);
         hipEvent_t ev0 = prof_begin(s);
-        hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),
-                           static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
+        if (H.flags & SF_DM) {  // channel superoperators: the paired density-matrix kernel (complex128)
+            if constexpr (sizeof(R) == 8)
+                hipLaunchKernelGGL(k_stage_pair_dm<double>, dim3(grid), dim3(threads), lds, s,
+                                   static_cast<cx<double> *>(psi.amps), static_cast<cx<double> *>(lam.amps),
+                                   hdrs + i, instrs, pool, ntiles, gpart);
+            else
+                rc = fail(GOQU_ERR_UNSUPPORTED, "paired density-matrix stages need complex128");
+        } else {
+            hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),
+                               static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
+        }
         if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "paired stage launch failed");
         prof_end(ev0, s, PROF_STAGE_PAIR, 4.0 * psi.size() * sizeof(cx<R>));  // read + write of psi and lambda
         if (!rc && H.ngrad) {

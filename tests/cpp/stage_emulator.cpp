@@ -184,6 +184,11 @@ bool check_code(const Instr &I, const std::vector<double> &pool, std::string &er
     } else if (code < K_GRAD + 10) {
         ok = I.op == I_GRAD && I.smask == grad_masks[code - K_GRAD];
     }
+    if ((I.flags & (F_SIDE_A | F_SIDE_B)) && I.op != I_PARITY) {
+        ok = false;
+        err = "a one-state flag on an op other than I_PARITY";
+        return ok;
+    }
     if (!ok) err = "instruction code " + std::to_string(code) + " does not select op " + std::to_string(I.op);
     return ok;
 }
@@ -404,10 +409,13 @@ int run(Program P, cd *st, int nq, std::string &err, cd *st2 = nullptr, std::vec
                             break;
                         }
                         case I_PARITY: {
-                            if (st2) {
-                                err = "I_PARITY in a paired program";
+                            // paired programs (density-matrix reverse sweeps): a channel on one
+                            // state or the other, as the flags say
+                            if ((I.flags & F_SIDE_A) && (I.flags & F_SIDE_B)) {
+                                err = "I_PARITY on neither state";
                                 return 1;
                             }
+                            if ((w == 1 && (I.flags & F_SIDE_A)) || (w == 0 && (I.flags & F_SIDE_B))) break;
                             const bool real = I.flags & F_REAL;
                             auto B = [&](int off) { return real ? cd(pool[2 * off], 0) : cd(pool[2 * off], pool[2 * off + 1]); };
                             if (I.k == 2) {
@@ -714,6 +722,47 @@ int emu_sv_reverse(double *psi, double *lam, int nq_internal, int nq_user, const
     }
     return 0;
 }
+}
+
+// The density-matrix adjoint's fused reverse sweep over ops [k0, k1) on (rho, h) (2n-qubit
+// vectors) in place; vals[i] receives Re <h_k| D_i rho_k> of derivative i.
+extern "C" int emu_dm_reverse(double *rho, double *h, int n, const goqu_op_t *ops, int k0, int k1,
+                              const goqu_deriv_t *derivs, int nd, int tile_bits, int low_bits, double *vals,
+                              int64_t *num_stages) {
+    std::vector<EOp> rev;
+    bool fused = false;
+    std::string err;
+    int rc = build_dm_reverse(ops, k0, k1, n, derivs, nd, rev, fused, err);
+    if (rc) {
+        g_err = err;
+        return rc;
+    }
+    if (!fused) {
+        g_err = "not fusable: " + err;
+        return 200;
+    }
+    PlanConfig cfg;
+    cfg.tile_bits = tile_bits;
+    cfg.low_bits = low_bits;
+    const int nq = std::max(2 * n, kMinInternalQubits);
+    Program P = plan_program(rev, nq, cfg);
+    if (num_stages) *num_stages = (int64_t)P.stages.size();
+    std::vector<double> pv;
+    rc = run(P, reinterpret_cast<cd *>(rho), nq, err, reinterpret_cast<cd *>(h), &pv);
+    if (rc) {
+        g_err = err;
+        return 100;
+    }
+    for (int i = 0; i < nd; i++) vals[i] = 0;
+    for (int j = 0; j < P.num_grad; j++) {
+        const int slot = P.grad_slots[j];
+        if (slot < 0 || slot >= nd) {
+            g_err = "gradient slot outside the derivative list";
+            return 100;
+        }
+        vals[slot] += pv[j];
+    }
+    return 0;
 }
 
 extern "C" int emu_plan_stats(const goqu_op_t *ops, int nops, int nq, int tile_bits, int low_bits, int64_t *stages,

@@ -37,6 +37,10 @@ def lib():
                                      C.POINTER(N.GoquOp), C.c_int, C.POINTER(N.GoquDeriv), C.c_int, C.c_int,
                                      C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.emu_sv_reverse.restype = C.c_int
+        L.emu_dm_reverse.argtypes = [C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.POINTER(N.GoquOp),
+                                     C.c_int, C.c_int, C.POINTER(N.GoquDeriv), C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.emu_dm_reverse.restype = C.c_int
         L.emu_coverage.argtypes = [C.POINTER(C.c_int64), C.c_int, C.c_int]
         L.emu_coverage.restype = C.c_int
         _lib = L
@@ -65,6 +69,21 @@ def sv_reverse(psi: np.ndarray, lam: np.ndarray, nq_user: int, buf: "N.OpBuffer"
     dp = lambda a: a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
     rc = lib().emu_sv_reverse(dp(psi), dp(lam), nqi, nq_user, buf.array(), len(buf), derivs, nd, tile_bits,
                               low_bits, vals.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ns))
+    if rc:
+        raise RuntimeError(f"emulator status {rc}: {lib().emu_error().decode()}")
+    return vals[:nd], ns.value
+
+
+def dm_reverse(rho: np.ndarray, h: np.ndarray, n: int, ops, k0: int, k1: int, derivs, nd: int, tile_bits: int = 8,
+               low_bits: int = 3):
+    """The density-matrix adjoint's fused reverse sweep over ops [k0, k1) on rho and h (complex128
+    vectors of 2n qubits, padded to the internal width) in place.  Returns (per-derivative
+    Re <h|D rho>, stage count); raises when the ops do not allow the fused sweep."""
+    vals = np.zeros(max(nd, 1))
+    ns = C.c_int64(0)
+    dp = lambda a: a.view(np.float64).ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    rc = lib().emu_dm_reverse(dp(rho), dp(h), n, ops, k0, k1, derivs, nd, tile_bits, low_bits,
+                              vals.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ns))
     if rc:
         raise RuntimeError(f"emulator status {rc}: {lib().emu_error().decode()}")
     return vals[:nd], ns.value

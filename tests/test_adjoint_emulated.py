@@ -215,3 +215,108 @@ def test_fused_reverse_controlled_record_derivative():
         fd = (-energy_of(t + 2 * h) + 8 * energy_of(t + h) - 8 * energy_of(t - h) + energy_of(t - 2 * h)) / (12 * h)
         assert abs(g - fd) < 1e-9, (tile, g, fd)
     assert E.coverage(reset=True)["grad_ctrl"] > 0
+
+
+def _dm_noise(p=0.2):
+    from goqu_amd import noise as NZ
+
+    k1 = np.sqrt(p) * np.array([[0, 1], [1j, 0]])
+    nm = NZ.New()
+    nm.AddDefaultError(1, NZ.Depolarizing1Q(0.02))
+    nm.AddDefaultError(2, NZ.Depolarizing2Q(0.03))
+    nm.AddGateError("H", NZ.Channel("cplx1", 1, [np.sqrt(1 - p) * np.eye(2), k1]))
+    nm.AddGateError("RY", NZ.AmplitudeDamping(0.05))
+    nm.AddGateError("delay", NZ.PhaseDamping(0.04))
+    return nm
+
+
+def _dm_energy(c, names, x, H, nm, n):
+    from goqu_amd import densitymatrix as D
+
+    rho = O.dm_zero(n)
+    assert O.dm_apply_ops(rho, n, D.marshal(Bind(c, dict(zip(names, x))), nm)) == 0
+    return sum(O.dm_expect(rho, n, t) for t in H.Terms()).real
+
+
+def _h_matrix(H, n):
+    """vec(H): H[r][c] at index r 2^n + c (row bits high, the density matrix's layout)."""
+    dim = 1 << n
+    M = np.zeros((dim, dim), dtype=np.complex128)
+    for col in range(dim):
+        e = np.zeros(dim, dtype=np.complex128)
+        e[col] = 1
+        M[:, col] = apply_h(H, e)
+    return M.reshape(-1)
+
+
+@pytest.mark.parametrize("n,tile,low", [(2, 6, 2), (3, 6, 2), (4, 8, 3)])
+def test_dm_fused_reverse_vs_finite_differences(n, tile, low):
+    """The density-matrix adjoint's fused reverse sweep (build_dm_reverse: EOP_GRAD ops on the row
+    qubits, gates inverted on rho and h together, channels as S^-1 on rho and S^dag on h) run by
+    the emulator on (rho, vec(H)): gradients against 4th-order central differences of oracle
+    energies (the oracle's Kraus loop), under depolarizing, amplitude-damping, phase-damping and a
+    complex parity-block channel; rho is uncomputed back to |0><0|."""
+    th = NewVector("p", 6 * n)
+    b = NewBuilder("dmf", n)
+    k = 0
+    for layer in range(2):
+        for q in range(n):
+            b.SymU3(th.At(k), th.At(k + 1), th.At(k + 2), q)
+            k += 3
+        for q in range(layer % 2, n - 1, 2):
+            b.CNOT(q, q + 1)
+        b.H(0).SymRY(th.At(k % (6 * n)), n - 1).Delay(n - 1, 100.0)
+        if n > 2:
+            b.SymRZZ(th.At((k + 1) % (6 * n)), 0, n - 1)
+    c = b.Build()
+    names = FreeParameters(c)
+    nm = _dm_noise()
+    rng = np.random.default_rng(40 + n)
+    H = random_hamiltonian(n, 6, rng, 3)
+    x = rng.uniform(0, 2 * math.pi, len(names))
+    prog = GR.AdjointProgram(c, names, density=True, noise=nm)
+    buf, derivs, nd, keep = prog.build(x)
+    rho = O.dm_zero(n)
+    assert O.dm_apply_ops(rho, n, buf) == 0
+    hv = _h_matrix(H, n)
+    e = float(np.real(np.vdot(hv, rho)))
+    assert abs(e - _dm_energy(c, names, x, H, nm, n)) < 1e-12
+    nqi = max(2 * n, 6)
+    R = np.zeros(1 << nqi, dtype=np.complex128)
+    Hv = np.zeros(1 << nqi, dtype=np.complex128)
+    R[: len(rho)] = rho
+    Hv[: len(hv)] = hv
+    vals, _ = E.dm_reverse(R, Hv, n, buf.array(), 0, len(buf), derivs, nd, tile_bits=tile, low_bits=low)
+    g = np.zeros(len(names))
+    for i in range(nd):
+        g[derivs[i].param] += derivs[i].scale * 2.0 * vals[i]
+    fd = fd4(lambda v: _dm_energy(c, names, v, H, nm, n), x)
+    assert np.max(np.abs(g - fd)) < 1e-9, np.max(np.abs(g - fd))
+    zero = np.zeros(1 << nqi, dtype=np.complex128)
+    zero[0] = 1
+    assert np.max(np.abs(R - zero)) < 1e-11, "reverse sweep did not uncompute rho"
+
+
+def test_dm_fused_reverse_declines_resets_and_singular_channels():
+    """A reset (not invertible) and a fully depolarizing channel (singular superoperator) leave the
+    fused sweep unavailable: the driver then takes the checkpointed method."""
+    from goqu_amd import noise as NZ
+
+    n = 2
+    th = NewVector("p", 2)
+    for variant in range(2):
+        b = NewBuilder("dmr", n)
+        b.SymRY(th.At(0), 0).CNOT(0, 1).SymRX(th.At(1), 1)
+        nm = NZ.New()
+        if variant == 0:
+            b.Reset(1)
+        else:
+            nm.AddGateError("CNOT", NZ.Depolarizing2Q(15.0 / 16.0))
+        c = b.Build()
+        names = FreeParameters(c)
+        prog = GR.AdjointProgram(c, names, density=True, noise=nm)
+        buf, derivs, nd, keep = prog.build(np.array([0.3, 0.7]))
+        R = np.zeros(1 << 6, dtype=np.complex128)
+        Hv = np.zeros(1 << 6, dtype=np.complex128)
+        with pytest.raises(RuntimeError, match="status 200"):
+            E.dm_reverse(R, Hv, n, buf.array(), 0, len(buf), derivs, nd, tile_bits=6, low_bits=2)
