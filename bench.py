@@ -291,7 +291,9 @@ def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
     pairs, Depolarizing1Q(1e-3) / Depolarizing2Q(1e-2) default errors and an AmplitudeDamping(1e-3)
     Delay per qubit at each layer end.  Forward: Evolve + <sum Z_i>, value = stage sweeps x
     32 B x 4^n / time.  Gradient: <sum Z_i> and all 3 x 14 x layers U3 angles by the
-    checkpointed density-matrix adjoint."""
+    density-matrix adjoint (a forward pass, then paired reverse sweeps over rho and h = vec(H)
+    that uncompute rho through the inverse channels); its roofline objects cover the paired
+    kernel (read + write of both states per sweep) and the forward sweeps."""
     from goqu_amd import densitymatrix
     from goqu_amd import gradient as GR
     from goqu_amd import noise as NZ
@@ -339,9 +341,14 @@ def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
            "forward_GBps": round(launches * sweep / t_fwd / 1e9, 1),
            "k_stage_avg_ms": round(kms / max(launches, 1), 3), "energy": e, "roofline": roof}
     g = GR.AdjointDensity(c, H, names, noise=nm, checkpoint_every=16)
+    N.profile_read()
+    N.profile(True)
     t0 = time.perf_counter()
     e2, grad = g.value_and_grad(x)
     out["gradient_s"] = round(time.perf_counter() - t0, 3)
+    out["gradient_roofline"] = [r for r in (_roof("k_stage_pair_dm<double>", N.PROF_STAGE_PAIR),
+                                            _roof("k_stage_dm<double>", N.PROF_STAGE_DM)) if r]
+    N.profile(False)
     out["gradient_params"] = len(names)
     out["gradient_energy_matches_forward"] = bool(abs(e2 - e) < 1e-9)
     return out

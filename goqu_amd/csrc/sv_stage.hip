@@ -1207,33 +1207,34 @@ __global__ void __launch_bounds__(512) k_stage_dm(cx<R> *__restrict__ st, const 
                                      blob_at(pool, bo), ntiles, nullptr, nullptr);
 }
 
-template <typename R>
-__global__ void __launch_bounds__(256) k_stage_pair(cx<R> *__restrict__ psi, cx<R> *__restrict__ lam,
-                                                    const StageHeader *__restrict__ hdr,
-                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                                    uint64_t ntiles, double *__restrict__ gpart) {
-    // the gradient partials follow the tile's remap area in dynamic LDS (the remaps address LDS
-    // from 0, so the kernel holds no static LDS)
-    extern __shared__ unsigned char smem_raw[];
-    double *gacc = reinterpret_cast<double *>(smem_raw + (size_t(8) << hdr->m));
-    stage_body<R, true, true, false>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
-}
-// Paired density-matrix stages (the reverse sweep of the density-matrix adjoint): rho and
-// h = vec(H) together, channel superoperators on one state or the other (F_SIDE_A / F_SIDE_B).
-template <typename R>
-__global__ void __launch_bounds__(256) k_stage_pair_dm(cx<R> *__restrict__ psi, cx<R> *__restrict__ lam,
-                                                    const StageHeader *__restrict__ hdr,
-                                                    const Instr *__restrict__ prog, const R *__restrict__ pool,
-                                                    uint64_t ntiles, double *__restrict__ gpart) {
-    // the gradient partials follow the tile's remap area in dynamic LDS (the remaps address LDS
-    // from 0, so the kernel holds no static LDS)
-    extern __shared__ unsigned char smem_raw[];
-    double *gacc = reinterpret_cast<double *>(smem_raw + (size_t(8) << hdr->m));
-    stage_body<R, true, true, true>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);
-}
+// Paired stages (adjoint reverse sweeps: psi and lambda, or rho and h, through the same tiles).
+// The lean kernels run stages without 3-4-slot dense ops in at most 168 VGPRs (three waves per
+// SIMD; the two states' amplitudes alone take 128); the _big kernels run the stages that hold
+// them (SF_BIG).  The gradient partials follow the tile's remap area in dynamic LDS (the remaps
+// address LDS from 0, so the kernels hold no static LDS).
+#define GOQU_PAIR_KERNEL(NAME, ATTR, BIG, DM)                                                          \
+    template <typename R>                                                                             \
+    __global__ void __launch_bounds__(256) ATTR NAME(cx<R> *__restrict__ psi, cx<R> *__restrict__ lam,  \
+                                                     const StageHeader *__restrict__ hdr,             \
+                                                     const Instr *__restrict__ prog,                  \
+                                                     const R *__restrict__ pool, uint64_t ntiles,      \
+                                                     double *__restrict__ gpart) {                    \
+        extern __shared__ unsigned char smem_raw[];                                                   \
+        double *gacc = reinterpret_cast<double *>(smem_raw + (size_t(8) << hdr->m));                 \
+        stage_body<R, true, BIG, DM>(psi, lam, hdr, prog, pool, ntiles, gpart, gacc);                 \
+    }
+#define GOQU_THREE_WAVES __attribute__((amdgpu_waves_per_eu(3, 8)))
+GOQU_PAIR_KERNEL(k_stage_pair, GOQU_THREE_WAVES, false, false)
+GOQU_PAIR_KERNEL(k_stage_pair_big, , true, false)
+// density-matrix reverse sweeps: rho and h = vec(H), channel superoperators on one state or the
+// other (F_SIDE_A / F_SIDE_B)
+GOQU_PAIR_KERNEL(k_stage_pair_dm, GOQU_THREE_WAVES, false, true)
+GOQU_PAIR_KERNEL(k_stage_pair_dm_big, , true, true)
+#undef GOQU_THREE_WAVES
+#undef GOQU_PAIR_KERNEL
 
-// The eight stage kernels take minutes each to compile.  build.py compiles this file once per
-// kernel with GOQU_STAGE_KERNEL = 1..8 (that kernel's code only), in parallel, and once without
+// The eleven stage kernels take minutes each to compile.  build.py compiles this file once per
+// kernel with GOQU_STAGE_KERNEL = 1..11 (that kernel's code only), in parallel, and once without
 // it: the host code below, which refers to the kernels as extern instantiations.
 #define GOQU_STAGE_INST(EXT, K, R)                                                                \
     EXT template __global__ void K<R>(cx<R> *__restrict__, const StageHeader *__restrict__,          \
@@ -1260,6 +1261,12 @@ GOQU_PAIR_INST(, k_stage_pair, float)
 GOQU_STAGE_INST(, k_stage_dm, double)
 #elif GOQU_STAGE_KERNEL == 8
 GOQU_PAIR_INST(, k_stage_pair_dm, double)
+#elif GOQU_STAGE_KERNEL == 9
+GOQU_PAIR_INST(, k_stage_pair_big, double)
+#elif GOQU_STAGE_KERNEL == 10
+GOQU_PAIR_INST(, k_stage_pair_big, float)
+#elif GOQU_STAGE_KERNEL == 11
+GOQU_PAIR_INST(, k_stage_pair_dm_big, double)
 #endif
 }  // namespace goqu
 #else  // the host-code object
@@ -1271,6 +1278,9 @@ GOQU_PAIR_INST(extern, k_stage_pair, double)
 GOQU_PAIR_INST(extern, k_stage_pair, float)
 GOQU_STAGE_INST(extern, k_stage_dm, double)
 GOQU_PAIR_INST(extern, k_stage_pair_dm, double)
+GOQU_PAIR_INST(extern, k_stage_pair_big, double)
+GOQU_PAIR_INST(extern, k_stage_pair_big, float)
+GOQU_PAIR_INST(extern, k_stage_pair_dm_big, double)
 
 // ---------------------------------------------------------------------------------------------
 // Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS) holding the 4 lowest qubits.  On the
@@ -1460,16 +1470,18 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
         // 8-byte LDS elements, then the per-wave gradient partials
         const size_t lds = (size_t(1) << m) * 8 + kMaxGradPerStage * 8 * sizeof(double);
         hipEvent_t ev0 = prof_begin(s);
-        if (H.flags & SF_DM) {  // channel superoperators: the paired density-matrix kernel (complex128)
+        const bool big = H.flags & SF_BIG;
+        if (H.flags & SF_DM) {  // channel superoperators: the paired density-matrix kernels (complex128)
             if constexpr (sizeof(R) == 8)
-                hipLaunchKernelGGL(k_stage_pair_dm<double>, dim3(grid), dim3(threads), lds, s,
-                                   static_cast<cx<double> *>(psi.amps), static_cast<cx<double> *>(lam.amps),
-                                   hdrs + i, instrs, pool, ntiles, gpart);
+                hipLaunchKernelGGL(big ? k_stage_pair_dm_big<double> : k_stage_pair_dm<double>, dim3(grid),
+                                   dim3(threads), lds, s, static_cast<cx<double> *>(psi.amps),
+                                   static_cast<cx<double> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
             else
                 rc = fail(GOQU_ERR_UNSUPPORTED, "paired density-matrix stages need complex128");
         } else {
-            hipLaunchKernelGGL(k_stage_pair<R>, dim3(grid), dim3(threads), lds, s, static_cast<cx<R> *>(psi.amps),
-                               static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool, ntiles, gpart);
+            hipLaunchKernelGGL(big ? k_stage_pair_big<R> : k_stage_pair<R>, dim3(grid), dim3(threads), lds, s,
+                               static_cast<cx<R> *>(psi.amps), static_cast<cx<R> *>(lam.amps), hdrs + i, instrs, pool,
+                               ntiles, gpart);
         }
         if (hipGetLastError() != hipSuccess) rc = fail(GOQU_ERR_DEVICE, "paired stage launch failed");
         prof_end(ev0, s, PROF_STAGE_PAIR, 4.0 * psi.size() * sizeof(cx<R>));  // read + write of psi and lambda

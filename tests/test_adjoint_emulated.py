@@ -320,3 +320,43 @@ def test_dm_fused_reverse_declines_resets_and_singular_channels():
         Hv = np.zeros(1 << 6, dtype=np.complex128)
         with pytest.raises(RuntimeError, match="status 200"):
             E.dm_reverse(R, Hv, n, buf.array(), 0, len(buf), derivs, nd, tile_bits=6, low_bits=2)
+
+
+def test_dm_fused_reverse_three_qubit_dense():
+    """A fixed 3-qubit unitary (dense ops on three row and three column register slots: stages
+    for the paired kernels' big variants) in the density-matrix reverse sweep: rho = |psi><psi|
+    from the statevector oracle is uncomputed to |0><0| and the gradient matches statevector
+    oracle finite differences (the density-matrix oracle, like the reference, takes no 3-qubit
+    gates)."""
+    from goqu_amd import gate as G
+
+    rng = np.random.default_rng(23)
+    a = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    q, r = np.linalg.qr(a)
+    u3q = q * (np.diag(r) / np.abs(np.diag(r)))
+    n = 3
+    th = NewVector("q", 6)
+    b = NewBuilder("dmbig", n)
+    for i in range(n):
+        b.SymRY(th.At(i), i)
+    b.Apply(G.Unitary("u3q", u3q.reshape(-1), 3), 2, 0, 1)
+    for i in range(n):
+        b.SymRX(th.At(n + i), i)
+    c = b.Build()
+    names = FreeParameters(c)
+    H = random_hamiltonian(n, 6, rng, 3)
+    x = rng.uniform(0, 2 * math.pi, len(names))
+    prog = GR.AdjointProgram(c, names, density=True)
+    buf, derivs, nd, keep = prog.build(x)
+    psi = O.sv_evolve(Bind(c, dict(zip(names, x))))
+    R = np.outer(psi, psi.conj()).reshape(-1)
+    Hv = _h_matrix(H, n)
+    vals, _ = E.dm_reverse(R, Hv, n, buf.array(), 0, len(buf), derivs, nd, tile_bits=6, low_bits=2)
+    g = np.zeros(len(names))
+    for i in range(nd):
+        g[derivs[i].param] += derivs[i].scale * 2.0 * vals[i]
+    fd = fd4(lambda v: energy(c, names, v, H), x)
+    assert np.max(np.abs(g - fd)) < 1e-9
+    zero = np.zeros(len(R), dtype=np.complex128)
+    zero[0] = 1
+    assert np.max(np.abs(R - zero)) < 1e-12
