@@ -1171,9 +1171,10 @@ __device__ __forceinline__ const T *blob_at(const T *p, uint64_t byte_off) {
     return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + byte_off);
 }
 
-// Three single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops
+// Single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops
 // (few enough VGPRs for four waves per SIMD, and code small enough for short branches; a fifth
-// wave, forced by spilling, measured slower),
+// wave, forced by spilling, measured slower), k_stage_dm / k_stage_dm_big the density-matrix stages
+// without / with them (channel superoperators as well),
 // k_stage_big the stages that hold them, k_stage_dm density-matrix stages (channel
 // superoperators as well).
 template <typename R>
@@ -1198,6 +1199,16 @@ __global__ void __launch_bounds__(512) k_stage_big(cx<R> *__restrict__ st, const
 }
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage_dm(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
+                                                  const Instr *__restrict__ prog, const R *__restrict__ pool,
+                                                  uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride,
+                                                  uint64_t blob_stride) {
+    if (act && !act[blockIdx.y]) return;  // a shot that this program does not reach
+    const uint64_t bo = blockIdx.y * blob_stride;  // state y's own program (0: one shared program)
+    stage_body<R, false, false, true>(st + blockIdx.y * shot_stride, nullptr, blob_at(hdr, bo), blob_at(prog, bo),
+                                     blob_at(pool, bo), ntiles, nullptr, nullptr);
+}
+template <typename R>
+__global__ void __launch_bounds__(512) k_stage_dm_big(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                   const Instr *__restrict__ prog, const R *__restrict__ pool,
                                                   uint64_t ntiles, const uint8_t *__restrict__ act, uint64_t shot_stride,
                                                   uint64_t blob_stride) {
@@ -1233,8 +1244,8 @@ GOQU_PAIR_KERNEL(k_stage_pair_dm_big, , true, true)
 #undef GOQU_THREE_WAVES
 #undef GOQU_PAIR_KERNEL
 
-// The eleven stage kernels take minutes each to compile.  build.py compiles this file once per
-// kernel with GOQU_STAGE_KERNEL = 1..11 (that kernel's code only), in parallel, and once without
+// The twelve stage kernels take minutes each to compile.  build.py compiles this file once per
+// kernel with GOQU_STAGE_KERNEL = 1..12 (that kernel's code only), in parallel, and once without
 // it: the host code below, which refers to the kernels as extern instantiations.
 #define GOQU_STAGE_INST(EXT, K, R)                                                                \
     EXT template __global__ void K<R>(cx<R> *__restrict__, const StageHeader *__restrict__,          \
@@ -1267,6 +1278,8 @@ GOQU_PAIR_INST(, k_stage_pair_big, double)
 GOQU_PAIR_INST(, k_stage_pair_big, float)
 #elif GOQU_STAGE_KERNEL == 11
 GOQU_PAIR_INST(, k_stage_pair_dm_big, double)
+#elif GOQU_STAGE_KERNEL == 12
+GOQU_STAGE_INST(, k_stage_dm_big, double)
 #endif
 }  // namespace goqu
 #else  // the host-code object
@@ -1281,6 +1294,7 @@ GOQU_PAIR_INST(extern, k_stage_pair_dm, double)
 GOQU_PAIR_INST(extern, k_stage_pair_big, double)
 GOQU_PAIR_INST(extern, k_stage_pair_big, float)
 GOQU_PAIR_INST(extern, k_stage_pair_dm_big, double)
+GOQU_STAGE_INST(extern, k_stage_dm_big, double)
 
 // ---------------------------------------------------------------------------------------------
 // Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS) holding the 4 lowest qubits.  On the
@@ -1403,8 +1417,9 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, cons
         const dim3 g(grid, nshots);
         if (dp.stages[i].flags & SF_DM) {
             if constexpr (sizeof(R) == 8)
-                hipLaunchKernelGGL(k_stage_dm<double>, g, dim3(threads), lds, s, static_cast<cx<double> *>(st.amps),
-                                   hdrs + i, instrs, pool, ntiles, act, stride, bstride);
+                hipLaunchKernelGGL((dp.stages[i].flags & SF_BIG) ? k_stage_dm_big<double> : k_stage_dm<double>, g,
+                                   dim3(threads), lds, s, static_cast<cx<double> *>(st.amps), hdrs + i, instrs, pool,
+                                   ntiles, act, stride, bstride);
             else
                 return fail(GOQU_ERR_UNSUPPORTED, "density-matrix stages run in complex128 only");
         } else if (dp.stages[i].flags & SF_BIG)

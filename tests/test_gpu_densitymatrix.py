@@ -230,3 +230,27 @@ def test_dm_wide_controlled_gate():
     dm.Evolve(c)
     psi = O.sv_evolve(c)
     assert np.max(np.abs(dm.DensityMatrix() - np.outer(psi, psi.conj()).reshape(-1))) <= TOL
+
+
+def test_dm_three_qubit_dense_gates():
+    """3-qubit gates act on rho exactly as dense ops on three row and three column register slots
+    (stages for the big density-matrix kernel), next to channel-free 1-2 qubit gates: rho equals
+    |psi><psi| of the statevector oracle (the density-matrix oracle, like the reference, takes no
+    3-qubit gates)."""
+    from scipy.stats import unitary_group
+
+    n = 6
+    rng = np.random.default_rng(76)
+    U = unitary_group.rvs(8, random_state=7)
+    b = NewBuilder("dm3", n)
+    for q in range(n):
+        b.U3(*rng.uniform(0, 2 * np.pi, 3), q)
+    b.Apply(G.Unitary("U", U.reshape(-1), 3), 4, 0, 2)
+    b.CNOT(1, 3).Apply(G.CCX, 0, 5, 3).Apply(G.CSWAP, 2, 1, 4)
+    b.Apply(G.Unitary("U", U.reshape(-1), 3), 1, 5, 3)
+    c = b.Build()
+    dm = D.New(n)
+    dm.Evolve(c)
+    psi = O.sv_evolve(c)
+    assert np.max(np.abs(dm.DensityMatrix() - np.outer(psi, psi.conj()).reshape(-1))) <= TOL
+    dm.Close()
