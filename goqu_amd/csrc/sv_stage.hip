@@ -67,10 +67,12 @@ __device__ __forceinline__ void global_layout(GAddr &G, const uint8_t *__restric
     const uint32_t g0 = gw[0], g1 = gw[1], g2 = gw[2], g3 = gw[3];
 #pragma unroll
     for (int s = 0; s < 4; s++) G.rq[s] = 1ull << byte_of(g0, g1, g2, g3, s);
+    // thread-id bits deposited arithmetically (a per-bit select would keep one lane mask per bit
+    // in SGPRs for the whole kernel)
     uint64_t tg = base;
 #pragma unroll
     for (int u = 0; u < 12; u++)
-        if (u < m - 4 && (tid >> u & 1)) tg |= 1ull << byte_of(g0, g1, g2, g3, 4 + u);
+        if (u < m - 4) tg |= (uint64_t)((tid >> u) & 1u) << byte_of(g0, g1, g2, g3, 4 + u);
     G.tg = tg;
 }
 template <int r>
@@ -104,7 +106,7 @@ __device__ __forceinline__ uint32_t lds_lane(const uint8_t *__restrict__ lay, in
     uint32_t tl = 0;
 #pragma unroll
     for (int u = 0; u < 12; u++)
-        if (u < m - 4 && (tid >> u & 1)) tl |= 1u << byte_of(l0, l1, l2, l3, 4 + u);
+        if (u < m - 4) tl |= ((tid >> u) & 1u) << byte_of(l0, l1, l2, l3, 4 + u);
     return swz8(tl) * 8u;
 }
 
@@ -940,11 +942,13 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
         __builtin_amdgcn_s_setprio(0);
     };
 
-    // Tiles blockIdx.x, + gridDim.x, ... (one per workgroup unless there are more than 2^20).
-    // Fetching the next tile while the ops run on the current one would need a second register set:
-    // two waves per SIMD instead of four, measured 45 % slower (the op stream has too many short
-    // dependencies to run at that occupancy).
-    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // Single-state stages: one tile per workgroup (the grid covers every tile), with no loop, so
+    // that the compiler does not hoist tile-invariant values into SGPRs it then spills (one tile
+    // is all a workgroup ever runs).  Paired stages: tiles blockIdx.x, + gridDim.x, ... (their
+    // gradient partials are sized by the grid, which stays at most 2^20).  Fetching the next tile
+    // while the ops run on the current one would need a second register set: two waves per SIMD
+    // instead of four, measured 45 % slower.
+    auto run_tile = [&](const uint64_t tile) __attribute__((always_inline)) {
         const uint64_t base = tile_base(tile);
         cx<R> a[16], b[16];  // b: lambda in paired mode, unused otherwise
         load_tile(a, b, base);
@@ -1154,6 +1158,11 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             });
             __builtin_amdgcn_s_setprio(0);
         }
+    };
+    if constexpr (PAIR) {
+        for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) run_tile(tile);
+    } else {
+        if (blockIdx.x < ntiles) run_tile(blockIdx.x);
     }
     if constexpr (PAIR) {
         __syncthreads();
@@ -1409,10 +1418,11 @@ static int run_program_t(State &st, const DeviceProgram &dp, hipStream_t s, cons
         const uint64_t ntiles = 1ull << (st.nq - m);
         const unsigned threads = 1u << (m - kRegBits);
         const size_t lds = (size_t(1) << m) * 8;  // 8-byte LDS elements
-        // one tile per workgroup (grid-stride beyond 2^20 tiles): measured faster than persistent
-        // workgroups sized to the device, whose waves stay in step and load in bursts
-        const uint64_t cap = 1ull << 20;
-        const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
+        // one tile per workgroup, the grid covers every tile (the kernels run one tile each):
+        // measured faster than persistent workgroups sized to the device, whose waves stay in
+        // step and load in bursts
+        if (ntiles > (1ull << 31) / threads) return fail(GOQU_ERR_UNSUPPORTED, "state too large for one launch");
+        const unsigned grid = (unsigned)ntiles;
         hipEvent_t ev0 = prof_begin(s);
         const dim3 g(grid, nshots);
         if (dp.stages[i].flags & SF_DM) {
