@@ -942,12 +942,10 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
         __builtin_amdgcn_s_setprio(0);
     };
 
-    // Single-state stages: one tile per workgroup (the grid covers every tile), with no loop, so
-    // that the compiler does not hoist tile-invariant values into SGPRs it then spills (one tile
-    // is all a workgroup ever runs).  Paired stages: tiles blockIdx.x, + gridDim.x, ... (their
-    // gradient partials are sized by the grid, which stays at most 2^20).  Fetching the next tile
-    // while the ops run on the current one would need a second register set: two waves per SIMD
-    // instead of four, measured 45 % slower.
+    // One tile per workgroup (the grid covers every tile), with no loop, so that the compiler does
+    // not hoist tile-invariant values into SGPRs it then spills (one tile is all a workgroup ever
+    // runs).  Fetching the next tile while the ops run on the current one would need a second
+    // register set: two waves per SIMD instead of four, measured 45 % slower.
     auto run_tile = [&](const uint64_t tile) __attribute__((always_inline)) {
         const uint64_t base = tile_base(tile);
         cx<R> a[16], b[16];  // b: lambda in paired mode, unused otherwise
@@ -1159,11 +1157,7 @@ __device__ __forceinline__ void stage_body(cx<R> *__restrict__ st, cx<R> *__rest
             __builtin_amdgcn_s_setprio(0);
         }
     };
-    if constexpr (PAIR) {
-        for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) run_tile(tile);
-    } else {
-        if (blockIdx.x < ntiles) run_tile(blockIdx.x);
-    }
+    if (blockIdx.x < ntiles) run_tile(blockIdx.x);
     if constexpr (PAIR) {
         __syncthreads();
         const int nw = (blockDim.x + 63) >> 6;  // a block of fewer than 64 threads is one wave
@@ -1476,7 +1470,9 @@ static int run_pair_t(State &psi, State &lam, const Program &P, hipStream_t s, s
     const int m = P.stages.empty() ? 0 : P.stages[0].m;
     if (m > 12) return fail(GOQU_ERR_UNSUPPORTED, "paired stages need tiles of at most 2^12 amplitudes");
     const uint64_t ntiles = 1ull << (psi.nq - m);
-    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 1ull << 20);
+    // one tile per workgroup; the per-workgroup gradient partials (64 doubles each) bound the grid
+    if (ntiles > (1ull << 21)) return fail(GOQU_ERR_UNSUPPORTED, "paired stages: state too large");
+    const unsigned grid = (unsigned)ntiles;
     const int ng = P.num_grad;
     // per-stage partials (slot-major, one per workgroup), reduced after every stage into gsum;
     // grow-only per host thread (every call ends synchronised)
