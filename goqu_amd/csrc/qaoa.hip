@@ -43,6 +43,7 @@ struct QInst {
     uint64_t psi, lam;                  // element offsets into the amplitude pool (large)
     uint64_t cut;                       // byte offset of cut(z), z = 0..2^n-1, in the cut pool (large)
     int64_t part;                       // offset into the partials (large)
+    int64_t tab;                        // offset into the layer tables (k_qaoa_tables)
     int32_t ntiles, small;
 };
 
@@ -107,17 +108,37 @@ __device__ __forceinline__ double block_sum1(double v, double *sh) {
 }
 
 // phase table: tab[c] = exp(-i gamma/2 (|E| - 2c)), c = 0..|E|
-__device__ __forceinline__ void build_tab(c2d *tab, int ne, double gamma) {
-    for (int c = threadIdx.x; c <= ne; c += blockDim.x) {
+// The layer tables of every instance, computed once per batch: for layer k at I.tab + k (ne + 2),
+// the cost phases exp(-i gamma_k / 2 (ne - 2c)) for c = 0..ne, then (cos beta_k, sin beta_k).  The
+// tile kernels copy a layer's phases into LDS instead of evaluating ne + 2 sines and cosines per
+// workgroup.
+__global__ void k_qaoa_tables(const QInst *__restrict__ insts, c2d *__restrict__ tabs) {
+    const QInst &I = insts[blockIdx.x];
+    const int ne = I.nedges, T = ne + 2;
+    for (int e = threadIdx.x; e < I.p * T; e += blockDim.x) {
+        const int k = e / T, c = e % T;
         double sn, cs;
-        sincos(-0.5 * gamma * (double)(ne - 2 * c), &sn, &cs);
-        tab[c] = {cs, sn};
+        if (c <= ne) sincos(-0.5 * I.gamma[k] * (double)(ne - 2 * c), &sn, &cs);
+        else sincos(I.beta[k], &sn, &cs);
+        tabs[I.tab + e] = {cs, sn};
     }
+}
+// layer k's cost phases into LDS
+__device__ __forceinline__ void load_tab(c2d *tab, const QInst &I, const c2d *__restrict__ tabs, int k) {
+    const c2d *src = tabs + I.tab + (int64_t)k * (I.nedges + 2);
+    for (int c = threadIdx.x; c <= I.nedges; c += blockDim.x) tab[c] = src[c];
     __syncthreads();
+}
+// (cos beta_k, sin beta_k) of layer k
+__device__ __forceinline__ void beta_cs(const QInst &I, const c2d *__restrict__ tabs, int k, double &cb, double &sb) {
+    const c2d v = tabs[I.tab + (int64_t)k * (I.nedges + 2) + I.nedges + 1];
+    cb = v.x;
+    sb = v.y;
 }
 
 // ---- small instances: everything in LDS ---------------------------------------------------------
 __global__ void __launch_bounds__(kQThreads) k_qaoa_small(const QInst *__restrict__ insts,
+                                                          const c2d *__restrict__ tabs,
                                                           const int32_t *__restrict__ ids,
                                                           double *__restrict__ energies,
                                                           const int64_t *__restrict__ goff,
@@ -139,11 +160,11 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_small(const QInst *__restric
     for (uint32_t z = threadIdx.x; z < N; z += blockDim.x) P[z] = {a, 0.0};
     __syncthreads();
     for (int k = 0; k < p; k++) {
-        build_tab(tab, ne, I.gamma[k]);
+        load_tab(tab, I, tabs, k);
         for (uint32_t z = threadIdx.x; z < N; z += blockDim.x) P[z] = cmul(P[z], tab[cut_of(sup, n, z)]);
         __syncthreads();
         double sb, cb;
-        sincos(I.beta[k], &sb, &cb);
+        beta_cs(I, tabs, k, cb, sb);
         for (int q = 0; q < n; q++) rx_pass(P, n, q, cb, sb);
     }
     // energy and lambda = H psi (H diagonal: -cut)
@@ -160,7 +181,7 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_small(const QInst *__restric
     double *g = grads + goff[id];
     for (int k = p - 1; k >= 0; k--) {
         double sb, cb;
-        sincos(I.beta[k], &sb, &cb);
+        beta_cs(I, tabs, k, cb, sb);
         double wy = 0;
         for (int q = 0; q < n; q++) wy += x_overlap(P, L, n, q).y;
         wy = block_sum1(wy, sh);
@@ -168,7 +189,7 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_small(const QInst *__restric
             rx_pass(P, n, q, cb, -sb);
             rx_pass(L, n, q, cb, -sb);
         }
-        build_tab(tab, ne, I.gamma[k]);
+        load_tab(tab, I, tabs, k);
         double gy = 0;
         for (uint32_t z = threadIdx.x; z < N; z += blockDim.x) {
             const int c = cut_of(sup, n, z);
@@ -304,6 +325,7 @@ __device__ __forceinline__ void relayout_states(c2d (&P)[16], c2d (&L)[16], doub
 // the forward kernel keeps half the registers and runs more waves.
 template <bool REV>
 __global__ void __launch_bounds__(kQThreads) k_qaoa_phase(const QInst *__restrict__ insts,
+                                                          const c2d *__restrict__ tabs,
                                                           const QWork *__restrict__ work, int nwork,
                                                           c2d *__restrict__ pool, uint8_t *__restrict__ cuts,
                                                           double *__restrict__ parts) {
@@ -367,9 +389,9 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_phase(const QInst *__restric
         }
     }
     double sb, cb;
-    sincos(I.beta[k], &sb, &cb);
+    beta_cs(I, tabs, k, cb, sb);
     if (!rev && isA) {  // cost layer: exp(-i gamma/2 S(z)); layer 0 computes and keeps cut(z)
-        build_tab(tab, ne, I.gamma[k]);
+        load_tab(tab, I, tabs, k);
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             int c;
@@ -402,7 +424,7 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_phase(const QInst *__restric
         wy = block_sum1(wy, sh);
         if (threadIdx.x == 0) part[1 + 3 * k + (isA ? 1 : 2)] = wy;
         if (isA) {  // cost derivative and uncompute (all mixer qubits of the layer are undone now)
-            build_tab(tab, ne, I.gamma[k]);
+            load_tab(tab, I, tabs, k);
             double gy = 0;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
@@ -474,7 +496,7 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     std::vector<QInst> insts(num_instances);
     std::vector<int32_t> small_ids, large_ids;
     uint64_t pool_elems = 0, cut_bytes = 0;
-    int64_t parts = 0;
+    int64_t parts = 0, tab_elems = 0;
     for (int i = 0; i < num_instances; i++) {
         QInst &I = insts[i];
         std::memset(&I, 0, sizeof(I));
@@ -501,6 +523,8 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
                 return fail(GOQU_ERR_INVALID_ARG, "instance " + std::to_string(i) + ": repeated edge");
             I.up[a] |= 1u << b;
         }
+        I.tab = tab_elems;
+        tab_elems += (int64_t)I.p * (I.nedges + 2);
         I.small = I.n <= kQTileBits;
         if (I.small) {
             small_ids.push_back(i);
@@ -548,7 +572,7 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
         }
     }
     // device buffers
-    DevBuf d_inst, d_small, d_large, d_goff, d_e, d_g, d_pool, d_parts, d_work, d_cut;
+    DevBuf d_inst, d_small, d_large, d_goff, d_e, d_g, d_pool, d_parts, d_work, d_cut, d_tabs;
     const size_t ng = (size_t)(goff[num_instances]);
     size_t wtotal = 0;
     std::vector<size_t> woff(launches.size());
@@ -563,6 +587,7 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     GOQU_HIP(dmalloc((void **)&d_e.p, num_instances * 8));
     GOQU_HIP(dmalloc((void **)&d_g.p, std::max<size_t>(1, ng) * 8));
     GOQU_HIP(dmalloc((void **)&d_work.p, std::max<size_t>(1, wtotal) * sizeof(QWork)));
+    GOQU_HIP(dmalloc((void **)&d_tabs.p, (size_t)std::max<int64_t>(1, tab_elems) * sizeof(c2d)));
     if (pool_elems) {
         hipError_t e = dmalloc((void **)&d_pool.p, pool_elems * sizeof(c2d));
         if (e != hipSuccess) {
@@ -602,9 +627,12 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     GOQU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const c2d *tabs = static_cast<const c2d *>(d_tabs.p);
+    hipLaunchKernelGGL(k_qaoa_tables, dim3((unsigned)num_instances), dim3(128), 0, s,
+                       static_cast<const QInst *>(d_inst.p), static_cast<c2d *>(d_tabs.p));
     if (!small_ids.empty())
         hipLaunchKernelGGL(k_qaoa_small, dim3((unsigned)small_ids.size()), dim3(kQThreads), lds_small, s,
-                           static_cast<const QInst *>(d_inst.p), static_cast<const int32_t *>(d_small.p),
+                           static_cast<const QInst *>(d_inst.p), tabs, static_cast<const int32_t *>(d_small.p),
                            static_cast<double *>(d_e.p), static_cast<const int64_t *>(d_goff.p),
                            static_cast<double *>(d_g.p));
     for (size_t ph = 0; ph < launches.size(); ph++) {
@@ -613,12 +641,12 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
         const unsigned blocks = (unsigned)(v.back().first_block + insts[v.back().inst].ntiles);
         if (ph % 2)
             hipLaunchKernelGGL(k_qaoa_phase<true>, dim3(blocks), dim3(kQThreads), 0, s,
-                               static_cast<const QInst *>(d_inst.p), static_cast<const QWork *>(d_work.p) + woff[ph],
+                               static_cast<const QInst *>(d_inst.p), tabs, static_cast<const QWork *>(d_work.p) + woff[ph],
                                (int)v.size(), static_cast<c2d *>(d_pool.p), static_cast<uint8_t *>(d_cut.p),
                                static_cast<double *>(d_parts.p));
         else
             hipLaunchKernelGGL(k_qaoa_phase<false>, dim3(blocks), dim3(kQThreads), 0, s,
-                               static_cast<const QInst *>(d_inst.p), static_cast<const QWork *>(d_work.p) + woff[ph],
+                               static_cast<const QInst *>(d_inst.p), tabs, static_cast<const QWork *>(d_work.p) + woff[ph],
                                (int)v.size(), static_cast<c2d *>(d_pool.p), static_cast<uint8_t *>(d_cut.p),
                                static_cast<double *>(d_parts.p));
     }
