@@ -51,6 +51,18 @@ def measured_traffic(n: int, layers: int):
     return None
 
 
+def section_traffic(profile: str, kernel: str):
+    """HBM bytes per launch of `kernel` from a committed profile summary under profiles/ (the
+    calibrated FETCH_SIZE / WRITE_SIZE measurement of the same section workload), else None."""
+    f = ROOT / "profiles" / profile
+    key = f"traffic_bytes_per_launch[{kernel}]:"
+    if f.exists():
+        for line in f.read_text().splitlines():
+            if line.startswith(key):
+                return int(line.split(":")[1])
+    return None
+
+
 def gate_bytes(op, n: int) -> float:
     """Algorithmic bytes of one gate applied on its own (complex128)."""
     full = 32.0 * (1 << n)
@@ -146,8 +158,9 @@ def _gpu_rate(circ, n: int) -> float:
     return sum(gate_bytes(op, n) for op in circ.Ops() if op.Gate is not None) / dt / 1e9
 
 
-def _roof(kernel: str, bin_id: int, bound: str = "hbm"):
-    """Roofline object of one profiled kernel over everything recorded since the last reset."""
+def _roof(kernel: str, bin_id: int, bound: str = "hbm", profile: str = None):
+    """Roofline object of one profiled kernel over everything recorded since the last reset;
+    `traffic` from the committed profile summary of the section (`profile`), when it has one."""
     from goqu_amd import _native as N
 
     ms, launches, amount = N.profile_read_kernel(bin_id)
@@ -161,7 +174,7 @@ def _roof(kernel: str, bin_id: int, bound: str = "hbm"):
         peak, unit = HBM_PEAK_GBPS, "GB/s"
     return {"kernel": kernel, "bound": bound, "achieved": round(achieved, 1), "peak": peak, "unit": unit,
             "frac": round(achieved / peak, 4), "launches": launches, "avg_launch_ms": round(ms / launches, 4),
-            "traffic": None}
+            "traffic": section_traffic(profile, kernel) if profile else None}
 
 
 def adjoint_section(qubits: int, reps: int, terms: int, ps_params: int, seed: int = 3) -> dict:
@@ -198,9 +211,10 @@ def adjoint_section(qubits: int, reps: int, terms: int, ps_params: int, seed: in
     t0 = time.perf_counter()
     e, g = adj.value_and_grad(x)
     t_adj = time.perf_counter() - t0
-    roofs = [r for r in (_roof("k_stage_pair<double>", N.PROF_STAGE_PAIR),
-                         _roof("k_pauli_apply_tile<double>", N.PROF_HAMILTONIAN),
-                         _roof("k_stage<double>", N.PROF_STAGE)) if r]
+    prof = "adjoint_efficientsu2_24q_c128.md" if (qubits, reps, terms) == (24, 7, 200) else None
+    roofs = [r for r in (_roof("k_stage_pair<double>", N.PROF_STAGE_PAIR, profile=prof),
+                         _roof("k_pauli_apply_tile<double>", N.PROF_HAMILTONIAN, profile=prof),
+                         _roof("k_stage<double>", N.PROF_STAGE, profile=prof)) if r]
     N.profile(False)
     f = GR.CostFunc(c, H, names, sim=sim)
     f(x)
@@ -332,7 +346,8 @@ def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
     e = dm.ExpectPauliSum(H)
     t_fwd = time.perf_counter() - t0
     kms, launches, _ = N.profile_read()
-    roof = _roof("k_stage_dm<double>", N.PROF_STAGE_DM)
+    prof = "k_stage_dm_densitymatrix_14q.md" if (n, layers) == (14, 20) else None
+    roof = _roof("k_stage_dm<double>", N.PROF_STAGE_DM, profile=prof)
     N.profile(False)
     dm.Close()
     sweep = 32.0 * (1 << (2 * n))
@@ -346,8 +361,8 @@ def dm_section(n: int = 14, layers: int = 20, seed: int = 9) -> dict:
     t0 = time.perf_counter()
     e2, grad = g.value_and_grad(x)
     out["gradient_s"] = round(time.perf_counter() - t0, 3)
-    out["gradient_roofline"] = [r for r in (_roof("k_stage_pair_dm<double>", N.PROF_STAGE_PAIR),
-                                            _roof("k_stage_dm<double>", N.PROF_STAGE_DM)) if r]
+    out["gradient_roofline"] = [r for r in (_roof("k_stage_pair_dm<double>", N.PROF_STAGE_PAIR, profile=prof),
+                                            _roof("k_stage_dm<double>", N.PROF_STAGE_DM, profile=prof)) if r]
     N.profile(False)
     out["gradient_params"] = len(names)
     out["gradient_energy_matches_forward"] = bool(abs(e2 - e) < 1e-9)
