@@ -40,13 +40,15 @@ HBM_PEAK_GBPS = 8000.0
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_stage_statevector_30q_c128.md"
 
 
-def measured_traffic(n: int, layers: int):
+def measured_traffic(n: int, layers: int, tile_bits: int):
     """HBM bytes per k_stage launch from the committed FETCH_SIZE / WRITE_SIZE summary (calibrated
-    against sweeps of known size), for the workload that summary covers; None otherwise."""
+    against sweeps of known size), for the workload and tile shape that summary covers; None
+    otherwise.  The 2^12-tile figure is the plain line, other tile sizes carry [tile_bits]."""
     if (n, layers) != (30, 40) or not TRAFFIC_PROFILE.exists():
         return None
+    key = "traffic_bytes_per_launch:" if tile_bits == 12 else f"traffic_bytes_per_launch[{tile_bits}]:"
     for line in TRAFFIC_PROFILE.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key):
             return int(line.split(":")[1])
     return None
 
@@ -539,6 +541,7 @@ def main() -> int:
     value = sweeps_per_step * sweep_bytes / (ms_step / 1e3) / 1e9
     gate_equiv = sum(gate_bytes(op, n) for op in circ.Ops() if op.Gate is not None) / (ms_step / 1e3) / 1e9
     avg_launch_ms = kms / max(launches, 1)
+    tile_bits = plan[0] or int(os.environ.get("GOQU_TILE_BITS", "12"))  # (0, 0): the library default
     achieved = sweep_bytes / (avg_launch_ms / 1e3) / 1e9
     out = {
         "metric": "30q c128 gate-apply GB/s vs HBM peak; adjoint grads/s vs param-shift",
@@ -559,7 +562,7 @@ def main() -> int:
                    "tile_shape": {"tile_bits": plan[0], "low_bits": plan[1], "autotuned": not args.no_tune,
                                   "candidates_s": {k: round(v, 4) for k, v in tune_times.items()}}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(n, args.layers),
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(n, args.layers, tile_bits),
                      "kernel": "k_stage<double>", "avg_launch_ms": round(avg_launch_ms, 4)},
         "cpu_baseline": None,
     }

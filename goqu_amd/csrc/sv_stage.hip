@@ -1301,8 +1301,8 @@ GOQU_STAGE_INST(extern, k_stage_dm_big, double)
 
 // ---------------------------------------------------------------------------------------------
 // Tiles of 2^12 amplitudes (256 threads, 32 KiB of LDS) holding the 4 lowest qubits.  On the
-// 30-qubit depth-40 circuit: 61 sweeps of 7.85 ms (485 ms per Evolve) against 58 of 8.37 ms (492 ms)
-// with the 3 lowest qubits; 2^13 tiles cut the sweeps to 50-53 but each takes 9.1-9.5 ms.
+// 30-qubit depth-40 circuit: 61 sweeps of 6.89 ms; 2^13 tiles cut the sweeps to 53 but each takes
+// 7.46 ms, which the autotuner weighs per circuit.
 // GOQU_TILE_BITS (8..13) and GOQU_LOW_BITS (2..6) override the default, goqu_sv_set_plan sets a
 // handle's shape (the autotuner's choice).  Paired programs (adjoint reverse sweeps, two states
 // per tile) stay at most 2^12: their kernel holds twice the amplitudes.
