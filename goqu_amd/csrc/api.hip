@@ -10,6 +10,8 @@
 
 #include <cstdio>
 
+#include <mutex>
+
 #include "runtime.h"
 
 namespace goqu {
@@ -77,7 +79,83 @@ hipError_t dmalloc(void **p, size_t bytes) {
         *p = nullptr;
         return hipErrorOutOfMemory;
     }
-    return hipMalloc(p, bytes);
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        cache_release_all();
+        e = hipMalloc(p, bytes);
+    }
+    return e;
+}
+
+namespace {
+constexpr int kCacheDevices = 64;
+struct Kept {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_cache_mu;
+Kept g_kept[kCacheDevices][CACHE_KINDS];
+
+int cache_device() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kCacheDevices) return -1;
+    return dev;
+}
+}  // namespace
+
+hipError_t cache_take(int kind, void **p, size_t bytes, size_t *got) {
+    *p = nullptr;
+    *got = 0;
+    const int dev = cache_device();
+    Kept old;
+    if (dev >= 0 && kind >= 0 && kind < CACHE_KINDS) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        Kept &k = g_kept[dev][kind];
+        if (k.p && k.bytes >= bytes) {
+            *p = k.p;
+            *got = k.bytes;
+            k = Kept();
+            return hipSuccess;
+        }
+        old = k;
+        k = Kept();
+    }
+    if (old.p) (void)hipFree(old.p);
+    const hipError_t e = dmalloc(p, bytes);
+    if (e == hipSuccess) *got = bytes;
+    return e;
+}
+
+void cache_give(int kind, void *p, size_t bytes) noexcept {
+    if (!p) return;
+    const int dev = cache_device();
+    void *drop = p;
+    if (dev >= 0 && kind >= 0 && kind < CACHE_KINDS) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        Kept &k = g_kept[dev][kind];
+        if (!k.p || k.bytes < bytes) {  // keep the larger of the two
+            drop = k.p;
+            k.p = p;
+            k.bytes = bytes;
+        }
+    }
+    if (drop) (void)hipFree(drop);
+}
+
+void cache_release_all() noexcept {
+    const int dev = cache_device();
+    if (dev < 0) return;
+    Kept all[CACHE_KINDS];
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (int k = 0; k < CACHE_KINDS; k++) {
+            all[k] = g_kept[dev][k];
+            g_kept[dev][k] = Kept();
+        }
+    }
+    for (auto &k : all)
+        if (k.p) (void)hipFree(k.p);
 }
 
 static int check_device() {

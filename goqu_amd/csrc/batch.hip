@@ -101,22 +101,20 @@ class Batcher {
   public:
     Batcher(const BatchJob &job, hipStream_t s) : J(job), s(s) {}
     ~Batcher() {
-        for (void *p : {(void *)states, (void *)blobs, (void *)dsums, (void *)dz, (void *)dprobs, (void *)dtmp})
+        for (void *p : {(void *)dsums, (void *)dz, (void *)dprobs, (void *)dtmp})
             if (p) (void)hipFree(p);
     }
 
+    // called once, before any launch: the states and the program blobs come from the retained
+    // buffers of earlier batches when those are large enough
     int prepare(int max_states, size_t blob_bytes) {
         const size_t sb = (size_t(1) << J.nq) * (J.prec == GOQU_C128 ? 16 : 8);
-        if (max_states > cap_states || blob_bytes * max_states > cap_blob) {
-            if (states) GOQU_HIP(hipFree(states));
-            if (blobs) GOQU_HIP(hipFree(blobs));
-            states = nullptr;
-            blobs = nullptr;
-            cap_states = std::max(cap_states, max_states);
-            cap_blob = std::max(cap_blob, blob_bytes * max_states);
-            GOQU_HIP(dmalloc(&states, sb * cap_states));
-            GOQU_HIP(dmalloc(&blobs, cap_blob));
-        }
+        cap_states = max_states;
+        cap_blob = blob_bytes * max_states;
+        GOQU_HIP(cstates.take(sb * cap_states));
+        GOQU_HIP(cblobs.take(cap_blob));
+        states = cstates.p;
+        blobs = cblobs.p;
         // reduction partials of the batched Pauli kernels, at their largest (no reallocation
         // while kernels are queued)
         int rc = scr.ensure((size_t)std::max(4096, chunk_cap) * 32 * 8 + 4096);
@@ -221,7 +219,13 @@ class Batcher {
     const BatchJob &J;
     hipStream_t s;
     Scratch scr;
+    CachedDevBuf cstates{CACHE_BATCH_STATES}, cblobs{CACHE_BATCH_BLOBS};
     void *states = nullptr, *blobs = nullptr;
+    // every launch of the batch has completed (its stream synchronized)
+    void idle() {
+        cstates.idle();
+        cblobs.idle();
+    }
     int cap_states = 0;
     size_t cap_blob = 0;
     int chunk_cap = 0;
@@ -364,6 +368,8 @@ int run_batch(const BatchJob &J, int ninst, const goqu_op_t *ops, const int64_t 
         if (rc) return rc;
         GOQU_HIP(hipStreamSynchronize(s));
     }
+    GOQU_HIP(hipStreamSynchronize(s));
+    B.idle();
     return GOQU_OK;
 }
 

@@ -323,13 +323,14 @@ int check_program(const goqu_dyn_instr_t *prog, int ninstr, int num_ops, const i
 
 // Device buffers of one batch of shots (sized for the largest batch of the call).
 struct DynBuffers {
+    CachedDevBuf amps_buf{CACHE_DYN_STATES};  // the shots' states: retained for the next call
     void *amps = nullptr;
     uint8_t *act = nullptr, *taken = nullptr, *clbits = nullptr;
     int *drawn = nullptr, *outcome = nullptr, *flags = nullptr, *bits = nullptr;
     double *u = nullptr, *part = nullptr, *scale = nullptr;
     ~DynBuffers() {
-        for (void *p : {amps, (void *)act, (void *)taken, (void *)clbits, (void *)drawn, (void *)outcome,
-                        (void *)flags, (void *)bits, (void *)u, (void *)part, (void *)scale})
+        for (void *p : {(void *)act, (void *)taken, (void *)clbits, (void *)drawn, (void *)outcome, (void *)flags,
+                        (void *)bits, (void *)u, (void *)part, (void *)scale})
             if (p) (void)hipFree(p);
     }
 };
@@ -521,7 +522,8 @@ int run_dynamic(bool density, int n, State &handle_state, hipStream_t s, const P
 
     DynBuffers buf;
     const int levels = P.depth + 1;
-    GOQU_HIP(dmalloc(&buf.amps, (size_t)B * stride * handle_state.elem));
+    GOQU_HIP(buf.amps_buf.take((size_t)B * stride * handle_state.elem));
+    buf.amps = buf.amps_buf.p;
     GOQU_HIP(dmalloc((void **)&buf.act, (size_t)levels * B));
     GOQU_HIP(dmalloc((void **)&buf.taken, (size_t)levels * B));
     GOQU_HIP(dmalloc((void **)&buf.clbits, (size_t)B * std::max(nclbits, 1)));
@@ -582,6 +584,7 @@ int run_dynamic(bool density, int n, State &handle_state, hipStream_t s, const P
             GOQU_HIP(hipStreamSynchronize(s));
         }
     }
+    buf.amps_buf.idle();
     return GOQU_OK;
 }
 

@@ -14,13 +14,14 @@
 // layer, so each term may be taken anywhere inside its layer's uncompute.
 //
 // Instances with n <= 12 run entirely in one workgroup's LDS (psi and lambda together).  Larger
-// instances are swept in tiles of 4096 amplitudes: phase A holds qubits 0..11, phase B the
-// qubits 0..L-1 and 12..n-1; the phases of all instances advance together, one launch per
-// phase, and the whole sequence is captured in a single hipGraph.  A tile lives in registers, 16
-// amplitudes per thread: the mixer runs on register bits, and two or three LDS relayouts bring
-// the other tile bits into registers.  cut(z) is computed once per instance (first cost layer)
-// and kept as one byte per amplitude; the energy and lambda = H psi are formed when the reverse
-// sweep first loads psi.
+// instances are swept in passes over tiles of 4096 amplitudes (A tiles: qubits 0..11; B tiles: qubits
+// 0..L-1 and 12..n-1), p + 1 passes forward and p + 1 reverse, each applying the second half of
+// one layer's mixer, the next cost layer and the first half of the next mixer; the passes of all
+// instances advance together, one launch per step, and the whole sequence is captured in a single
+// hipGraph.  A tile lives in registers, 16 amplitudes per thread: the mixer runs on register bits,
+// and LDS relayouts bring the other tile bits into registers.  cut(z) is computed once per
+// instance (first cost layer) and kept as one byte per amplitude; the energy and lambda = H psi
+// are formed when the reverse sweep first loads psi.
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -47,8 +48,6 @@ struct QInst {
     int32_t ntiles, small;
 };
 
-// phase codes for large instances
-enum : int32_t { PH_FWD_A = 0, PH_FWD_B = 1, PH_REV_B = 2, PH_REV_A = 3 };
 
 // cut(z) from the upper adjacency masks staged in LDS
 __device__ __forceinline__ int cut_of(const uint32_t *up, int n, uint32_t z) {
@@ -209,20 +208,37 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_small(const QInst *__restric
     }
 }
 
-// ---- large instances: one tile of 4096 amplitudes per workgroup, one launch per phase ----------
+// ---- large instances: one tile of 4096 amplitudes per workgroup, one launch per pass -------------
+// A pass sweeps an instance's state once, in tiles of one of two kinds: A tiles hold qubits 0..11,
+// B tiles qubits 0..L-1 and 12..n-1 (L = 24 - n; the tile index supplies the rest).  Layer k's
+// mixer is the product of its A half M_k^A (RX on qubits 0..11) and its B half M_k^B (qubits
+// 12..n-1), which commute, and each cost layer C_k is diagonal, so the forward circuit regroups into
+// p + 1 passes of alternating kinds,
+//   [C_0 M_0^A] [M_0^B C_1 M_1^B] [M_1^A C_2 M_2^A] ... [M_{p-1}^X],
+// each applying the second half of one layer's mixer, the next cost layer and the first half of
+// the next mixer; the reverse sweep runs the same passes inverted, last to first.  With one pass per
+// half layer instead, a batch moved (192 p - 16) bytes per amplitude; this way it moves 96 p + 48.
 struct QWork {
     int32_t inst;
-    int32_t phase;  // PH_*
-    int32_t layer;
-    int32_t first_block;  // prefix over the launch
+    int8_t half;               // 0: A tiles, 1: B tiles
+    int8_t pre, cost, post;    // in forward order: mixer half of layer `pre`, cost layer `cost`, mixer
+                               // half of layer `post` (-1: absent)
+    int32_t flags;             // Q_*
+    int32_t first_block;       // prefix over the launch
+};
+enum : int32_t {
+    Q_INIT = 1,     // first forward pass: psi starts as H^n |0>, cut(z) is computed and kept
+    Q_ENERGY = 2,   // first reverse pass: the energy, and lambda = H psi
+    Q_NOSTORE = 4,  // last reverse pass: psi and lambda are not needed afterwards
 };
 
-__device__ __forceinline__ uint64_t tile_global(const QInst &I, int phase, uint64_t tile, uint32_t l) {
-    if (phase == PH_FWD_A || phase == PH_REV_A) return (tile << kQTileBits) | l;
+// (an instance index: below 2^22, so 32 bits; loads and stores add it to the instance's uniform base)
+__device__ __forceinline__ uint32_t tile_global(const QInst &I, int half, uint32_t tile, uint32_t l) {
+    if (!half) return (tile << kQTileBits) | l;
     // B: local bits 0..L-1 -> qubits 0..L-1, local bits L..11 -> qubits 12..n-1; the tile index
     // supplies qubits L..11
     const int Lb = 2 * kQTileBits - I.n;
-    const uint64_t lo = l & ((1u << Lb) - 1), hi = l >> Lb;
+    const uint32_t lo = l & ((1u << Lb) - 1), hi = l >> Lb;
     return lo | (tile << Lb) | (hi << kQTileBits);
 }
 
@@ -247,15 +263,57 @@ __device__ __forceinline__ constexpr uint32_t qreg(uint32_t r) {
 // LDS element of local index l: low bits XOR bits 8-11 (spreads the layouts' strided accesses)
 __device__ __forceinline__ constexpr uint32_t qswz(uint32_t l) { return l ^ ((l >> 8) & 15u); }
 
-// RX on register bit J of a 16-amplitude tile: [a0,a1] <- [c a0 - i s a1, -i s a0 + c a1]
+// In-place register arithmetic.  Each amplitude keeps its own registers through inline asm with
+// tied operands: written as plain expressions, the scheduler computed all 16 (or 32) results into
+// fresh registers and copied them back at every branch merge, and the reverse kernel needed over
+// 256 VGPRs (one wave per SIMD).
+//
+// RX: [a0, a1] <- [c a0 - i s a1, -i s a0 + c a1], which is two real rotations, of (x0, y1) and of
+// (y0, x1)
+__device__ __forceinline__ void rx_pair(c2d &a0, c2d &a1, double c, double s) {
+    double t0, t1, t2, t3;
+    asm volatile(
+        "v_mul_f64 %[t0], %[c], %[x0]\n\t"
+        "v_mul_f64 %[t1], %[s], %[x0]\n\t"
+        "v_mul_f64 %[t2], %[c], %[y0]\n\t"
+        "v_mul_f64 %[t3], %[s], %[y0]\n\t"
+        "v_fma_f64 %[x0], %[s], %[y1], %[t0]\n\t"
+        "v_fma_f64 %[y1], %[c], %[y1], -%[t1]\n\t"
+        "v_fma_f64 %[y0], -%[s], %[x1], %[t2]\n\t"
+        "v_fma_f64 %[x1], %[c], %[x1], %[t3]"
+        : [x0] "+v"(a0.x), [y0] "+v"(a0.y), [x1] "+v"(a1.x), [y1] "+v"(a1.y), [t0] "=&v"(t0),
+          [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
+        : [c] "v"(c), [s] "v"(s));
+}
+// a <- a * (tc + i ts), or a * (tc - i ts) when CONJ
+template <bool CONJ>
+__device__ __forceinline__ void cmul_in(c2d &a, double tc, double ts) {
+    double t0, t1;
+    if constexpr (CONJ)
+        asm volatile(
+            "v_mul_f64 %[t0], %[ts], %[y]\n\t"
+            "v_mul_f64 %[t1], %[ts], %[x]\n\t"
+            "v_fma_f64 %[x], %[tc], %[x], %[t0]\n\t"
+            "v_fma_f64 %[y], %[tc], %[y], -%[t1]"
+            : [x] "+v"(a.x), [y] "+v"(a.y), [t0] "=&v"(t0), [t1] "=&v"(t1)
+            : [tc] "v"(tc), [ts] "v"(ts));
+    else
+        asm volatile(
+            "v_mul_f64 %[t0], %[ts], %[y]\n\t"
+            "v_mul_f64 %[t1], %[ts], %[x]\n\t"
+            "v_fma_f64 %[x], %[tc], %[x], -%[t0]\n\t"
+            "v_fma_f64 %[y], %[tc], %[y], %[t1]"
+            : [x] "+v"(a.x), [y] "+v"(a.y), [t0] "=&v"(t0), [t1] "=&v"(t1)
+            : [tc] "v"(tc), [ts] "v"(ts));
+}
+
+// RX on register bit J of a 16-amplitude tile
 template <int J>
 __device__ __forceinline__ void rx_reg(c2d (&A)[16], double c, double s) {
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         if (r & (1 << J)) continue;
-        const c2d a0 = A[r], a1 = A[r | (1 << J)];
-        A[r] = {c * a0.x + s * a1.y, c * a0.y - s * a1.x};
-        A[r | (1 << J)] = {c * a1.x + s * a0.y, c * a1.y - s * a0.x};
+        rx_pair(A[r], A[r | (1 << J)], c, s);
     }
 }
 // Im <lambda| X_J |psi> over the register pairs of bit J
@@ -266,7 +324,16 @@ __device__ __forceinline__ double xov_reg(const c2d (&P)[16], const c2d (&L)[16]
     for (int r = 0; r < 16; r++) {
         if (r & (1 << J)) continue;
         const c2d p0 = P[r], p1 = P[r | (1 << J)], l0 = L[r], l1 = L[r | (1 << J)];
-        w += l0.x * p1.y - l0.y * p1.x + l1.x * p0.y - l1.y * p0.x;
+        // one fused chain, ordered before the in-place rotations of the same registers (free to
+        // move, it would need copies of the amplitudes the rotations overwrite)
+        asm volatile(
+            "v_fma_f64 %[w], %[l0x], %[p1y], %[w]\n\t"
+            "v_fma_f64 %[w], -%[l0y], %[p1x], %[w]\n\t"
+            "v_fma_f64 %[w], %[l1x], %[p0y], %[w]\n\t"
+            "v_fma_f64 %[w], -%[l1y], %[p0x], %[w]"
+            : [w] "+v"(w)
+            : [l0x] "v"(l0.x), [l0y] "v"(l0.y), [l1x] "v"(l1.x), [l1y] "v"(l1.y), [p0x] "v"(p0.x),
+              [p0y] "v"(p0.y), [p1x] "v"(p1.x), [p1y] "v"(p1.y));
     }
     return w;
 }
@@ -274,15 +341,23 @@ __device__ __forceinline__ double xov_reg(const c2d (&P)[16], const c2d (&L)[16]
 // moves a register tile from layout FROM to layout TO through LDS (real parts, then imaginary)
 template <int FROM, int TO>
 __device__ __forceinline__ void relayout(c2d (&A)[16], double *buf, uint32_t lane, uint32_t wave) {
-    const uint32_t wt = qswz(qthread<FROM>(lane, wave)), rt = qswz(qthread<TO>(lane, wave));
+    // byte offsets into buf (8-byte elements)
+    const uint32_t wt0 = qswz(qthread<FROM>(lane, wave)) << 3, rt0 = qswz(qthread<TO>(lane, wave)) << 3;
+    unsigned char *const bb = reinterpret_cast<unsigned char *>(buf);
 #pragma unroll
     for (int part = 0; part < 2; part++) {
+        // opaque copies per pass: the 16 addresses of a pass are recomputed (one XOR each), not
+        // held in registers across passes and states
+        uint32_t wt, rt;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(wt) : "v"(wt0));
 #pragma unroll
-        for (int r = 0; r < 16; r++) buf[wt ^ qswz(qreg<FROM>(r))] = part ? A[r].y : A[r].x;
+        for (int r = 0; r < 16; r++)
+            *reinterpret_cast<double *>(bb + (wt ^ (qswz(qreg<FROM>(r)) << 3))) = part ? A[r].y : A[r].x;
         __syncthreads();
+        asm volatile("v_mov_b32 %0, %1" : "=v"(rt) : "v"(rt0));
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const double v = buf[rt ^ qswz(qreg<TO>(r))];
+            const double v = *reinterpret_cast<const double *>(bb + (rt ^ (qswz(qreg<TO>(r)) << 3)));
             if (part)
                 A[r].y = v;
             else
@@ -321,17 +396,49 @@ __device__ __forceinline__ void relayout_states(c2d (&P)[16], c2d (&L)[16], doub
     if constexpr (REV) relayout<FROM, TO>(L, buf, lane, wave);
 }
 
-// REV: the launch holds only reverse phases (psi and lambda), otherwise only forward phases (psi):
-// the forward kernel keeps half the registers and runs more waves.
+// Sum of v[0..3] over the workgroup, in a fixed order; the totals are valid in thread 0
+__device__ __forceinline__ void block_sum4(double (&v)[4], double *sh) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) sh[(threadIdx.x >> 6) * 4 + i] = v[i];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            double t = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[w * 4 + i];
+            v[i] = t;
+        }
+}
+
+// gy += S Im(conj(l) p), ordered before the in-place updates of the same registers
+__device__ __forceinline__ void cost_overlap(double &gy, double S, const c2d &l, const c2d &p) {
+    double t;
+    asm volatile(
+        "v_mul_f64 %[t], %[S], %[lx]\n\t"
+        "v_fma_f64 %[g], %[t], %[py], %[g]\n\t"
+        "v_mul_f64 %[t], %[S], %[ly]\n\t"
+        "v_fma_f64 %[g], -%[t], %[px], %[g]"
+        : [g] "+v"(gy), [t] "=&v"(t)
+        : [S] "v"(S), [lx] "v"(l.x), [ly] "v"(l.y), [px] "v"(p.x), [py] "v"(p.y));
+}
+
+// One pass of every work item of the launch (REV: the launch holds only reverse passes, over psi and
+// lambda; otherwise only forward passes, over psi: the forward kernel keeps half the registers and
+// runs more waves).
 template <bool REV>
-__global__ void __launch_bounds__(kQThreads) k_qaoa_phase(const QInst *__restrict__ insts,
-                                                          const c2d *__restrict__ tabs,
-                                                          const QWork *__restrict__ work, int nwork,
-                                                          c2d *__restrict__ pool, uint8_t *__restrict__ cuts,
-                                                          double *__restrict__ parts) {
+__global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict__ insts,
+                                                         const c2d *__restrict__ tabs,
+                                                         const QWork *__restrict__ work, int nwork,
+                                                         c2d *__restrict__ pool, uint8_t *__restrict__ cuts,
+                                                         double *__restrict__ parts) {
     __shared__ double buf[1 << kQTileBits];
     __shared__ c2d tab[kQMaxEdges + 1];
-    __shared__ double sh[16];
+    __shared__ double sh[4 * (kQThreads / 64)];
     __shared__ uint32_t sup[kQMaxQubits];
     // which work item: last with first_block <= blockIdx.x
     int lo = 0, hi = nwork;
@@ -342,108 +449,207 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_phase(const QInst *__restric
     }
     const QWork W = work[lo];
     const QInst &I = insts[W.inst];
-    const uint64_t tile = blockIdx.x - W.first_block;
-    const int n = I.n, ne = I.nedges, k = W.layer, ph = W.phase;
+    const uint32_t tile = blockIdx.x - W.first_block;
+    const int n = I.n, ne = I.nedges, half = W.half, kpre = W.pre, kcost = W.cost, kpost = W.post;
+    const int flags = W.flags;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     c2d *psi = pool + I.psi, *lam = pool + I.lam;
     uint8_t *cut = cuts + I.cut;
     double *part = parts + I.part + (int64_t)tile * I.slot_stride;
-    stage_up(sup, I);
-    constexpr bool rev = REV;
-    const bool isA = ph == PH_FWD_A || ph == PH_REV_A;
-    const int Lb = 2 * kQTileBits - n;
-    const int b0 = isA ? 0 : Lb;  // local bits b0..11 carry this phase's mixer qubits
+    // the first forward pass computes cut(z); adj/deg: neighbours and degree of vertices 8..11
+    uint32_t adj[4] = {0, 0, 0, 0};
+    int deg[4] = {0, 0, 0, 0};
+    if (!REV && (flags & Q_INIT)) {
+        stage_up(sup, I);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int v = 8 + j;
+            uint32_t a = v < n ? I.up[v] : 0u;
+            for (int i = 0; i < v && i < n; i++) a |= ((I.up[i] >> v) & 1u) << i;
+            adj[j] = a;
+            deg[j] = __popc(a);
+        }
+    }
+    const int b0 = half ? 2 * kQTileBits - n : 0;  // local bits b0..11 carry this pass's mixer qubits
     c2d P[16], L[16];
-    // global index of register r in layout 0: the tile mapping deposits bits, so it splits into
-    // the thread's part and the register index's part
-    const uint64_t g0 = tile_global(I, ph, tile, qthread<0>(lane, wave));
-    const uint64_t d0 = tile_global(I, ph, 0, 1u << 8), d1 = tile_global(I, ph, 0, 2u << 8);
-    const uint64_t d2 = tile_global(I, ph, 0, 4u << 8), d3 = tile_global(I, ph, 0, 8u << 8);
-    auto gidx = [&](int r) __attribute__((always_inline)) {
-        return g0 | ((r & 1) ? d0 : 0) | ((r & 2) ? d1 : 0) | ((r & 4) ? d2 : 0) | ((r & 8) ? d3 : 0);
+
+    // Instance index of register r in layout LAY: the tile mapping deposits bits, so it is the
+    // thread's part OR the register index's part (four uniform bit deposits).  Each group of
+    // loads or stores derives its 16 indices from its own opaque copy of the thread's part, so they
+    // are recomputed (one OR each) instead of being held in registers across the kernel.
+    struct Idx {
+        uint32_t t, d0, d1, d2, d3;
     };
-    // load (layer 0 starts from H^n |0>: every amplitude (1/sqrt2)^n)
-    if (ph == PH_FWD_A && k == 0) {
+    auto make_idx = [&](auto layc) __attribute__((always_inline)) {
+        constexpr int LAY = decltype(layc)::value;
+        constexpr int base = LAY == 0 ? 8 : LAY == 1 ? 0 : 4;
+        uint32_t t;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"(tile_global(I, half, tile, qthread<LAY>(lane, wave))));
+        return Idx{t, tile_global(I, half, 0, 1u << base), tile_global(I, half, 0, 2u << base),
+                   tile_global(I, half, 0, 4u << base), tile_global(I, half, 0, 8u << base)};
+    };
+    auto at = [](const Idx &x, int r) __attribute__((always_inline)) {
+        return x.t | ((r & 1) ? x.d0 : 0) | ((r & 2) ? x.d1 : 0) | ((r & 4) ? x.d2 : 0) | ((r & 8) ? x.d3 : 0);
+    };
+    // amplitude at an instance index, as a 32-bit byte offset from the state's uniform base (below
+    // 2^26), so loads and stores take one offset register each (scalar base + vector offset)
+    auto amp = [](c2d *base, uint32_t g) __attribute__((always_inline)) -> c2d & {
+        return *reinterpret_cast<c2d *>(reinterpret_cast<unsigned char *>(base) + (g << 4));
+    };
+    using L0 = std::integral_constant<int, 0>;
+    using L2 = std::integral_constant<int, 2>;
+
+    // load (the first forward pass starts from H^n |0>: every amplitude (1/sqrt2)^n)
+    double red[4] = {0, 0, 0, 0};  // energy, post-layer overlap, pre-layer overlap, cost overlap
+    if (!REV && (flags & Q_INIT)) {
         double a = 1.0;
         for (int q = 0; q < n; q++) a *= 0.70710678118654752440;
 #pragma unroll
         for (int r = 0; r < 16; r++) P[r] = {a, 0.0};
     } else {
+        const Idx ix = make_idx(L0{});
 #pragma unroll
-        for (int r = 0; r < 16; r++) P[r] = psi[gidx(r)];
+        for (int r = 0; r < 16; r++) P[r] = amp(psi, at(ix, r));
     }
     if constexpr (REV) {
-        if (ph == PH_REV_B && k == I.p - 1) {  // lambda = H psi (H = -cut) and the energy
-            double e = 0;
+        const Idx ix = make_idx(L0{});
+        if (flags & Q_ENERGY) {  // lambda = H psi (H = -cut) and the energy
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const double h = -(double)cut[gidx(r)];
-                e += h * (P[r].x * P[r].x + P[r].y * P[r].y);
+                const double h = -(double)cut[at(ix, r)];
+                red[0] += h * (P[r].x * P[r].x + P[r].y * P[r].y);
                 L[r] = {h * P[r].x, h * P[r].y};
             }
-            e = block_sum1(e, sh);
-            if (threadIdx.x == 0) part[0] = e;
         } else {
 #pragma unroll
-            for (int r = 0; r < 16; r++) L[r] = lam[gidx(r)];
+            for (int r = 0; r < 16; r++) L[r] = amp(lam, at(ix, r));
         }
     }
-    double sb, cb;
-    beta_cs(I, tabs, k, cb, sb);
-    if (!rev && isA) {  // cost layer: exp(-i gamma/2 S(z)); layer 0 computes and keeps cut(z)
+
+    // One layer's mixer half over the register groups: layout 0 holds local bits 8-11, layout 1
+    // bits 0-3, layout 2 bits 4-7.  b0 <= 3: all three groups; b0 in 4..7: layouts 0 and 2;
+    // otherwise layout 0 alone.  mix_out starts in layout 0 and ends in layout 2 (0 when b0 >= 8),
+    // mix_back runs the groups the other way round, back to layout 0 (coalesced stores).  REV: the
+    // overlaps and the uncompute of both states.
+    auto mix_out = [&](int k, double &wy) __attribute__((always_inline)) {
+        double cb, sb;
+        beta_cs(I, tabs, k, cb, sb);
+        mix_group<REV, 0>(P, L, b0, cb, sb, wy);
+        if (b0 <= 3) {
+            relayout_states<REV, 0, 1>(P, L, buf, lane, wave);
+            mix_group<REV, 1>(P, L, b0, cb, sb, wy);
+            relayout_states<REV, 1, 2>(P, L, buf, lane, wave);
+            mix_group<REV, 2>(P, L, b0, cb, sb, wy);
+        } else if (b0 <= 7) {
+            relayout_states<REV, 0, 2>(P, L, buf, lane, wave);
+            mix_group<REV, 2>(P, L, b0, cb, sb, wy);
+        }
+    };
+    auto mix_back = [&](int k, double &wy) __attribute__((always_inline)) {
+        double cb, sb;
+        beta_cs(I, tabs, k, cb, sb);
+        if (b0 <= 3) {
+            mix_group<REV, 2>(P, L, b0, cb, sb, wy);
+            relayout_states<REV, 2, 1>(P, L, buf, lane, wave);
+            mix_group<REV, 1>(P, L, b0, cb, sb, wy);
+            relayout_states<REV, 1, 0>(P, L, buf, lane, wave);
+        } else if (b0 <= 7) {
+            mix_group<REV, 2>(P, L, b0, cb, sb, wy);
+            relayout_states<REV, 2, 0>(P, L, buf, lane, wave);
+        }
+        mix_group<REV, 0>(P, L, b0, cb, sb, wy);
+    };
+    // cost layer k in layout LAY: forward exp(-i gamma/2 S(z)) (the first pass computes and keeps
+    // cut(z)); reverse the derivative Im <lambda| S |psi> and, unless nothing is stored, the
+    // uncompute of both states
+    auto cost = [&](auto layc, int k) __attribute__((always_inline)) {
         load_tab(tab, I, tabs, k);
+        const Idx ix = make_idx(layc);
+        if constexpr (!REV) {
+            int c[16];
+            if (flags & Q_INIT) {
+                // cut(z) of the 16 registers: register 0 in full, every other from the register
+                // without its lowest set bit j, by flipping vertex 8 + j (this pass runs on A tiles
+                // in layout 0, whose register bits are qubits 8..11):
+                //   cut(z ^ e_v) = cut(z) + deg(v) - 2 |{u adjacent to v : z_u != z_v}|
+                c[0] = cut_of(sup, n, at(ix, 0));
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            int c;
-            if (k == 0) {
-                c = cut_of(sup, n, (uint32_t)gidx(r));
-                cut[gidx(r)] = (uint8_t)c;
+                for (int r = 1; r < 16; r++) {
+                    const int j = __builtin_ctz(r);
+                    const uint32_t z = at(ix, r ^ (1 << j));
+                    const uint32_t zv = (z >> (8 + j)) & 1u;
+                    const int diff = __popc(adj[j] & (zv ? ~z : z));
+                    c[r] = c[r ^ (1 << j)] + deg[j] - 2 * diff;
+                }
+#pragma unroll
+                for (int r = 0; r < 16; r++) cut[at(ix, r)] = (uint8_t)c[r];
             } else {
-                c = cut[gidx(r)];
+#pragma unroll
+                for (int r = 0; r < 16; r++) c[r] = cut[at(ix, r)];
             }
-            P[r] = cmul(P[r], tab[c]);
-        }
-    }
-    // mixer bits, register group by register group: layout 0 holds local bits 8-11, layout 1
-    // bits 0-3, layout 2 bits 4-7.  b0 <= 3: all three groups; b0 in 4..7: groups of layouts 0
-    // and 2; otherwise layout 0 alone.  Each path ends back in layout 0 (coalesced stores).
-    double wy = 0;
-    mix_group<REV, 0>(P, L, b0, cb, sb, wy);
-    if (b0 <= 3) {
-        relayout_states<REV, 0, 1>(P, L, buf, lane, wave);
-        mix_group<REV, 1>(P, L, b0, cb, sb, wy);
-        relayout_states<REV, 1, 2>(P, L, buf, lane, wave);
-        mix_group<REV, 2>(P, L, b0, cb, sb, wy);
-        relayout_states<REV, 2, 0>(P, L, buf, lane, wave);
-    } else if (b0 <= 7) {
-        relayout_states<REV, 0, 2>(P, L, buf, lane, wave);
-        mix_group<REV, 2>(P, L, b0, cb, sb, wy);
-        relayout_states<REV, 2, 0>(P, L, buf, lane, wave);
-    }
-    if constexpr (REV) {
-        wy = block_sum1(wy, sh);
-        if (threadIdx.x == 0) part[1 + 3 * k + (isA ? 1 : 2)] = wy;
-        if (isA) {  // cost derivative and uncompute (all mixer qubits of the layer are undone now)
-            load_tab(tab, I, tabs, k);
-            double gy = 0;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int c = cut[gidx(r)];
-                const double S = (double)(ne - 2 * c);
-                const c2d pz = P[r], lz = L[r];
-                gy += S * (lz.x * pz.y - lz.y * pz.x);  // Im(conj(l) S p)
-                const c2d t = {tab[c].x, -tab[c].y};
-                P[r] = cmul(pz, t);
-                L[r] = cmul(lz, t);
+                const c2d t = tab[c[r]];
+                cmul_in<false>(P[r], t.x, t.y);
             }
-            gy = block_sum1(gy, sh);
-            if (threadIdx.x == 0) part[1 + 3 * k] = gy;
+        } else {
+            const bool undo = !(flags & Q_NOSTORE);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int c = cut[at(ix, r)];
+                cost_overlap(red[3], (double)(ne - 2 * c), L[r], P[r]);
+                if (undo) {
+                    const c2d t = tab[c];
+                    cmul_in<true>(P[r], t.x, t.y);
+                    cmul_in<true>(L[r], t.x, t.y);
+                }
+                // table reads four registers at a time (hoisted, all 16 would hold 64 VGPRs)
+                if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    };
+
+    // forward: mixer half `pre`, cost `cost`, mixer half `post`; reverse: the same inverted, in the
+    // opposite order
+    const int kfirst = REV ? kpost : kpre, klast = REV ? kpre : kpost;
+    bool in2 = false;  // the registers are in layout 2
+    if (kfirst >= 0) {
+        mix_out(kfirst, red[1]);
+        in2 = b0 <= 7;
+    }
+    if (kcost >= 0) {
+        if (in2) cost(L2{}, kcost);
+        else cost(L0{}, kcost);
+    }
+    if (klast >= 0) {
+        if (in2) {
+            mix_back(klast, red[2]);
+            in2 = false;
+        } else {
+            mix_out(klast, red[2]);
+            in2 = b0 <= 7;
+        }
+    }
+    const bool store = !(REV && (flags & Q_NOSTORE));
+    if (store) {
+        if (in2) relayout_states<REV, 2, 0>(P, L, buf, lane, wave);
+        const Idx ix = make_idx(L0{});
+        if constexpr (REV) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) amp(lam, at(ix, r)) = L[r];
         }
 #pragma unroll
-        for (int r = 0; r < 16; r++) lam[gidx(r)] = L[r];
+        for (int r = 0; r < 16; r++) amp(psi, at(ix, r)) = P[r];
     }
-#pragma unroll
-    for (int r = 0; r < 16; r++) psi[gidx(r)] = P[r];
+    if constexpr (REV) {
+        block_sum4(red, sh);
+        if (threadIdx.x == 0) {
+            if (flags & Q_ENERGY) part[0] = red[0];
+            if (kpost >= 0) part[1 + 3 * kpost + 1 + half] = red[1];
+            if (kpre >= 0) part[1 + 3 * kpre + 1 + half] = red[2];
+            if (kcost >= 0) part[1 + 3 * kcost] = red[3];
+        }
+    }
 }
 
 // energies / gradients of the large instances from their per-tile partials (fixed order)
@@ -470,13 +676,6 @@ __global__ void k_qaoa_finish(const QInst *__restrict__ insts, const int32_t *__
         grads[goff[id] + 2 * k + 1] = 2.0 * (ga + gb);
     }
 }
-
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-};
 
 }  // namespace goqu
 
@@ -544,35 +743,35 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     }
     std::vector<int64_t> goff(num_instances + 1);
     for (int i = 0; i <= num_instances; i++) goff[i] = param_offsets[i] - param_offsets[0];
-    // phase schedule of the large instances: forward A,B per layer, then reverse B,A per layer
-    int max_phases = 0;
-    for (int id : large_ids) max_phases = std::max(max_phases, 4 * insts[id].p);
+    // pass schedule of the large instances: forward passes f = 0..p, then the same passes inverted
+    // from f = p down to 0; every instance advances one pass per step
+    int max_steps = 0;
+    for (int id : large_ids) max_steps = std::max(max_steps, 2 * (insts[id].p + 1));
     // launches[2 s]: the forward work of step s, launches[2 s + 1]: its reverse work
-    std::vector<std::vector<QWork>> launches(2 * max_phases);
-    for (int s = 0; s < max_phases; s++) {
+    std::vector<std::vector<QWork>> launches(2 * max_steps);
+    for (int s = 0; s < max_steps; s++) {
         int blocks[2] = {0, 0};
         for (int id : large_ids) {
             const int p = insts[id].p;
-            if (s >= 4 * p) continue;
+            if (s >= 2 * (p + 1)) continue;
+            const int rv = s > p;
+            const int f = rv ? 2 * p + 1 - s : s;  // the forward pass this step runs (inverted when rv)
             QWork w;
             w.inst = id;
-            int rv = 0;
-            if (s < 2 * p) {
-                w.layer = s / 2;
-                w.phase = (s % 2) ? PH_FWD_B : PH_FWD_A;
-            } else {
-                const int r = s - 2 * p;
-                w.layer = p - 1 - r / 2;
-                w.phase = (r % 2) ? PH_REV_A : PH_REV_B;
-                rv = 1;
-            }
+            w.half = (int8_t)(f & 1);
+            w.pre = (int8_t)(f >= 1 ? f - 1 : -1);
+            w.cost = (int8_t)(f < p ? f : -1);
+            w.post = (int8_t)(f < p ? f : -1);
+            w.flags = (!rv && f == 0 ? Q_INIT : 0) | (rv && f == p ? Q_ENERGY : 0) | (rv && f == 0 ? Q_NOSTORE : 0);
             w.first_block = blocks[rv];
             blocks[rv] += insts[id].ntiles;
             launches[2 * s + rv].push_back(w);
         }
     }
-    // device buffers
-    DevBuf d_inst, d_small, d_large, d_goff, d_e, d_g, d_pool, d_parts, d_work, d_cut, d_tabs;
+    // Every device buffer of the call is one carve of a single retained allocation: the instance
+    // table, id lists, parameter offsets and work items (uploaded with one copy), then the results,
+    // the layer tables and, for the large instances, the pool (psi, lambda), the per-tile partials
+    // and the cut bytes.
     const size_t ng = (size_t)(goff[num_instances]);
     size_t wtotal = 0;
     std::vector<size_t> woff(launches.size());
@@ -580,40 +779,58 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
         woff[s] = wtotal;
         wtotal += launches[s].size();
     }
-    GOQU_HIP(dmalloc((void **)&d_inst.p, insts.size() * sizeof(QInst)));
-    GOQU_HIP(dmalloc((void **)&d_small.p, std::max<size_t>(1, small_ids.size()) * 4));
-    GOQU_HIP(dmalloc((void **)&d_large.p, std::max<size_t>(1, large_ids.size()) * 4));
-    GOQU_HIP(dmalloc((void **)&d_goff.p, goff.size() * 8));
-    GOQU_HIP(dmalloc((void **)&d_e.p, num_instances * 8));
-    GOQU_HIP(dmalloc((void **)&d_g.p, std::max<size_t>(1, ng) * 8));
-    GOQU_HIP(dmalloc((void **)&d_work.p, std::max<size_t>(1, wtotal) * sizeof(QWork)));
-    GOQU_HIP(dmalloc((void **)&d_tabs.p, (size_t)std::max<int64_t>(1, tab_elems) * sizeof(c2d)));
-    if (pool_elems) {
-        hipError_t e = dmalloc((void **)&d_pool.p, pool_elems * sizeof(c2d));
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            return fail(GOQU_ERR_OUT_OF_MEMORY, "qaoa batch: state pool of " + std::to_string(pool_elems * 16) +
-                                                    " bytes does not fit; split the batch");
-        }
-        GOQU_HIP(dmalloc((void **)&d_parts.p, std::max<int64_t>(1, parts) * 8));
-        GOQU_HIP(dmalloc((void **)&d_cut.p, cut_bytes));
+    size_t carved = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = carved;
+        carved += (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_inst = carve(insts.size() * sizeof(QInst));
+    const size_t o_small = carve(small_ids.size() * 4);
+    const size_t o_large = carve(large_ids.size() * 4);
+    const size_t o_goff = carve(goff.size() * 8);
+    const size_t o_work = carve(wtotal * sizeof(QWork));
+    const size_t up_bytes = carved;
+    const size_t o_e = carve((size_t)num_instances * 8);
+    const size_t o_g = carve(ng * 8);
+    const size_t o_tabs = carve((size_t)tab_elems * sizeof(c2d));
+    const size_t o_pool = carve((size_t)pool_elems * sizeof(c2d));
+    const size_t o_parts = carve((size_t)parts * 8);
+    const size_t o_cut = carve((size_t)cut_bytes);
+    CachedDevBuf dbuf(CACHE_QAOA);
+    if (dbuf.take(carved) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(GOQU_ERR_OUT_OF_MEMORY, "qaoa batch: state pool of " + std::to_string(pool_elems * 16) +
+                                                " bytes does not fit; split the batch");
     }
+    unsigned char *const base = static_cast<unsigned char *>(dbuf.p);
+    const QInst *d_inst = reinterpret_cast<const QInst *>(base + o_inst);
+    const int32_t *d_small = reinterpret_cast<const int32_t *>(base + o_small);
+    const int32_t *d_large = reinterpret_cast<const int32_t *>(base + o_large);
+    const int64_t *d_goff = reinterpret_cast<const int64_t *>(base + o_goff);
+    const QWork *d_work = reinterpret_cast<const QWork *>(base + o_work);
+    double *d_e = reinterpret_cast<double *>(base + o_e);
+    double *d_g = reinterpret_cast<double *>(base + o_g);
+    c2d *d_tabs = reinterpret_cast<c2d *>(base + o_tabs);
+    c2d *pool = reinterpret_cast<c2d *>(base + o_pool);
+    double *partials = reinterpret_cast<double *>(base + o_parts);
+    uint8_t *cutb = base + o_cut;
+    std::vector<unsigned char> up(up_bytes);
+    std::memcpy(up.data() + o_inst, insts.data(), insts.size() * sizeof(QInst));
+    std::memcpy(up.data() + o_small, small_ids.data(), small_ids.size() * 4);
+    std::memcpy(up.data() + o_large, large_ids.data(), large_ids.size() * 4);
+    std::memcpy(up.data() + o_goff, goff.data(), goff.size() * 8);
+    for (size_t s = 0; s < launches.size(); s++)
+        if (!launches[s].empty())
+            std::memcpy(up.data() + o_work + woff[s] * sizeof(QWork), launches[s].data(),
+                        launches[s].size() * sizeof(QWork));
     hipStream_t s;
     GOQU_HIP(hipStreamCreate(&s));
     struct StreamGuard {
         hipStream_t s;
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } sg{s};
-    GOQU_HIP(hipMemcpyAsync(d_inst.p, insts.data(), insts.size() * sizeof(QInst), hipMemcpyHostToDevice, s));
-    if (!small_ids.empty())
-        GOQU_HIP(hipMemcpyAsync(d_small.p, small_ids.data(), small_ids.size() * 4, hipMemcpyHostToDevice, s));
-    if (!large_ids.empty())
-        GOQU_HIP(hipMemcpyAsync(d_large.p, large_ids.data(), large_ids.size() * 4, hipMemcpyHostToDevice, s));
-    GOQU_HIP(hipMemcpyAsync(d_goff.p, goff.data(), goff.size() * 8, hipMemcpyHostToDevice, s));
-    std::vector<QWork> wflat;
-    for (auto &v : launches) wflat.insert(wflat.end(), v.begin(), v.end());
-    if (!wflat.empty())
-        GOQU_HIP(hipMemcpyAsync(d_work.p, wflat.data(), wflat.size() * sizeof(QWork), hipMemcpyHostToDevice, s));
+    GOQU_HIP(hipMemcpyAsync(base, up.data(), up_bytes, hipMemcpyHostToDevice, s));
     GOQU_HIP(hipStreamSynchronize(s));
 
     size_t lds_small = 0;
@@ -627,35 +844,29 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     GOQU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const c2d *tabs = static_cast<const c2d *>(d_tabs.p);
-    hipLaunchKernelGGL(k_qaoa_tables, dim3((unsigned)num_instances), dim3(128), 0, s,
-                       static_cast<const QInst *>(d_inst.p), static_cast<c2d *>(d_tabs.p));
+    const c2d *tabs = d_tabs;
+    hipLaunchKernelGGL(k_qaoa_tables, dim3((unsigned)num_instances), dim3(128), 0, s, d_inst, d_tabs);
     if (!small_ids.empty())
         hipLaunchKernelGGL(k_qaoa_small, dim3((unsigned)small_ids.size()), dim3(kQThreads), lds_small, s,
-                           static_cast<const QInst *>(d_inst.p), tabs, static_cast<const int32_t *>(d_small.p),
-                           static_cast<double *>(d_e.p), static_cast<const int64_t *>(d_goff.p),
-                           static_cast<double *>(d_g.p));
+                           d_inst, tabs, d_small, d_e, d_goff, d_g);
     for (size_t ph = 0; ph < launches.size(); ph++) {
         const auto &v = launches[ph];
         if (v.empty()) continue;
         const unsigned blocks = (unsigned)(v.back().first_block + insts[v.back().inst].ntiles);
         if (ph % 2)
-            hipLaunchKernelGGL(k_qaoa_phase<true>, dim3(blocks), dim3(kQThreads), 0, s,
-                               static_cast<const QInst *>(d_inst.p), tabs, static_cast<const QWork *>(d_work.p) + woff[ph],
-                               (int)v.size(), static_cast<c2d *>(d_pool.p), static_cast<uint8_t *>(d_cut.p),
-                               static_cast<double *>(d_parts.p));
+            hipLaunchKernelGGL(k_qaoa_pass<true>, dim3(blocks), dim3(kQThreads), 0, s,
+                               d_inst, tabs, d_work + woff[ph],
+                               (int)v.size(), pool, cutb, partials);
         else
-            hipLaunchKernelGGL(k_qaoa_phase<false>, dim3(blocks), dim3(kQThreads), 0, s,
-                               static_cast<const QInst *>(d_inst.p), tabs, static_cast<const QWork *>(d_work.p) + woff[ph],
-                               (int)v.size(), static_cast<c2d *>(d_pool.p), static_cast<uint8_t *>(d_cut.p),
-                               static_cast<double *>(d_parts.p));
+            hipLaunchKernelGGL(k_qaoa_pass<false>, dim3(blocks), dim3(kQThreads), 0, s,
+                               d_inst, tabs, d_work + woff[ph],
+                               (int)v.size(), pool, cutb, partials);
     }
     if (!large_ids.empty())
         hipLaunchKernelGGL(k_qaoa_finish, dim3((unsigned)((large_ids.size() + 127) / 128)), dim3(128), 0, s,
-                           static_cast<const QInst *>(d_inst.p), static_cast<const int32_t *>(d_large.p),
-                           (int)large_ids.size(), static_cast<const double *>(d_parts.p),
-                           static_cast<double *>(d_e.p), static_cast<const int64_t *>(d_goff.p),
-                           static_cast<double *>(d_g.p));
+                           d_inst, d_large,
+                           (int)large_ids.size(), partials,
+                           d_e, d_goff, d_g);
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (ce != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
     ce = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
@@ -663,11 +874,12 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
     if (ce == hipSuccess) ce = hipGraphLaunch(exec, s);
     prof_end(ev0, s, PROF_QAOA, 0.0);
     if (ce == hipSuccess) ce = hipStreamSynchronize(s);
+    if (ce == hipSuccess) dbuf.idle();
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (ce != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("qaoa graph: ") + hipGetErrorString(ce));
-    GOQU_HIP(hipMemcpy(energies, d_e.p, num_instances * 8, hipMemcpyDeviceToHost));
-    if (ng) GOQU_HIP(hipMemcpy(grads + param_offsets[0], d_g.p, ng * 8, hipMemcpyDeviceToHost));
+    GOQU_HIP(hipMemcpy(energies, d_e, num_instances * 8, hipMemcpyDeviceToHost));
+    if (ng) GOQU_HIP(hipMemcpy(grads + param_offsets[0], d_g, ng * 8, hipMemcpyDeviceToHost));
     return GOQU_OK;
     GOQU_GUARD_END
 }

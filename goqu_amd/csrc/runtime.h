@@ -32,8 +32,44 @@ int on_exception() noexcept;
 // n-th lowering of an op list throws std::bad_alloc.
 enum FaultKind { FAULT_NONE = 0, FAULT_DEVICE_ALLOC = 1, FAULT_HOST_ALLOC = 2 };
 bool fault_fires(int kind) noexcept;
-// hipMalloc through the fault-injection hook
+// hipMalloc through the fault-injection hook; when the device reports out-of-memory, the retained
+// buffers below are handed back to it and the allocation is tried once more
 hipError_t dmalloc(void **p, size_t bytes);
+
+// Retained device buffers.  The batch entry points (QAOA instance pools, circuit batches,
+// dynamic-circuit shot states) take their largest buffers from here and give them back when the
+// call ends.  Handing tens of GiB back to the device and requesting them again makes the driver
+// clear the recycled memory: on MI355X a 4,096-instance QAOA batch (a 42 GiB pool) paid 0.4-1.2 s
+// in hipMalloc and about 1 s in hipFree per call once the device's fresh memory was used up, against
+// 0.3 s for the batch itself.  So one given-back buffer per kind and device is kept for the next
+// call; a later call needing more frees it and allocates anew.
+enum CacheKind { CACHE_QAOA = 0, CACHE_BATCH_STATES, CACHE_BATCH_BLOBS, CACHE_DYN_STATES, CACHE_KINDS };
+// a buffer of at least `bytes` (its true size in *got), from the cache or from dmalloc
+hipError_t cache_take(int kind, void **p, size_t bytes, size_t *got);
+// keeps `p` (of `bytes`) for the next cache_take of this kind; the device must be done with it
+void cache_give(int kind, void *p, size_t bytes) noexcept;
+// frees every retained buffer of the current device
+void cache_release_all() noexcept;
+
+// RAII holder: takes from the cache, gives back on destruction.  The owner calls idle() once the
+// device has finished every use of the buffer (its stream synchronized); a holder destroyed without
+// that (an error path) waits for the whole device first, so a kept buffer is never still in use.
+struct CachedDevBuf {
+    int kind;
+    void *p = nullptr;
+    size_t bytes = 0;
+    bool is_idle = false;
+    explicit CachedDevBuf(int k) : kind(k) {}
+    CachedDevBuf(const CachedDevBuf &) = delete;
+    CachedDevBuf &operator=(const CachedDevBuf &) = delete;
+    hipError_t take(size_t need) { return cache_take(kind, &p, need, &bytes); }
+    void idle() { is_idle = true; }
+    ~CachedDevBuf() {
+        if (!p) return;
+        if (!is_idle) (void)hipDeviceSynchronize();
+        cache_give(kind, p, bytes);
+    }
+};
 // throws std::bad_alloc when a host-allocation fault is armed and due (before op-list lowering)
 void host_fault_point();
 

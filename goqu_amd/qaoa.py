@@ -91,30 +91,47 @@ def EfficientSU2(num_qubits: int, reps: int) -> Tuple[Circuit, int]:
 class MaxCutInstance:
     def __init__(self, num_qubits: int, edges: Sequence[Tuple[int, int]], layers: int, params: Sequence[float]):
         self.num_qubits = num_qubits
-        self.edges = [(int(a), int(b)) for a, b in edges]
+        # immutable, so the flat (a0, b0, a1, b1, ...) int32 copy used by batch marshalling stays valid
+        self.edges = tuple((int(a), int(b)) for a, b in edges)
+        self._edge_arr = np.array(self.edges, dtype=np.int32).reshape(-1)
         self.layers = layers
         self.params = np.asarray(params, dtype=np.float64)
         if len(self.params) != 2 * layers:
             raise ValueError("params must be (gamma_0, beta_0, gamma_1, beta_1, ...)")
 
 
+class MarshalledBatch:
+    """The flat arrays goqu_qaoa_batch takes (qubits, layers, edge offsets, edges, parameter
+    offsets, parameters) for a list of instances."""
+
+    def __init__(self, instances: Sequence[MaxCutInstance]):
+        ni = len(instances)
+        self.ni = ni
+        self.nq = np.fromiter((it.num_qubits for it in instances), dtype=np.int32, count=ni)
+        self.lay = np.fromiter((it.layers for it in instances), dtype=np.int32, count=ni)
+        arrs = [it._edge_arr for it in instances]
+        self.eoff = np.zeros(ni + 1, dtype=np.int64)
+        np.cumsum(np.fromiter((len(a) // 2 for a in arrs), dtype=np.int64, count=ni), out=self.eoff[1:])
+        self.poff = np.zeros(ni + 1, dtype=np.int64)
+        np.cumsum(2 * self.lay.astype(np.int64), out=self.poff[1:])
+        self.edges = np.concatenate(arrs) if ni and self.eoff[-1] else np.zeros(2, dtype=np.int32)
+        self.params = np.concatenate([it.params for it in instances]) if ni else np.zeros(1)
+
+
+def RunMarshalled(m: MarshalledBatch) -> Tuple[np.ndarray, np.ndarray]:
+    """One goqu_qaoa_batch call on arrays already marshalled: the energies, and every instance's
+    gradient concatenated in instance order (m.poff delimits them)."""
+    energies = np.zeros(max(m.ni, 1))
+    grads = np.zeros(max(int(m.poff[-1]), 1))
+    N.check(N.lib().goqu_qaoa_batch(m.ni, N.i32ptr(m.nq), N.i32ptr(m.lay), N.i64ptr(m.eoff), N.i32ptr(m.edges),
+                                    N.i64ptr(m.poff), N.dptr(m.params), N.dptr(energies), N.dptr(grads)))
+    return energies[:m.ni], grads
+
+
 def BatchMaxCut(instances: Sequence[MaxCutInstance]) -> Tuple[np.ndarray, List[np.ndarray]]:
     """Energy <H> and its gradient (gamma_0, beta_0, ...) for every instance, all instances in
     one device launch graph (goqu_qaoa_batch)."""
-    ni = len(instances)
-    nq = np.array([it.num_qubits for it in instances], dtype=np.int32)
-    lay = np.array([it.layers for it in instances], dtype=np.int32)
-    eoff = np.zeros(ni + 1, dtype=np.int64)
-    poff = np.zeros(ni + 1, dtype=np.int64)
-    for i, it in enumerate(instances):
-        eoff[i + 1] = eoff[i] + len(it.edges)
-        poff[i + 1] = poff[i] + 2 * it.layers
-    edges = np.array([q for it in instances for e in it.edges for q in e], dtype=np.int32)
-    if len(edges) == 0:
-        edges = np.zeros(2, dtype=np.int32)
-    params = np.concatenate([it.params for it in instances]) if ni else np.zeros(1)
-    energies = np.zeros(max(ni, 1))
-    grads = np.zeros(max(int(poff[-1]), 1))
-    N.check(N.lib().goqu_qaoa_batch(ni, N.i32ptr(nq), N.i32ptr(lay), N.i64ptr(eoff), N.i32ptr(edges), N.i64ptr(poff),
-                                    N.dptr(params), N.dptr(energies), N.dptr(grads)))
-    return energies[:ni], [grads[poff[i]:poff[i + 1]].copy() for i in range(ni)]
+    m = MarshalledBatch(instances)
+    energies, grads = RunMarshalled(m)
+    po = m.poff.tolist()
+    return energies, [grads[a:b] for a, b in zip(po, po[1:])]

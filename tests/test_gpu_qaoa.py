@@ -74,3 +74,23 @@ def test_batch_22_qubits():
     e2, g2 = GR.Adjoint(c, H, FreeParameters(c)).value_and_grad(x)
     assert abs(e[0] - e2) < 1e-10
     assert np.max(np.abs(g[0] - g2)) < 1e-9
+
+
+def test_batch_passes_every_tile_shape_and_depth():
+    """Instances whose B tiles run the mixer on one, two or three register groups (13-16, 17-20,
+    21-22 qubits) at every depth 1-4 in one batch (p + 1 forward and p + 1 reverse passes each,
+    the steps of different depths interleaved), each against the generic adjoint."""
+    rng = np.random.default_rng(31)
+    shapes = [(13, 4), (16, 1), (17, 3), (18, 2), (20, 4), (21, 1), (22, 3), (19, 2)]
+    insts = [Q.MaxCutInstance(n, random_graph(n, 0.3, rng), p, rng.uniform(0, math.pi, 2 * p)) for n, p in shapes]
+    energies, grads = Q.BatchMaxCut(insts)
+    for it, e, g in zip(insts, energies, grads):
+        H = Q.MaxCutHamiltonian(it.edges, it.num_qubits)
+        c = Q.QAOACircuit(H, it.layers)
+        e2, g2 = GR.Adjoint(c, H, FreeParameters(c)).value_and_grad(it.params)
+        assert abs(e - e2) < 1e-10, (it.num_qubits, it.layers, e, e2)
+        assert np.max(np.abs(g - g2)) < 1e-9, (it.num_qubits, it.layers, g, g2)
+    # a second call of the same batch reuses the retained device pool: identical results
+    energies2, grads2 = Q.BatchMaxCut(insts)
+    assert np.array_equal(energies, energies2)
+    assert all(np.array_equal(a, b) for a, b in zip(grads, grads2))
