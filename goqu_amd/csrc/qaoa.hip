@@ -436,10 +436,17 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict
                                                          const QWork *__restrict__ work, int nwork,
                                                          c2d *__restrict__ pool, uint8_t *__restrict__ cuts,
                                                          double *__restrict__ parts) {
+    // 32 KiB of LDS in all, so five forward workgroups fit a CU: the relayout buffer also holds the
+    // cost table, the adjacency masks and the reduction slots, which are never live during a relayout
+    // (a cost layer ends with a barrier before the next relayout writes; a relayout ends with one
+    // after its reads)
     __shared__ double buf[1 << kQTileBits];
-    __shared__ c2d tab[kQMaxEdges + 1];
-    __shared__ double sh[4 * (kQThreads / 64)];
-    __shared__ uint32_t sup[kQMaxQubits];
+    c2d *const tab = reinterpret_cast<c2d *>(buf);                                  // 3,712 B
+    uint32_t *const sup = reinterpret_cast<uint32_t *>(buf) + 928;                 // 88 B
+    double *const sh = buf + 480;                                                   // 128 B
+    static_assert((kQMaxEdges + 1) * sizeof(c2d) <= 928 * 4 && 928 * 4 + kQMaxQubits * 4 <= 480 * 8 &&
+                      (480 + 4 * (kQThreads / 64)) * 8 <= sizeof(double) << kQTileBits,
+                  "LDS carve");
     // which work item: last with first_block <= blockIdx.x
     int lo = 0, hi = nwork;
     while (hi - lo > 1) {
@@ -607,6 +614,7 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict
                 if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
             }
         }
+        __syncthreads();  // the table and masks share LDS with the next relayout
     };
 
     // forward: mixer half `pre`, cost `cost`, mixer half `post`; reverse: the same inverted, in the
@@ -652,28 +660,30 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict
     }
 }
 
-// energies / gradients of the large instances from their per-tile partials (fixed order)
-__global__ void k_qaoa_finish(const QInst *__restrict__ insts, const int32_t *__restrict__ ids, int nids,
-                              const double *__restrict__ parts, double *__restrict__ energies,
-                              const int64_t *__restrict__ goff, double *__restrict__ grads) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nids) return;
-    const int id = ids[t];
+// energies / gradients of the large instances from their per-tile partials: one wave per instance,
+// lane l summing tiles l, l + 64, ... and a fixed butterfly across the lanes (the same order every run)
+__global__ void __launch_bounds__(64) k_qaoa_finish(const QInst *__restrict__ insts, const int32_t *__restrict__ ids,
+                                                    const double *__restrict__ parts, double *__restrict__ energies,
+                                                    const int64_t *__restrict__ goff, double *__restrict__ grads) {
+    const int id = ids[blockIdx.x];
     const QInst &I = insts[id];
     const double *pt = parts + I.part;
-    double e = 0;
-    for (int tl = 0; tl < I.ntiles; tl++) e += pt[(int64_t)tl * I.slot_stride];
-    energies[id] = e;
+    const int ns = I.slot_stride;  // 1 + 3p: energy, then (gamma, beta A half, beta B half) per layer
+    auto total = [&](int j) {
+        double v = 0;
+        for (int t = threadIdx.x; t < I.ntiles; t += 64) v += pt[(int64_t)t * ns + j];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        return v;
+    };
+    const double e = total(0);
+    if (threadIdx.x == 0) energies[id] = e;
     for (int k = 0; k < I.p; k++) {
-        double gg = 0, ga = 0, gb = 0;
-        for (int tl = 0; tl < I.ntiles; tl++) {
-            const double *s = pt + (int64_t)tl * I.slot_stride + 1 + 3 * k;
-            gg += s[0];
-            ga += s[1];
-            gb += s[2];
+        const double gg = total(1 + 3 * k), ga = total(2 + 3 * k), gb = total(3 + 3 * k);
+        if (threadIdx.x == 0) {
+            grads[goff[id] + 2 * k] = gg;
+            grads[goff[id] + 2 * k + 1] = 2.0 * (ga + gb);
         }
-        grads[goff[id] + 2 * k] = gg;
-        grads[goff[id] + 2 * k + 1] = 2.0 * (ga + gb);
     }
 }
 
@@ -863,9 +873,7 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
                                (int)v.size(), pool, cutb, partials);
     }
     if (!large_ids.empty())
-        hipLaunchKernelGGL(k_qaoa_finish, dim3((unsigned)((large_ids.size() + 127) / 128)), dim3(128), 0, s,
-                           d_inst, d_large,
-                           (int)large_ids.size(), partials,
+        hipLaunchKernelGGL(k_qaoa_finish, dim3((unsigned)large_ids.size()), dim3(64), 0, s, d_inst, d_large, partials,
                            d_e, d_goff, d_g);
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (ce != hipSuccess) return fail(GOQU_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
