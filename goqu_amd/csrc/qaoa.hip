@@ -106,32 +106,48 @@ __device__ __forceinline__ double block_sum1(double v, double *sh) {
     return t;
 }
 
-// phase table: tab[c] = exp(-i gamma/2 (|E| - 2c)), c = 0..|E|
-// The layer tables of every instance, computed once per batch: for layer k at I.tab + k (ne + 2),
-// the cost phases exp(-i gamma_k / 2 (ne - 2c)) for c = 0..ne, then (cos beta_k, sin beta_k).  The
-// tile kernels copy a layer's phases into LDS instead of evaluating ne + 2 sines and cosines per
+// The layer tables of every instance, computed once per batch: for layer k at I.tab + k (ne + 3),
+// the cost phases exp(-i gamma_k / 2 (ne - 2c)) for c = 0..ne, then (cos beta_k, sin beta_k), then
+// the shear form of the mixer rotation (tan(b/2), sin b) with b = beta_k folded by pi into
+// cos b >= 0 (RX(2 beta) = -RX(2 (beta - pi)); the tile kernels drop that global sign, which
+// multiplies psi and lambda alike and cancels in the energy and every gradient term).  The tile
+// kernels copy a layer's phases into LDS instead of evaluating ne + 3 sines and cosines per
 // workgroup.
+constexpr int kQTabExtra = 3;
 __global__ void k_qaoa_tables(const QInst *__restrict__ insts, c2d *__restrict__ tabs) {
     const QInst &I = insts[blockIdx.x];
-    const int ne = I.nedges, T = ne + 2;
+    const int ne = I.nedges, T = ne + kQTabExtra;
     for (int e = threadIdx.x; e < I.p * T; e += blockDim.x) {
         const int k = e / T, c = e % T;
         double sn, cs;
-        if (c <= ne) sincos(-0.5 * I.gamma[k] * (double)(ne - 2 * c), &sn, &cs);
-        else sincos(I.beta[k], &sn, &cs);
+        if (c <= ne) {
+            sincos(-0.5 * I.gamma[k] * (double)(ne - 2 * c), &sn, &cs);
+        } else {
+            sincos(I.beta[k], &sn, &cs);
+            if (c == ne + 2) {
+                if (cs < 0) cs = -cs, sn = -sn;
+                cs = sn / (1.0 + cs);  // tan(b/2), |.| <= 1
+            }
+        }
         tabs[I.tab + e] = {cs, sn};
     }
 }
 // layer k's cost phases into LDS
 __device__ __forceinline__ void load_tab(c2d *tab, const QInst &I, const c2d *__restrict__ tabs, int k) {
-    const c2d *src = tabs + I.tab + (int64_t)k * (I.nedges + 2);
+    const c2d *src = tabs + I.tab + (int64_t)k * (I.nedges + kQTabExtra);
     for (int c = threadIdx.x; c <= I.nedges; c += blockDim.x) tab[c] = src[c];
     __syncthreads();
 }
 // (cos beta_k, sin beta_k) of layer k
 __device__ __forceinline__ void beta_cs(const QInst &I, const c2d *__restrict__ tabs, int k, double &cb, double &sb) {
-    const c2d v = tabs[I.tab + (int64_t)k * (I.nedges + 2) + I.nedges + 1];
+    const c2d v = tabs[I.tab + (int64_t)k * (I.nedges + kQTabExtra) + I.nedges + 1];
     cb = v.x;
+    sb = v.y;
+}
+// (tan(b/2), sin b) of layer k's folded mixer angle
+__device__ __forceinline__ void beta_shear(const QInst &I, const c2d *__restrict__ tabs, int k, double &tb, double &sb) {
+    const c2d v = tabs[I.tab + (int64_t)k * (I.nedges + kQTabExtra) + I.nedges + 2];
+    tb = v.x;
     sb = v.y;
 }
 
@@ -268,22 +284,20 @@ __device__ __forceinline__ constexpr uint32_t qswz(uint32_t l) { return l ^ ((l 
 // fresh registers and copied them back at every branch merge, and the reverse kernel needed over
 // 256 VGPRs (one wave per SIMD).
 //
-// RX: [a0, a1] <- [c a0 - i s a1, -i s a0 + c a1], which is two real rotations, of (x0, y1) and of
-// (y0, x1)
-__device__ __forceinline__ void rx_pair(c2d &a0, c2d &a1, double c, double s) {
-    double t0, t1, t2, t3;
+// RX: [a0, a1] <- [c a0 - i s a1, -i s a0 + c a1] (up to the global sign of the folded angle, with
+// t = tan(b/2)), which is two real rotations, of (x0, y1) and of (y0, x1), each as three in-place
+// shears (x0 += t y1, y1 -= s x0, x0 += t y1; y0 -= t x1, x1 += s y0, y0 -= t x1): six FMAs and no
+// temporaries.  The inverse is the same with (-t, -s).
+__device__ __forceinline__ void rx_pair(c2d &a0, c2d &a1, double t, double s) {
     asm volatile(
-        "v_mul_f64 %[t0], %[c], %[x0]\n\t"
-        "v_mul_f64 %[t1], %[s], %[x0]\n\t"
-        "v_mul_f64 %[t2], %[c], %[y0]\n\t"
-        "v_mul_f64 %[t3], %[s], %[y0]\n\t"
-        "v_fma_f64 %[x0], %[s], %[y1], %[t0]\n\t"
-        "v_fma_f64 %[y1], %[c], %[y1], -%[t1]\n\t"
-        "v_fma_f64 %[y0], -%[s], %[x1], %[t2]\n\t"
-        "v_fma_f64 %[x1], %[c], %[x1], %[t3]"
-        : [x0] "+v"(a0.x), [y0] "+v"(a0.y), [x1] "+v"(a1.x), [y1] "+v"(a1.y), [t0] "=&v"(t0),
-          [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
-        : [c] "v"(c), [s] "v"(s));
+        "v_fma_f64 %[x0], %[t], %[y1], %[x0]\n\t"
+        "v_fma_f64 %[y0], -%[t], %[x1], %[y0]\n\t"
+        "v_fma_f64 %[y1], -%[s], %[x0], %[y1]\n\t"
+        "v_fma_f64 %[x1], %[s], %[y0], %[x1]\n\t"
+        "v_fma_f64 %[x0], %[t], %[y1], %[x0]\n\t"
+        "v_fma_f64 %[y0], -%[t], %[x1], %[y0]"
+        : [x0] "+v"(a0.x), [y0] "+v"(a0.y), [x1] "+v"(a1.x), [y1] "+v"(a1.y)
+        : [t] "v"(t), [s] "v"(s));
 }
 // a <- a * (tc + i ts), or a * (tc - i ts) when CONJ
 template <bool CONJ>
@@ -368,7 +382,7 @@ __device__ __forceinline__ void relayout(c2d (&A)[16], double *buf, uint32_t lan
 }
 
 // the mixer on the register bits of layout LAY that carry local bits >= b0 (forward), or the
-// overlaps and the uncompute of both states (reverse)
+// overlaps and the uncompute of both states (reverse); (cb, sb) = (tan(b/2), sin b) of the layer
 template <bool REV, int LAY>
 __device__ __forceinline__ void mix_group(c2d (&P)[16], c2d (&L)[16], int b0, double cb, double sb, double &wy) {
     constexpr int base = LAY == 0 ? 8 : LAY == 1 ? 0 : 4;
@@ -377,8 +391,8 @@ __device__ __forceinline__ void mix_group(c2d (&P)[16], c2d (&L)[16], int b0, do
         if (base + J < b0) return;
         if constexpr (REV) {
             wy += xov_reg<J>(P, L);
-            rx_reg<J>(P, cb, -sb);
-            rx_reg<J>(L, cb, -sb);
+            rx_reg<J>(P, -cb, -sb);
+            rx_reg<J>(L, -cb, -sb);
         } else {
             rx_reg<J>(P, cb, sb);
         }
@@ -540,7 +554,7 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict
     // overlaps and the uncompute of both states.
     auto mix_out = [&](int k, double &wy) __attribute__((always_inline)) {
         double cb, sb;
-        beta_cs(I, tabs, k, cb, sb);
+        beta_shear(I, tabs, k, cb, sb);
         mix_group<REV, 0>(P, L, b0, cb, sb, wy);
         if (b0 <= 3) {
             relayout_states<REV, 0, 1>(P, L, buf, lane, wave);
@@ -554,7 +568,7 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict
     };
     auto mix_back = [&](int k, double &wy) __attribute__((always_inline)) {
         double cb, sb;
-        beta_cs(I, tabs, k, cb, sb);
+        beta_shear(I, tabs, k, cb, sb);
         if (b0 <= 3) {
             mix_group<REV, 2>(P, L, b0, cb, sb, wy);
             relayout_states<REV, 2, 1>(P, L, buf, lane, wave);
@@ -733,7 +747,7 @@ extern "C" int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits,
             I.up[a] |= 1u << b;
         }
         I.tab = tab_elems;
-        tab_elems += (int64_t)I.p * (I.nedges + 2);
+        tab_elems += (int64_t)I.p * (I.nedges + kQTabExtra);
         I.small = I.n <= kQTileBits;
         if (I.small) {
             small_ids.push_back(i);
