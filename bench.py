@@ -262,9 +262,10 @@ def sampling_section(n: int = 20, layers: int = 20, shots: int = 1024) -> dict:
 def qaoa_batch_section(num: int = 4096, seed: int = 5) -> dict:
     """4,096 MaxCut instances, n ~ U{10..22}, G(n, 0.3), p ~ U{1..4}, gamma/beta ~ U[0, pi):
     energy and all 2p gradient entries of every instance in one captured launch sequence.
-    Device bytes count what the tile kernels move for instances above 12 qubits: (96 p + 48) B x
-    2^n (p + 1 forward passes over psi, the first writing only, and p + 1 reverse passes over psi
-    and lambda, the last reading only); smaller instances stay in LDS.  The timed call follows
+    Device bytes count the amplitudes the tile kernels move for instances above 12 qubits:
+    (96 p + 32) B x 2^n (p + 1 forward passes over psi, the first writing only; p + 1 reverse
+    passes over psi and lambda, the first reading psi alone, the last writing nothing); smaller
+    instances stay in LDS.  The timed call follows
     one untimed call of the same batch.  `s` is the whole Python call; large_instance_GBps divides the device bytes by the goqu_qaoa_batch call alone (inputs
     marshalled into flat arrays first, as a Go caller passes them), which includes the upload, the
     graph and the download."""
@@ -297,7 +298,7 @@ def qaoa_batch_section(num: int = 4096, seed: int = 5) -> dict:
     gms, _, _ = N.profile_read_kernel(N.PROF_QAOA)
     N.profile(False)
     big = [it for it in insts if it.num_qubits > 12]
-    dev_bytes = sum((96.0 * it.layers + 48.0) * (1 << it.num_qubits) for it in big)
+    dev_bytes = sum((96.0 * it.layers + 32.0) * (1 << it.num_qubits) for it in big)
     roof = None
     if gms > 0:
         a = dev_bytes / (gms / 1e3) / 1e9

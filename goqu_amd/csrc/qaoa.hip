@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_small(const QInst *__restric
 //   [C_0 M_0^A] [M_0^B C_1 M_1^B] [M_1^A C_2 M_2^A] ... [M_{p-1}^X],
 // each applying the second half of one layer's mixer, the next cost layer and the first half of
 // the next mixer; the reverse sweep runs the same passes inverted, last to first.  With one pass per
-// half layer instead, a batch moved (192 p - 16) bytes per amplitude; this way it moves 96 p + 48.
+// half layer instead, a batch moved (192 p - 16) bytes per amplitude; this way it moves 96 p + 32.
 struct QWork {
     int32_t inst;
     int8_t half;               // 0: A tiles, 1: B tiles
@@ -470,8 +470,14 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict
     }
     const QWork W = work[lo];
     const QInst &I = insts[W.inst];
-    const uint32_t tile = blockIdx.x - W.first_block;
     const int n = I.n, ne = I.nedges, half = W.half, kpre = W.pre, kcost = W.cost, kpost = W.post;
+    // B tiles of 22-qubit instances read and write 64-byte runs (qubits 0-1), half a 128-byte cache
+    // line; the other half belongs to the tile next in qubit 2.  Workgroups go to the eight XCDs
+    // round-robin by index, so the pair is given to workgroups b and b + 8 (one XCD, one L2, running
+    // together) instead of b and b + 1 (two XCDs).
+    uint32_t tile = blockIdx.x - W.first_block;
+    if (half && 2 * kQTileBits - n < 3 && I.ntiles >= 16)
+        tile = (tile & ~15u) | ((tile & 7u) << 1) | ((tile >> 3) & 1u);
     const int flags = W.flags;
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     c2d *psi = pool + I.psi, *lam = pool + I.lam;
