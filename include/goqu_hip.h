@@ -191,7 +191,9 @@ int goqu_dm_adjoint(goqu_dm_t *h, const goqu_op_t *ops, int32_t num_ops,
  * ir.FreeParameters order (gamma_0, beta_0, gamma_1, ...).  The circuit is H on every qubit,
  * then per layer RZZ(gamma_k) on every edge and RX(2 beta_k) on every qubit; H is
  * sum_edges (-0.5 I + 0.5 Z_i Z_j) (algorithm/qaoa/maxcut.go:13-36).  energies[i] and
- * grads[param_offsets[i] ...] are written.  All instances run in one captured launch graph. */
+ * grads[param_offsets[i] ...] are written.  All instances run in one captured launch graph.
+ * The call's device memory stays reserved for the next batch call until the process ends (a
+ * failed device allocation anywhere in the library releases it first). */
 int goqu_qaoa_batch(int32_t num_instances, const int32_t *num_qubits, const int32_t *layers,
                     const int64_t *edge_offsets, const int32_t *edges,
                     const int64_t *param_offsets, const double *params, double *energies,
