@@ -1174,12 +1174,10 @@ __device__ __forceinline__ const T *blob_at(const T *p, uint64_t byte_off) {
     return reinterpret_cast<const T *>(reinterpret_cast<const char *>(p) + byte_off);
 }
 
-// Single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops
-// (few enough VGPRs for four waves per SIMD, and code small enough for short branches; a fifth
-// wave, forced by spilling, measured slower), k_stage_dm / k_stage_dm_big the density-matrix stages
-// without / with them (channel superoperators as well),
-// k_stage_big the stages that hold them, k_stage_dm density-matrix stages (channel
-// superoperators as well).
+// Single-state instantiations: k_stage runs statevector stages without 3-4-slot dense ops (few
+// enough VGPRs for four waves per SIMD, and code small enough for short branches; a fifth wave,
+// forced by spilling, measured slower), k_stage_big the stages that hold them; k_stage_dm /
+// k_stage_dm_big the density-matrix stages without / with them (channel superoperators as well).
 template <typename R>
 __global__ void __launch_bounds__(512) k_stage(cx<R> *__restrict__ st, const StageHeader *__restrict__ hdr,
                                                const Instr *__restrict__ prog, const R *__restrict__ pool,
