@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <future>
 #include <map>
 
 #include "runtime.h"
@@ -102,6 +103,28 @@ int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goq
         if (rc) return fail(rc, "op " + std::to_string(k) + ": " + err);
         all.insert(all.end(), low[k].begin(), low[k].end());
     }
+    // The fused reverse program depends on the ops and derivatives alone: it is built and planned
+    // on a second host thread while the device runs the forward pass and the Hamiltonian (planning
+    // a 24-qubit EfficientSU2 reverse program costs milliseconds of host time).
+    struct RevPlan {
+        int rc = GOQU_OK;
+        bool fused = false;
+        std::string err;
+        Program P;
+    };
+    std::future<RevPlan> rev_plan = std::async(std::launch::async, [&]() {
+        RevPlan r;
+        try {
+            std::vector<EOp> rev;
+            r.rc = build_sv_reverse(ops, nops, nq_user, derivs, nd, rev, r.fused, r.err);
+            if (!r.rc && r.fused && !rev.empty())
+                r.P = plan_program(rev, st.nq, default_plan_config(st.prec, st.nq, true));
+        } catch (...) {
+            r.rc = GOQU_ERR_OUT_OF_MEMORY;
+            r.err = "host memory exhausted while planning the reverse pass";
+        }
+        return r;
+    });
     int rc = fill_basis(st, 0, s);
     if (rc) return rc;
     rc = run_eops(st, all, s);
@@ -125,16 +148,16 @@ int adjoint_sv(State &st, int nq_user, const goqu_op_t *ops, int nops, const goq
 
     // Fused reverse pass: one program, run on (psi, lam) together, in which every derivative is
     // an EOP_GRAD op placed right before the inverse of its op.
-    std::vector<EOp> rev;
-    bool fused = false;
-    rc = build_sv_reverse(ops, nops, nq_user, derivs, nd, rev, fused, err);
+    RevPlan rp = rev_plan.get();
+    const bool fused = rp.fused;
+    rc = rp.rc;
     if (!rc && fused) {
         std::vector<double> vals;
-        rc = run_pair(work, lam, rev, nd, vals, s);
+        rc = run_pair_planned(work, lam, rp.P, nd, vals, s);
         if (!rc)
             for (int i = 0; i < nd; i++) grads[derivs[i].param] += derivs[i].scale * 2.0 * vals[i];
     } else if (rc) {
-        rc = fail(rc, err);
+        rc = fail(rc, rp.err);
     }
     if (rc || fused) {
         cleanup();

@@ -221,6 +221,10 @@ int launch_prepared(State &st, const PreparedOps &p, hipStream_t s, const ShotBa
 // ops applied to psi and lam together; EOP_GRAD ops accumulate Re <lam| D |psi> per slot
 int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, std::vector<double> &slot_vals,
              hipStream_t s);
+// run_pair with the paired program already planned (plan_program(eops, psi.nq,
+// default_plan_config(psi.prec, psi.nq, true)))
+int run_pair_planned(State &psi, State &lam, const Program &P, int nslots, std::vector<double> &slot_vals,
+                     hipStream_t s);
 
 int fill_basis(State &st, uint64_t index, hipStream_t s);  // |index>
 // nstates states of view.size() amplitudes from view.amps on, each |0...0>

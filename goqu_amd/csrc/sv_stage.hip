@@ -1524,7 +1524,14 @@ int run_pair(State &psi, State &lam, const std::vector<EOp> &eops, int nslots, s
              hipStream_t s) {
     slot_vals.assign(nslots, 0.0);
     if (eops.empty()) return GOQU_OK;
-    Program P = plan_program(eops, psi.nq, default_plan_config(psi.prec, psi.nq, true));
+    return run_pair_planned(psi, lam, plan_program(eops, psi.nq, default_plan_config(psi.prec, psi.nq, true)), nslots,
+                            slot_vals, s);
+}
+
+int run_pair_planned(State &psi, State &lam, const Program &P, int nslots, std::vector<double> &slot_vals,
+                     hipStream_t s) {
+    slot_vals.assign(nslots, 0.0);
+    if (P.stages.empty()) return GOQU_OK;
     std::vector<double> prog_vals;
     int rc = psi.prec == GOQU_C128 ? run_pair_t<double>(psi, lam, P, s, prog_vals)
                                    : run_pair_t<float>(psi, lam, P, s, prog_vals);
