@@ -197,8 +197,21 @@ class AdjointProgram:
         self.index = {n: i for i, n in enumerate(self.names)}
         self.density = density
         self.noise = noise
+        self._fast = None
 
     def build(self, x: Sequence[float]):
+        """(op buffer, derivative records, count, kept arrays) at parameters x.  Statevector
+        circuits whose parameterised gates are all RX / RY / RZ of one expression reuse the records
+        of their first build and refill only the angle-dependent matrices, vectorised (the arrays
+        returned are overwritten by the next call)."""
+        if not self.density:
+            if self._fast is None:
+                self._fast = _FastRotations.make(self, x) or False
+            if self._fast:
+                return self._fast.refill(x)
+        return self._build_generic(x)
+
+    def _build_generic(self, x: Sequence[float]):
         b = {n: float(v) for n, v in zip(self.names, x)}
         buf = N.OpBuffer()
         derivs: List[Tuple[int, int, float, np.ndarray]] = []
@@ -239,6 +252,112 @@ class AdjointProgram:
             arr[i].scale = sc
             arr[i].dmatrix = N.dptr(flat)
         return buf, arr, len(derivs), keep
+
+
+class _FrozenOps:
+    """Op records built once, with their ctypes array cached (the matrices they point at are
+    refilled in place)."""
+
+    def __init__(self, buf: "N.OpBuffer"):
+        self.buf = buf
+        self._arr = buf.array()
+
+    def __len__(self):
+        return len(self.buf)
+
+    def array(self):
+        return self._arr
+
+
+class _FastRotations:
+    """The statevector adjoint program of a circuit whose parameterised gates are RX / RY / RZ of
+    one expression each: records, matrix storage and derivative records are laid out once; each
+    build evaluates the expressions and writes every rotation matrix and its derivative
+    -i/2 G M (G = X, Y, Z) with array arithmetic, the same values gate.RX/RY/RZ and
+    gate_derivative give."""
+
+    KINDS = {"RX": 0, "RY": 1, "RZ": 2}
+
+    @classmethod
+    def make(cls, prog: AdjointProgram, x: Sequence[float]) -> Optional["_FastRotations"]:
+        b = {n: float(v) for n, v in zip(prog.names, x)}
+        ops = []
+        for op in prog.circuit.Ops():
+            g = op.Gate
+            if g is None or op.ControlFlow is not None:
+                continue
+            if isinstance(g, SymbolicGate):
+                if g.ctor_name not in cls.KINDS or len(g.exprs) != 1:
+                    return None
+            ops.append(op)
+        self = cls()
+        self.prog = prog
+        buf = N.OpBuffer()
+        rot = []  # (record index, kind, expression)
+        for op in ops:
+            g = op.Gate
+            if isinstance(g, SymbolicGate):
+                k = N.add_sv_gate(buf, g.Bind(b), op.Qubits)
+                rot.append((k, cls.KINDS[g.ctor_name], g.exprs[0]))
+            else:
+                N.add_sv_gate(buf, g, op.Qubits)
+        nr = len(rot)
+        self.exprs = [e for _, _, e in rot]
+        self.kind = np.array([kd for _, kd, _ in rot], dtype=np.int64)
+        self.mats = np.zeros((max(nr, 1), 2, 2), dtype=np.complex128)
+        flat = self.mats.reshape(-1).view(np.float64)
+        for j, (k, _, _) in enumerate(rot):  # records point into one contiguous store
+            r = buf.records[k]
+            view = flat[8 * j:8 * j + 8]
+            buf.records[k] = (r[0], r[1], r[2], r[3], view, r[5], view.ctypes.data)
+        # derivative records: one per (rotation, free parameter of its expression)
+        dref = []
+        for j, (k, _, e) in enumerate(rot):
+            for p in dict.fromkeys(e.free()):
+                if p in prog.index:
+                    dref.append((j, k, prog.index[p], p, e))
+        self.dref = dref
+        self.dmats = np.zeros((max(nr, 1), 2, 2), dtype=np.complex128)
+        dflat = self.dmats.reshape(-1).view(np.float64)
+        self.darr = (N.GoquDeriv * max(1, len(dref)))()
+        for i, (j, k, pi, _, _) in enumerate(dref):
+            self.darr[i].op = k
+            self.darr[i].param = pi
+            self.darr[i].dmatrix = N.dptr(dflat[8 * j:8 * j + 8])
+        self.ops = _FrozenOps(buf)
+        self.keep = [self.mats, self.dmats]
+        return self
+
+    def refill(self, x: Sequence[float]):
+        b = {n: float(v) for n, v in zip(self.prog.names, x)}
+        nr = len(self.exprs)
+        if nr:
+            th = np.fromiter((e.eval(b) for e in self.exprs), dtype=np.float64, count=nr)
+            c, s = np.cos(th / 2), np.sin(th / 2)
+            M, D, kd = self.mats[:nr], self.dmats[:nr], self.kind
+            M[:] = 0
+            D[:] = 0
+            x_, y_, z_ = kd == 0, kd == 1, kd == 2
+            # RX = [[c, -i s], [-i s, c]], dRX = [[-s/2, -i c/2], [-i c/2, -s/2]]
+            M[x_, 0, 0] = M[x_, 1, 1] = c[x_]
+            M[x_, 0, 1] = M[x_, 1, 0] = -1j * s[x_]
+            D[x_, 0, 0] = D[x_, 1, 1] = -0.5 * s[x_]
+            D[x_, 0, 1] = D[x_, 1, 0] = -0.5j * c[x_]
+            # RY = [[c, -s], [s, c]], dRY = [[-s/2, -c/2], [c/2, -s/2]]
+            M[y_, 0, 0] = M[y_, 1, 1] = c[y_]
+            M[y_, 0, 1] = -s[y_]
+            M[y_, 1, 0] = s[y_]
+            D[y_, 0, 0] = D[y_, 1, 1] = -0.5 * s[y_]
+            D[y_, 0, 1] = -0.5 * c[y_]
+            D[y_, 1, 0] = 0.5 * c[y_]
+            # RZ = diag(c - i s, c + i s), dRZ = diag(-s/2 - i c/2, -s/2 + i c/2)
+            M[z_, 0, 0] = c[z_] - 1j * s[z_]
+            M[z_, 1, 1] = c[z_] + 1j * s[z_]
+            D[z_, 0, 0] = -0.5 * s[z_] - 0.5j * c[z_]
+            D[z_, 1, 1] = -0.5 * s[z_] + 0.5j * c[z_]
+        for i, (_, _, _, p, e) in enumerate(self.dref):
+            self.darr[i].scale = e.deriv(p, b)
+        return self.ops, self.darr, len(self.dref), self.keep
 
 
 def Adjoint(circuit: Circuit, hamiltonian: PauliSum, param_names: Sequence[str], sim=None,
