@@ -304,6 +304,71 @@ static EOp make_superop(const int *q, int k, const double *m) {
     return e;
 }
 
+// Channel superoperators that meet with nothing in between merge: a parity-block channel e moves
+// back past every op on other qubits (they commute) to the last op that touches its qubits; when
+// that op is a parity-block channel on a superset of e's row and column qubits, e is folded into
+// it (S <- E S, E embedded in the wider basis; the product of parity-keeping superoperators keeps
+// parity).  A noise model that ends each layer with a one-qubit channel (amplitude damping on
+// Delay) after a gate's own depolarizing channel then costs one parity-block op instead of two.
+static void merge_channels(std::vector<EOp> &ops) {
+    std::vector<bool> dead(ops.size(), false);
+    for (size_t i = 0; i < ops.size(); i++) {
+        EOp &e = ops[i];
+        if (e.kind != EOP_PARITY || e.cmask != 0) continue;
+        const uint64_t et = e.touch();
+        size_t j = i;
+        bool found = false;
+        while (j > 0) {
+            j--;
+            if (dead[j]) continue;
+            if (ops[j].is_barrier()) break;
+            if (ops[j].touch() & et) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) continue;
+        EOp &t = ops[j];
+        if (t.kind != EOP_PARITY || t.cmask != 0 || (t.touch() & et) != et) continue;
+        // position in t's index (MSB first) of each of e's qubits
+        const int K = t.k, D = 1 << K, ke = e.k, De = 1 << ke;
+        int pos[4];
+        for (int a = 0; a < ke; a++)
+            for (int b = 0; b < K; b++)
+                if (t.q[b] == e.q[a]) pos[a] = K - 1 - b;
+        uint32_t emask = 0;
+        for (int a = 0; a < ke; a++) emask |= 1u << pos[a];
+        auto sub = [&](int x) {  // x's bits at e's qubits, in e's index order
+            int v = 0;
+            for (int a = 0; a < ke; a++) v |= ((x >> pos[a]) & 1) << (ke - 1 - a);
+            return v;
+        };
+        std::vector<double> m(2 * (size_t)D * D, 0.0);
+        for (int o = 0; o < D; o++)
+            for (int c = 0; c < D; c++) {
+                double re = 0, im = 0;
+                for (int x = 0; x < D; x++) {
+                    if ((o & ~emask) != (x & ~emask)) continue;  // E acts on e's qubits only
+                    const double *ev = &e.m[2 * ((size_t)sub(o) * De + sub(x))];
+                    const double *tv = &t.m[2 * ((size_t)x * D + c)];
+                    re += ev[0] * tv[0] - ev[1] * tv[1];
+                    im += ev[0] * tv[1] + ev[1] * tv[0];
+                }
+                m[2 * ((size_t)o * D + c)] = re;
+                m[2 * ((size_t)o * D + c) + 1] = im;
+            }
+        t.m.swap(m);
+        dead[i] = true;
+    }
+    size_t w = 0;
+    for (size_t i = 0; i < ops.size(); i++)
+        if (!dead[i]) {
+            if (w != i) ops[w] = std::move(ops[i]);
+            w++;
+        }
+    ops.resize(w);
+}
+
 int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std::string &err) {
     const goqu_op_t *ops = static_cast<const goqu_op_t *>(ops_v);
     out.clear();
@@ -414,6 +479,7 @@ int lower_dm_ops(const void *ops_v, int nops, int n, std::vector<EOp> &out, std:
         }
         out.push_back(make_superop(q4, 2 * k, Sm.data()));
     }
+    merge_channels(out);
     return GOQU_OK;
 }
 

@@ -279,3 +279,24 @@ def test_wide_controlled_gates(nt, nc):
     s, _ = _padded(n)
     E.apply(s, n, O.sv_marshal(c), tile_bits=10, low_bits=3)
     assert np.max(np.abs(s - O.sv_evolve(c))) < 1e-12
+
+
+def test_dm_adjacent_channels_merge():
+    """A one-qubit channel that meets a parity-block channel on its qubit with nothing in between
+    (here: amplitude damping on Delay after a gate's depolarizing channel, across gates on other
+    qubits) is folded into it: the program has the same instructions as without it."""
+    from goqu_amd import densitymatrix as D
+    from goqu_amd import noise as NZ
+    from goqu_amd.circuit import NewBuilder
+
+    def instrs(with_delay):
+        b = NewBuilder("m", 3)
+        b.RX(0.3, 0).RY(0.4, 1).CNOT(1, 2)
+        if with_delay:
+            b.Delay(0, 1.0)
+        nm = NZ.New()
+        nm.AddDefaultError(1, NZ.Depolarizing1Q(0.03))
+        nm.AddGateError("delay", NZ.AmplitudeDamping(0.1))
+        return N.plan_stats(3, D.marshal(b.Build(), nm), density=True)[1]
+
+    assert instrs(True) == instrs(False)
