@@ -306,6 +306,7 @@ static int adjoint_dm_fused(State &rho, int n, const goqu_op_t *ops, int nops, c
     for (int c = 0; c < nck && !rc; c++) {
         std::vector<EOp> seg;
         for (int k = bounds[c]; k < bounds[c + 1]; k++) seg.insert(seg.end(), low[k].begin(), low[k].end());
+        merge_channels(seg);  // the forward pass needs rho only at the segment ends
         rc = run_eops(rho, seg, s);
         if (!rc && c < nck - 1) rc = alloc_like(ck[c], rho);
         if (!rc && c < nck - 1) rc = copy_state(ck[c], rho, s);

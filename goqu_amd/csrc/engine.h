@@ -244,6 +244,9 @@ int lower_sv_ops(const void *ops /* goqu_op_t* */, int nops, int nq_user,
 
 // Lowering for a density matrix of n qubits stored as a 2n-qubit vector.
 int lower_dm_ops(const void *ops, int nops, int n, std::vector<EOp> &out, std::string &err);
+// folds each parity-block channel into the last channel on a superset of its qubits when no op in
+// between touches those qubits (lower_dm_ops applies it to every list it lowers)
+void merge_channels(std::vector<EOp> &ops);
 
 // Greedy stage planner + register allocation (no EOP_WIDE ops: run_eops splits at them).
 Program plan_program(const std::vector<EOp> &eops, int nq_internal, const PlanConfig &cfg);

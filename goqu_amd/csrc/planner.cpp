@@ -310,7 +310,7 @@ static EOp make_superop(const int *q, int k, const double *m) {
 // it (S <- E S, E embedded in the wider basis; the product of parity-keeping superoperators keeps
 // parity).  A noise model that ends each layer with a one-qubit channel (amplitude damping on
 // Delay) after a gate's own depolarizing channel then costs one parity-block op instead of two.
-static void merge_channels(std::vector<EOp> &ops) {
+void merge_channels(std::vector<EOp> &ops) {
     std::vector<bool> dead(ops.size(), false);
     for (size_t i = 0; i < ops.size(); i++) {
         EOp &e = ops[i];
