@@ -620,18 +620,34 @@ __global__ void __launch_bounds__(kQThreads) k_qaoa_pass(const QInst *__restrict
                 cmul_in<false>(P[r], t.x, t.y);
             }
         } else {
+            // The cut bytes of the next four registers are fetched while the current four are
+            // processed: loaded between the in-place register updates (inline asm, which memory
+            // operations do not cross) each load's latency stood alone, and fetching all 16 up front
+            // costs the kernel its third wave per SIMD.
             const bool undo = !(flags & Q_NOSTORE);
+            int cur[4], nxt[4];
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int c = cut[at(ix, r)];
-                cost_overlap(red[3], (double)(ne - 2 * c), L[r], P[r]);
-                if (undo) {
-                    const c2d t = tab[c];
-                    cmul_in<true>(P[r], t.x, t.y);
-                    cmul_in<true>(L[r], t.x, t.y);
+            for (int i = 0; i < 4; i++) nxt[i] = cut[at(ix, i)];
+#pragma unroll
+            for (int g = 0; g < 4; g++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) cur[i] = nxt[i];
+                if (g < 3) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) nxt[i] = cut[at(ix, 4 * (g + 1) + i)];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int r = 4 * g + i;
+                    cost_overlap(red[3], (double)(ne - 2 * cur[i]), L[r], P[r]);
+                    if (undo) {
+                        const c2d t = tab[cur[i]];
+                        cmul_in<true>(P[r], t.x, t.y);
+                        cmul_in<true>(L[r], t.x, t.y);
+                    }
                 }
                 // table reads four registers at a time (hoisted, all 16 would hold 64 VGPRs)
-                if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
         __syncthreads();  // the table and masks share LDS with the next relayout
