@@ -207,7 +207,13 @@ class Circuit:
         return self._ops[i]
 
     def IsDynamic(self) -> bool:
-        """circuit.go:298-322"""
+        """circuit.go:298-322 (the op list is fixed at construction, so the answer is kept)"""
+        d = getattr(self, "_dynamic", None)
+        if d is None:
+            d = self._dynamic = self._is_dynamic()
+        return d
+
+    def _is_dynamic(self) -> bool:
         last = -1
         for i in range(len(self._ops) - 1, -1, -1):
             g = self._ops[i].Gate

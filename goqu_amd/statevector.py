@@ -154,9 +154,8 @@ class Sim:
             return counts
         idx = self.Sample(_uniforms(shots, uniforms, rng))
         vals, cnt = np.unique(idx, return_counts=True)
-        for v, k in zip(vals, cnt):
-            counts[format_bitstring(v, self.num_qubits)] = int(k)
-        return counts
+        fmt = f"0{self.num_qubits}b"  # format_bitstring's format, over Python ints
+        return {format(v, fmt): k for v, k in zip(vals.tolist(), cnt.tolist())}
 
     def MeasureQubit(self, q: int, u: float) -> int:
         out = C.c_int32(0)
